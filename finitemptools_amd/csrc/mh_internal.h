@@ -1,0 +1,115 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mpsheff.h"
+
+namespace mh {
+
+// Per-launch timing record (profiling mode only).
+struct ProfRec {
+  hipEvent_t start, stop;
+  int kclass;
+  double flops;
+};
+
+}  // namespace mh
+
+struct mh_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  // Grow-only workspace for contraction intermediates.
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+  // Small device buffers: packed MPO mode matrices and reduction scratch.
+  void *small = nullptr;
+  size_t small_bytes = 0;
+  double *red = nullptr;       // device reduction scratch (doubles)
+  size_t red_len = 0;
+  double *red_host = nullptr;  // pinned host mirror for scalar reads
+  // Profiling.
+  int profiling = 0;
+  std::vector<mh::ProfRec> prof_pending;
+  double prof_ms[MH_K_NCLASS] = {0, 0, 0};
+  int64_t prof_n[MH_K_NCLASS] = {0, 0, 0};
+  double prof_flops[MH_K_NCLASS] = {0, 0, 0};
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace mh {
+
+int fail(mh_handle h, int code, const char *fmt, ...);
+
+#define MH_HIP(h, expr)                                                        \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess)                                                      \
+      return ::mh::fail((h), MH_ERR_HIP, "%s failed: %s (%s:%d)", #expr,        \
+                        hipGetErrorString(_e), __FILE__, __LINE__);            \
+  } while (0)
+
+#define MH_TRY(expr)                                                           \
+  do {                                                                         \
+    int _s = (expr);                                                           \
+    if (_s != MH_OK) return _s;                                                \
+  } while (0)
+
+// Ensure the workspace holds at least `bytes`; returns MH_OK or an error.
+int ensure_ws(mh_handle h, size_t bytes);
+int ensure_small(mh_handle h, size_t bytes);
+int ensure_red(mh_handle h, size_t n);
+
+// Profiling brackets around one kernel launch.
+void prof_begin(mh_handle h, int kclass, double flops, int *slot);
+void prof_end(mh_handle h, int slot);
+int check_launch(mh_handle h, const char *what);
+
+// Strided GEMM: C = alpha * op(A) B + beta * C (see mh_gemm in the header).
+struct GemmDesc {
+  int M, N, K;
+  const void *A;
+  int64_t sam, sak;
+  const void *B;
+  int64_t sbk, sbn;
+  void *C;
+  int64_t scm, scn;
+  double alpha_re, alpha_im, beta_re, beta_im;
+  int conjA;
+};
+int gemm(mh_handle h, int dtype, int prec, const GemmDesc &g);
+
+// Mode-matrix kinds used to fold MPO tensors into a (ni x nj) matrix Mt so
+// that out[p, j, q] = sum_i in[p, i, q] Mt[i, j].
+enum ModeKind {
+  MODE_SITE = 0,   // i = w + wl*s,              j = s' + d*w'       (W[w,s',w',s])
+  MODE_TWO = 1,    // i = w + wl*(s1 + d1*s2),   j = s1' + d1*(s2' + d2*w')
+  MODE_RIGHT = 2   // i = s' + d*w',             j = w + wl*s
+};
+int build_mode_matrix(mh_handle h, int dtype, int kind, int wl, int wm, int wr,
+                      int d1, int d2, const void *W1, const void *W2, void *Mt);
+int mode_product(mh_handle h, int dtype, int P, int ni, int nj, int Q,
+                 const void *in, int64_t sIq, const void *Mt, void *out,
+                 int64_t sOq);
+
+int heff_apply(mh_handle h, const mh_heff_op &op, const void *x, void *y,
+               void *mode_cache);
+size_t heff_ws_bytes(const mh_heff_op &op);
+size_t heff_vec_len(const mh_heff_op &op);
+int heff_check(mh_handle h, const mh_heff_op &op);
+
+// Vector primitives on the handle's stream (results left on device).
+int vec_dot_dev(mh_handle h, int dtype, size_t n, const void *x, const void *y,
+                double *dres);
+int vec_nrm2sq_dev(mh_handle h, int dtype, size_t n, const void *x,
+                   double *dres);
+
+inline size_t dtype_bytes(int dtype) { return dtype == MH_C64 ? 16 : 8; }
+
+}  // namespace mh

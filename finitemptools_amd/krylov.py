@@ -1,0 +1,81 @@
+"""Device-resident Krylov solvers over an ``HeffOperator``.
+
+``simple_lanczos_solver`` mirrors auxiliary/simple_lanczos.jl:83-120 (lowest
+eigenpair, three-term recurrence, kmax steps, exit when beta < btol);
+``exponentiate`` stands in for KrylovKit.exponentiate as called by
+``_exp_evolve`` (algorithms/tdvp.jl:3-6).  Krylov vectors never leave HBM.
+"""
+import ctypes
+
+from . import _lib
+from .engine import as_colmajor, colmajor_empty, dtype_code, ptr
+
+
+def simple_lanczos_solver(A, v, wh="SR", kmax=10, btol=1.0e-8, reorth=False, verbosity=0):
+    """Lowest eigenpair of the operator ``A`` (an HeffOperator) started from ``v``.
+    Returns (eigenvalue, eigenvector, info) with info = 0 when beta < btol stopped
+    the recurrence and -1 when kmax steps ran (simple_lanczos.jl:108-113)."""
+    if wh not in ("SR", "SA"):
+        raise ValueError("only the lowest eigenpair ('SR'/'SA') is supported")
+    v = A.check_vector(v)
+    out = colmajor_empty(A.shape, A.dtype, v.device)
+    e = A.engine
+    h = e.bind_stream()
+    ev = ctypes.c_double()
+    info = ctypes.c_int()
+    nit = ctypes.c_int()
+    e.check(e.lib.mh_lanczos_eig(h, ctypes.byref(A.op), ptr(v), int(kmax), float(btol),
+                                 1 if reorth else 0, ctypes.byref(ev), ptr(out),
+                                 ctypes.byref(info), ctypes.byref(nit)), "lanczos_eig")
+    if verbosity >= 2:
+        print(f"lanczos: {nit.value} steps, info {info.value}")
+    return ev.value, out, info.value
+
+
+def eig_solver(A, x, maxiter, tol, reorth=False):
+    """``_eig_solver`` (algorithms/dmrg.jl:3-6)."""
+    val, vec, _ = simple_lanczos_solver(A, x, "SR", maxiter, tol, reorth=reorth)
+    return val, vec
+
+
+def exponentiate(A, dt, x, tol=1e-8, kdim=12):
+    """exp(dt * A) x with a Lanczos space of dimension <= kdim, sub-stepped so
+    that each sub-step's Krylov error estimate is below tol * |sub-step|."""
+    x = A.check_vector(x)
+    dt = complex(dt)
+    out = colmajor_empty(A.shape, A.dtype, x.device)
+    e = A.engine
+    h = e.bind_stream()
+    info = ctypes.c_int()
+    e.check(e.lib.mh_lanczos_expm(h, ctypes.byref(A.op), ptr(x), dt.real, dt.imag, int(kdim),
+                                  float(tol), ptr(out), ctypes.byref(info)), "lanczos_expm")
+    return out, info.value
+
+
+def exp_evolve(A, dt, x, tol):
+    """``_exp_evolve(f, dt, x, isherm, tol)`` (tdvp.jl:3-6) passes tol/|dt| to
+    KrylovKit as the accuracy per unit time; the engine takes the same."""
+    y, _ = exponentiate(A, dt, x, tol=tol / abs(dt) if dt != 0 else tol)
+    return y
+
+
+def dot(x, y):
+    """<x, y>, conjugate-linear in x (Julia ``dot``)."""
+    from .engine import engine
+    x = as_colmajor(x)
+    y = as_colmajor(y)
+    e = engine(x.device)
+    h = e.bind_stream()
+    out = (ctypes.c_double * 2)()
+    e.check(e.lib.mh_dot(h, dtype_code(x), x.numel(), ptr(x), ptr(y), out), "dot")
+    return complex(out[0], out[1]) if x.is_complex() else out[0]
+
+
+def norm(x):
+    from .engine import engine
+    x = as_colmajor(x)
+    e = engine(x.device)
+    h = e.bind_stream()
+    out = ctypes.c_double()
+    e.check(e.lib.mh_nrm2(h, dtype_code(x), x.numel(), ptr(x), ctypes.byref(out)), "nrm2")
+    return out.value

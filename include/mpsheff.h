@@ -1,0 +1,179 @@
+/*
+ * mpsheff.h -- C ABI of the MI355X (gfx950) effective-Hamiltonian engine.
+ *
+ * The engine evaluates the contractions behind H_eff.x and the left/right
+ * environment updates of finite-MPS algorithms (DMRG, TDVP) on one GPU.
+ * Every tensor is a device pointer to a dense column-major array whose index
+ * order is the Julia/TensorKit order of the reference:
+ *
+ *   MPS site tensor   A[a, s, b]            extents (Dl, d, Dr)
+ *   two-site tensor   X[a, s1, s2, b]       extents (Dl, d1, d2, Dr)
+ *   bond tensor       C[a, b]               extents (Dl, Dr)
+ *   MPO site tensor   W[wl, s_out, wr, s_in] extents (wl, d, wr, d)
+ *   environment       E[a_bra, w, a_ket]    extents (D, w, D)
+ *
+ * The first index is fastest in memory.  dtype MH_C64 means interleaved
+ * (re, im) pairs of doubles (Julia ComplexF64).  Every entry point returns an
+ * mh_status; on failure mh_last_error(h) describes the cause.  Nothing aborts.
+ *
+ * Work is enqueued on the handle's stream (mh_set_stream); outputs go into
+ * caller-provided device buffers.  Intermediates live in a workspace the
+ * handle owns; it grows on first use of a larger shape and is then reused.
+ * A handle is not thread-safe: one host thread drives one handle.
+ */
+#ifndef MPSHEFF_H
+#define MPSHEFF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mh_context *mh_handle;
+
+enum mh_status {
+  MH_OK = 0,
+  MH_ERR_ARG = 1,         /* null pointer, bad enum          (Julia ArgumentError)     */
+  MH_ERR_SHAPE = 2,       /* extents inconsistent            (Julia DimensionMismatch) */
+  MH_ERR_DTYPE = 3,       /* dtype/precision not supported   (Julia ArgumentError)     */
+  MH_ERR_HIP = 4,         /* a HIP runtime call failed                                  */
+  MH_ERR_OOM = 5,         /* device allocation failed                                    */
+  MH_ERR_NOCONV = 6,      /* Krylov iteration did not reach its tolerance               */
+  MH_ERR_UNSUPPORTED = 7  /* feature not built for this configuration                  */
+};
+
+enum mh_dtype { MH_F64 = 0, MH_C64 = 1 };
+
+/* Arithmetic used for the O(D^3) contractions.  Storage stays FP64/C64.
+ *   MH_PREC_F64    : FP64 MFMA (v_mfma_f64_16x16x4f64), the plain path.
+ *   MH_PREC_F32    : operands rounded to FP32, FP32 MFMA, FP32 accumulation.
+ *   MH_PREC_BF16X3 : each operand split as hi+lo BF16 pair, three BF16 MFMA
+ *                    products (hi*hi + hi*lo + lo*hi), FP32 accumulation.      */
+enum mh_prec { MH_PREC_F64 = 0, MH_PREC_F32 = 1, MH_PREC_BF16X3 = 2 };
+
+/* ---- handle lifecycle ---------------------------------------------------- */
+int mh_create(int device, mh_handle *out);
+int mh_destroy(mh_handle h);
+/* stream: a hipStream_t, or NULL for the null stream. */
+int mh_set_stream(mh_handle h, void *stream);
+int mh_synchronize(mh_handle h);
+const char *mh_last_error(mh_handle h);
+int mh_version(void);
+
+/* ---- device memory for hosts without their own allocator ----------------- */
+int mh_malloc(mh_handle h, size_t bytes, void **ptr);
+int mh_free(mh_handle h, void *ptr);
+int mh_upload(mh_handle h, void *dst_device, const void *src_host, size_t bytes);
+int mh_download(mh_handle h, void *dst_host, const void *src_device, size_t bytes);
+
+/* ---- effective Hamiltonians (algorithms/derivatives.jl) ------------------ */
+
+/* y[a,s',b] = sum L[a,w,a'] x[a',s,b'] W[w,s',w',s] R[b,w',b']
+ * replaces ac_prime(x::MPSTensor, m::MPOTensor, hleft, hright),
+ * algorithms/derivatives.jl:2-4.  L: (Dl,wl,Dl), R: (Dr,wr,Dr), W: (wl,d,wr,d). */
+int mh_ac_prime(mh_handle h, int dtype, int prec, int Dl, int d, int Dr, int wl,
+                int wr, const void *x, const void *W, const void *L,
+                const void *R, void *y);
+
+/* y[a,s1',s2',b] = sum L[a,w,a'] x[a',s1,s2,b'] W1[w,s1',m,s1] W2[m,s2',w',s2] R[b,w',b']
+ * replaces ac2_prime(x::MPOTensor, h1, h2, hleft, hright),
+ * algorithms/derivatives.jl:20-22. */
+int mh_ac2_prime(mh_handle h, int dtype, int prec, int Dl, int d1, int d2,
+                 int Dr, int wl, int wm, int wr, const void *x, const void *W1,
+                 const void *W2, const void *L, const void *R, void *y);
+
+/* y[a,b] = sum L[a,w,a'] x[a',b'] R[b,w,b']
+ * replaces c_prime(x::MPSBondTensor, hleft, hright), algorithms/derivatives.jl:52-54. */
+int mh_c_prime(mh_handle h, int dtype, int prec, int Dl, int Dr, int w,
+               const void *x, const void *L, const void *R, void *y);
+
+/* ---- environment updates (mpo/transfer.jl) ------------------------------- */
+
+/* hnew[b,w',b'] = sum hold[a,w,a'] psiB[a',s,b'] W[w,s',w',s] conj(psiA[a,s',b])
+ * replaces updateleft(hold, psiAj, hj::MPOTensor, psiBj), mpo/transfer.jl:31-34.
+ * hold: (Dl,wl,Dl), psiA/psiB: (Dl,d,Dr), hnew: (Dr,wr,Dr). */
+int mh_updateleft(mh_handle h, int dtype, int prec, int Dl, int d, int Dr,
+                  int wl, int wr, const void *hold, const void *psiA,
+                  const void *W, const void *psiB, void *hnew);
+
+/* hnew[a,w,a'] = sum hold[b,w',b'] psiB[a',s,b'] W[w,s',w',s] conj(psiA[a,s',b])
+ * replaces updateright(hold, psiAj, hj::MPOTensor, psiBj), mpo/transfer.jl:26-29.
+ * hold: (Dr,wr,Dr), hnew: (Dl,wl,Dl). */
+int mh_updateright(mh_handle h, int dtype, int prec, int Dl, int d, int Dr,
+                   int wl, int wr, const void *hold, const void *psiA,
+                   const void *W, const void *psiB, void *hnew);
+
+/* ---- operator descriptor for the device-resident Krylov drivers ---------- */
+enum mh_heff_kind { MH_HEFF0 = 0, MH_HEFF1 = 1, MH_HEFF2 = 2 };
+
+typedef struct mh_heff_op {
+  int kind;   /* mh_heff_kind: c_prime / ac_prime / ac2_prime              */
+  int dtype;  /* mh_dtype                                                  */
+  int prec;   /* mh_prec                                                   */
+  int Dl, d1, d2, Dr; /* d1 unused for HEFF0, d2 used only by HEFF2        */
+  int wl, wm, wr;     /* wm used only by HEFF2; HEFF0 uses wl (= wr)       */
+  const void *W1, *W2, *L, *R;
+} mh_heff_op;
+
+/* number of scalar entries of the vector the operator acts on */
+int mh_heff_size(const mh_heff_op *op, size_t *n);
+/* y = H_eff x for the described operator */
+int mh_heff_apply(mh_handle h, const mh_heff_op *op, const void *x, void *y);
+
+/* Lowest eigenpair by three-term Lanczos with the Krylov basis kept on the
+ * device; only the tridiagonal coefficients cross to the host.
+ * Follows simple_lanczos_solver(A, v, "SR", kmax, btol),
+ * auxiliary/simple_lanczos.jl:83-120 (via _eig_solver, algorithms/dmrg.jl:3-6):
+ * first step :27-42, further steps :49-68, early exit when beta < btol,
+ * host eigen of the k x k tridiagonal :115, Ritz vector :70-78.
+ * reorth = 0 reproduces the reference (no re-orthogonalisation); reorth = 1
+ * adds FP64 full re-orthogonalisation (used by the mixed-precision forms).
+ * info = 0 converged (beta < btol), -1 ran kmax steps (reference semantics).
+ * niter = Krylov dimension used.  xout may alias x0.                        */
+int mh_lanczos_eig(mh_handle h, const mh_heff_op *op, const void *x0, int kmax,
+                   double btol, int reorth, double *eigval, void *xout,
+                   int *info, int *niter);
+
+/* xout = exp(dt * H_eff) x0 by a Lanczos (Hermitian H_eff) Krylov space of
+ * dimension <= kdim, repeated in sub-steps until the Krylov error estimate of
+ * each sub-step is below tol.  Replaces _exp_evolve -> KrylovKit.exponentiate,
+ * algorithms/tdvp.jl:3-6.  dt = dt_re + i dt_im; a non-real dt needs MH_C64. */
+int mh_lanczos_expm(mh_handle h, const mh_heff_op *op, const void *x0,
+                    double dt_re, double dt_im, int kdim, double tol,
+                    void *xout, int *info);
+
+/* ---- Krylov vector primitives (simple_lanczos.jl:28-38,50-63) ------------ */
+/* out[0] + i out[1] = <x, y>  (conjugate-linear in x, as Julia dot) */
+int mh_dot(mh_handle h, int dtype, size_t n, const void *x, const void *y,
+           double *out);
+int mh_nrm2(mh_handle h, int dtype, size_t n, const void *x, double *out);
+/* y = a x + b y with complex a = (a_re, a_im), b = (b_re, b_im) */
+int mh_axpby(mh_handle h, int dtype, size_t n, double a_re, double a_im,
+             const void *x, double b_re, double b_im, void *y);
+
+/* ---- plain strided GEMM used by the sweep glue --------------------------- */
+/* C[m,n] = alpha * sum_k op(A)[m,k] B[k,n] + beta * C[m,n],
+ * element (m,k) of A at A + m*sam + k*sak (op = conj if conjA), likewise for
+ * B and C.  Used for the two-site contraction A[s]*A[s+1] (dmrg.jl:98). */
+int mh_gemm(mh_handle h, int dtype, int prec, int M, int N, int K,
+            const void *A, int64_t sam, int64_t sak, int conjA,
+            const void *B, int64_t sbk, int64_t sbn, void *C, int64_t scm,
+            int64_t scn, double alpha_re, double alpha_im, double beta_re,
+            double beta_im);
+
+/* ---- measurement --------------------------------------------------------- */
+enum mh_kernel_class { MH_K_GEMM = 0, MH_K_WSTAGE = 1, MH_K_VEC = 2, MH_K_NCLASS = 3 };
+/* enable = 1: bracket every launch with HIP events on the handle's stream */
+int mh_profile_enable(mh_handle h, int enable);
+/* total device milliseconds, launch count and algorithmic flops recorded for
+ * one kernel class since the last reset; waits for the recorded events.   */
+int mh_profile_read(mh_handle h, int kclass, double *ms, int64_t *launches,
+                    double *flops);
+int mh_profile_reset(mh_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPSHEFF_H */
