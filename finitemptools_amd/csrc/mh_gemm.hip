@@ -17,6 +17,8 @@
 // FP64: v_mfma_f64_16x16x4f64.  Lane l holds A[l%16][l/16], B[l/16][l%16];
 // accumulator register r of lane l is C[l/16 + 4*r][l%16] (measured).
 // Complex FP64: four real MFMAs per 16x16x4 step on split re/im LDS planes.
+#include <type_traits>
+
 #include "mh_internal.h"
 
 namespace mh {
@@ -25,6 +27,7 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 
 struct GemmP {
   int M, N, K;
+  int kchunk;  // K extent per split (multiple of the tile depth)
   const double *A;
   int64_t sam, sak;
   const double *B;
@@ -35,222 +38,283 @@ struct GemmP {
   int conjA;
 };
 
-constexpr int BK = 16;
+// Operand staging.  One operand tile is BR rows ("r" = m for A, n for B) by
+// BK values of k.  An operand contiguous along r ("r-major": A m-contiguous,
+// B n-contiguous) is staged in LDS as [k][r] with pitch BR + 4; an operand
+// contiguous along k ("k-major") as [r][k] with pitch BK + 2.  Either way a
+// thread moves pairs of elements adjacent along the contiguous dimension
+// (one 16-byte global load when the operand is suitably aligned, one 16-byte
+// LDS store always), so the LDS stores are conflict-free for both layouts.
+//
+// MFMA K order: within each 16-deep slice, sub-step s (0..3) of lane group
+// lk = lane/16 uses k = 4*lk + s.  The four sub-steps still cover the slice
+// once, A and B agree on the order, and a k-major fragment then reads four
+// consecutive k of one row with two 16-byte LDS reads.
+template <bool CPLX, int BR, int BK, bool RMAJ>
+struct Stage {
+  static constexpr int NP = CPLX ? 2 : 1;
+  static constexpr int PITCH = RMAJ ? BR + 4 : BK + 2;
+  static constexpr int SIZE = RMAJ ? BK * PITCH : BR * PITCH;
+  static constexpr int NPAIR = BR * BK / 512;
+  static_assert(NPAIR >= 1, "tile too small for 256 threads");
+  __device__ static void coord(int t, int i, int &r, int &k) {
+    const int e = t + 256 * i;
+    if (RMAJ) {
+      r = 2 * (e % (BR / 2));
+      k = e / (BR / 2);
+    } else {
+      k = 2 * (e % (BK / 2));
+      r = e / (BK / 2);
+    }
+  }
+};
 
-// ---------------------------------------------------------------- real FP64
-template <int BM, int BN, int WGM, int WGN, bool AMC, bool BKC>
+// One kernel for real (CPLX = false) and complex FP64.  Complex operands are
+// split into re/im LDS planes at the LDS store; each 16x16x4 step then runs
+// four real MFMAs (re*re, -im*im, re*im, im*re).
+//
+// Split-K: blockIdx.z selects the K range [z*kchunk, (z+1)*kchunk).  With
+// gridDim.z == 2 the partial tiles are added with FP64 atomics onto a C that
+// was zeroed beforehand: 0 + a + b == 0 + b + a exactly, so the result does
+// not depend on the order the two adds land (only used for beta == 0).
+template <bool CPLX, int BM, int BN, int BK, bool AMC, bool BKC, bool VEC>
 __global__ void __launch_bounds__(256)
-    k_gemm_f64(const GemmP p) {
+    k_gemm(const GemmP p) {
+  constexpr int WGM = 2, WGN = 2;
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
-  constexpr int PA = BM + 16, PB = BN + 16;
-  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
-  static_assert(WGM * WGN == 4, "4 waves per block");
-  static_assert(TM >= 1 && TN >= 1, "wave tile");
-  __shared__ double sA[2][BK * PA];
-  __shared__ double sB[2][BK * PB];
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int EW = CPLX ? 2 : 1;  // doubles per element in global memory
+  using SA = Stage<CPLX, BM, BK, AMC>;
+  using SB = Stage<CPLX, BN, BK, !BKC>;
+  static_assert(TM >= 1 && TN >= 1 && BK % 16 == 0, "tile");
+  __shared__ __align__(16) double sA[NP][2][SA::SIZE];
+  __shared__ __align__(16) double sB[NP][2][SB::SIZE];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int lr = lane & 15, lk = lane >> 4;
+  const int kbeg = blockIdx.z * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+  const double csign = p.conjA ? -1.0 : 1.0;
+  // operand descriptors: r-stride / k-stride in elements
+  const int64_t a_sr = p.sam, a_sk = p.sak, b_sr = p.sbn, b_sk = p.sbk;
 
-  double ra[LA], rb[LB];
-  auto load = [&](int k0) {
+  // Per-pair row offsets (clamped into the matrix) and row validity.
+  int64_t aoff[SA::NPAIR][2], boff[SB::NPAIR][2];
+  bool aok[SA::NPAIR][2], bok[SB::NPAIR][2];
 #pragma unroll
-    for (int i = 0; i < LA; ++i) {
-      int e = tid + 256 * i;
-      int mm = AMC ? (e % BM) : (e / BK);
-      int kk = AMC ? (e / BM) : (e % BK);
-      int gm = m0 + mm, gk = k0 + kk;
-      ra[i] = (gm < p.M && gk < p.K) ? p.A[gm * p.sam + gk * p.sak] : 0.0;
+  for (int i = 0; i < SA::NPAIR; ++i) {
+    int r, k;
+    SA::coord(tid, i, r, k);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int gr = m0 + r + (AMC ? e : 0);
+      const int kk = k + (AMC ? 0 : e);
+      aok[i][e] = gr < p.M;
+      // A 16-byte pair along rows starts at most at row M-2 (M is even when
+      // VEC), so both of its elements lie inside the matrix.
+      const int rmax = (VEC && AMC && e == 0) ? p.M - 2 : p.M - 1;
+      aoff[i][e] = (int64_t)min(gr, rmax) * a_sr + (int64_t)kk * a_sk;
     }
+  }
 #pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      int e = tid + 256 * i;
-      int nn = BKC ? (e / BK) : (e % BN);
-      int kk = BKC ? (e % BK) : (e / BN);
-      int gn = n0 + nn, gk = k0 + kk;
-      rb[i] = (gn < p.N && gk < p.K) ? p.B[gk * p.sbk + gn * p.sbn] : 0.0;
+  for (int i = 0; i < SB::NPAIR; ++i) {
+    int r, k;
+    SB::coord(tid, i, r, k);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int gr = n0 + r + (BKC ? 0 : e);
+      const int kk = k + (BKC ? e : 0);
+      bok[i][e] = gr < p.N;
+      const int rmax = (VEC && !BKC && e == 0) ? p.N - 2 : p.N - 1;
+      boff[i][e] = (int64_t)min(gr, rmax) * b_sr + (int64_t)kk * b_sk;
     }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < LA; ++i) {
-      int e = tid + 256 * i;
-      int mm = AMC ? (e % BM) : (e / BK);
-      int kk = AMC ? (e / BM) : (e % BK);
-      sA[buf][kk * PA + mm] = ra[i];
-    }
-#pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      int e = tid + 256 * i;
-      int nn = BKC ? (e / BK) : (e % BN);
-      int kk = BKC ? (e % BK) : (e / BN);
-      sB[buf][kk * PB + nn] = rb[i];
-    }
-  };
-
-  d4_t acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-
-  const int nk = (p.K + BK - 1) / BK;
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
-#pragma unroll
-    for (int k4 = 0; k4 < BK; k4 += 4) {
-      double a[TM], b[TN];
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-        a[i] = sA[buf][(k4 + lk) * PA + wm * WTM + i * 16 + lr];
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-        b[j] = sB[buf][(k4 + lk) * PB + wn * WTN + j * 16 + lr];
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j],
-                                                          0, 0, 0);
-    }
-    if (kt + 1 < nk) store(buf ^ 1);
-    __syncthreads();
   }
 
-  const double alpha = p.alpha_re, beta = p.beta_re;
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int n = n0 + wn * WTN + j * 16 + lr;
-      if (n >= p.N) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * WTM + i * 16 + lk + 4 * r;
-        if (m >= p.M) continue;
-        double *c = p.C + m * p.scm + n * p.scn;
-        double v = alpha * acc[i][j][r];
-        if (beta != 0.0) v += beta * (*c);
-        *c = v;
-      }
-    }
-}
+  double ra[NP][SA::NPAIR][2], rb[NP][SB::NPAIR][2];
 
-// ------------------------------------------------------------- complex FP64
-template <int BM, int BN, int WGM, int WGN, bool AMC, bool BKC>
-__global__ void __launch_bounds__(256)
-    k_gemm_c64(const GemmP p) {
-  constexpr int WTM = BM / WGM, WTN = BN / WGN;
-  constexpr int TM = WTM / 16, TN = WTN / 16;
-  constexpr int PA = BM + 16, PB = BN + 16;
-  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
-  __shared__ double sAr[2][BK * PA], sAi[2][BK * PA];
-  __shared__ double sBr[2][BK * PB], sBi[2][BK * PB];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WGM, wn = wave / WGM;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-  const int lr = lane & 15, lk = lane >> 4;
-  const double2 *A = reinterpret_cast<const double2 *>(p.A);
-  const double2 *B = reinterpret_cast<const double2 *>(p.B);
-  const double csign = p.conjA ? -1.0 : 1.0;
-
-  double2 ra[LA], rb[LB];
+  // Global -> registers, branch-free: k is clamped into [0, K) (a pair along
+  // k into [0, K-1)), rows were clamped once above, so every address is
+  // inside the matrix; out-of-range values are zeroed at the LDS store.
   auto load = [&](int k0) {
 #pragma unroll
-    for (int i = 0; i < LA; ++i) {
-      int e = tid + 256 * i;
-      int mm = AMC ? (e % BM) : (e / BK);
-      int kk = AMC ? (e / BM) : (e % BK);
-      int gm = m0 + mm, gk = k0 + kk;
-      ra[i] = (gm < p.M && gk < p.K) ? A[gm * p.sam + gk * p.sak]
-                                     : make_double2(0.0, 0.0);
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      if (VEC) {
+        const int kc = AMC ? min(k0 + k, p.K - 1) : min(k0 + k, p.K - 2);
+        const double *src = p.A + EW * (aoff[i][0] + (int64_t)(kc - k) * a_sk);
+        if constexpr (CPLX) {
+          const double4 v = *reinterpret_cast<const double4 *>(src);
+          ra[0][i][0] = v.x; ra[1][i][0] = v.y; ra[0][i][1] = v.z; ra[1][i][1] = v.w;
+        } else {
+          const double2 v = *reinterpret_cast<const double2 *>(src);
+          ra[0][i][0] = v.x; ra[0][i][1] = v.y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int kk = k + (AMC ? 0 : e);
+          const int kc = min(k0 + kk, p.K - 1);
+          const int64_t off = aoff[i][e] + (int64_t)(kc - kk) * a_sk;
+          if constexpr (CPLX) {
+            const double2 v = reinterpret_cast<const double2 *>(p.A)[off];
+            ra[0][i][e] = v.x;
+            ra[1][i][e] = v.y;
+          } else {
+            ra[0][i][e] = p.A[off];
+          }
+        }
+      }
     }
 #pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      int e = tid + 256 * i;
-      int nn = BKC ? (e / BK) : (e % BN);
-      int kk = BKC ? (e % BK) : (e / BN);
-      int gn = n0 + nn, gk = k0 + kk;
-      rb[i] = (gn < p.N && gk < p.K) ? B[gk * p.sbk + gn * p.sbn]
-                                     : make_double2(0.0, 0.0);
-    }
-  };
-  auto store = [&](int buf) {
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      if (VEC) {
+        const int kc = BKC ? min(k0 + k, p.K - 2) : min(k0 + k, p.K - 1);
+        const double *src = p.B + EW * (boff[i][0] + (int64_t)(kc - k) * b_sk);
+        if constexpr (CPLX) {
+          const double4 v = *reinterpret_cast<const double4 *>(src);
+          rb[0][i][0] = v.x; rb[1][i][0] = v.y; rb[0][i][1] = v.z; rb[1][i][1] = v.w;
+        } else {
+          const double2 v = *reinterpret_cast<const double2 *>(src);
+          rb[0][i][0] = v.x; rb[0][i][1] = v.y;
+        }
+      } else {
 #pragma unroll
-    for (int i = 0; i < LA; ++i) {
-      int e = tid + 256 * i;
-      int mm = AMC ? (e % BM) : (e / BK);
-      int kk = AMC ? (e / BM) : (e % BK);
-      sAr[buf][kk * PA + mm] = ra[i].x;
-      sAi[buf][kk * PA + mm] = csign * ra[i].y;
-    }
-#pragma unroll
-    for (int i = 0; i < LB; ++i) {
-      int e = tid + 256 * i;
-      int nn = BKC ? (e / BK) : (e % BN);
-      int kk = BKC ? (e % BK) : (e / BN);
-      sBr[buf][kk * PB + nn] = rb[i].x;
-      sBi[buf][kk * PB + nn] = rb[i].y;
+        for (int e = 0; e < 2; ++e) {
+          const int kk = k + (BKC ? e : 0);
+          const int kc = min(k0 + kk, p.K - 1);
+          const int64_t off = boff[i][e] + (int64_t)(kc - kk) * b_sk;
+          if constexpr (CPLX) {
+            const double2 v = reinterpret_cast<const double2 *>(p.B)[off];
+            rb[0][i][e] = v.x;
+            rb[1][i][e] = v.y;
+          } else {
+            rb[0][i][e] = p.B[off];
+          }
+        }
+      }
     }
   };
 
-  d4_t accr[TM][TN], acci[TM][TN];
+  // Registers -> LDS (masked: rows outside the matrix and k outside the
+  // split's range become zero).
+  auto store = [&](int buf, int k0) {
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      const bool k0ok = k0 + k < kend, k1ok = k0 + k + (AMC ? 0 : 1) < kend;
+      const bool ok0 = aok[i][0] && k0ok, ok1 = aok[i][1] && k1ok;
+      const int o = AMC ? k * SA::PITCH + r : r * SA::PITCH + k;
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      accr[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-      acci[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < NP; ++q) {
+        const double sg = (q == 1) ? csign : 1.0;
+        double2 v;
+        v.x = ok0 ? sg * ra[q][i][0] : 0.0;
+        v.y = ok1 ? sg * ra[q][i][1] : 0.0;
+        *reinterpret_cast<double2 *>(&sA[q][buf][o]) = v;
+      }
     }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      const bool k0ok = k0 + k < kend, k1ok = k0 + k + (BKC ? 1 : 0) < kend;
+      const bool ok0 = bok[i][0] && k0ok, ok1 = bok[i][1] && k1ok;
+      const int o = BKC ? r * SB::PITCH + k : k * SB::PITCH + r;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        double2 v;
+        v.x = ok0 ? rb[q][i][0] : 0.0;
+        v.y = ok1 ? rb[q][i][1] : 0.0;
+        *reinterpret_cast<double2 *>(&sB[q][buf][o]) = v;
+      }
+    }
+  };
 
-  const int nk = (p.K + BK - 1) / BK;
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+  d4_t acc[NP][TM][TN];
 #pragma unroll
-    for (int k4 = 0; k4 < BK; k4 += 4) {
-      double ar[TM], ai[TM], nai[TM], br[TN], bi[TN];
+  for (int q = 0; q < NP; ++q)
 #pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        int o = (k4 + lk) * PA + wm * WTM + i * 16 + lr;
-        ar[i] = sAr[buf][o];
-        ai[i] = sAi[buf][o];
-        nai[i] = -ai[i];
-      }
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        int o = (k4 + lk) * PB + wn * WTN + j * 16 + lr;
-        br[j] = sBr[buf][o];
-        bi[j] = sBi[buf][o];
-      }
+      for (int j = 0; j < TN; ++j) acc[q][i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  auto compute = [&](int buf) {
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+    for (int kb = 0; kb < BK; kb += 16) {
+      double a[NP][TM][4], b[NP][TN][4];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          const int r = wm * WTM + i * 16 + lr;
+          if (AMC) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) a[q][i][s] = sA[q][buf][(kb + 4 * lk + s) * SA::PITCH + r];
+          } else {
+            const double2 *src = reinterpret_cast<const double2 *>(&sA[q][buf][r * SA::PITCH + kb + 4 * lk]);
+            const double2 u = src[0], v = src[1];
+            a[q][i][0] = u.x; a[q][i][1] = u.y; a[q][i][2] = v.x; a[q][i][3] = v.y;
+          }
+        }
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
-          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
-          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], accr[i][j], 0, 0, 0);
-          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+          const int r = wn * WTN + j * 16 + lr;
+          if (!BKC) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) b[q][j][s] = sB[q][buf][(kb + 4 * lk + s) * SB::PITCH + r];
+          } else {
+            const double2 *src = reinterpret_cast<const double2 *>(&sB[q][buf][r * SB::PITCH + kb + 4 * lk]);
+            const double2 u = src[0], v = src[1];
+            b[q][j][0] = u.x; b[q][j][1] = u.y; b[q][j][2] = v.x; b[q][j][3] = v.y;
+          }
         }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            if constexpr (CPLX) {
+              const double nai = -a[1][i][s];
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
+              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[1][j][s], acc[0][i][j], 0, 0, 0);
+              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][i][s], b[0][j][s], acc[1][i][j], 0, 0, 0);
+            } else {
+              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
+            }
+          }
     }
-    if (kt + 1 < nk) store(buf ^ 1);
+  };
+
+  const int nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nk > 0) {
+    load(kbeg);
+    store(0, kbeg);
     __syncthreads();
+    // Steady state: prefetch step kt+1 into registers, run step kt from LDS,
+    // then write the prefetched tile to the other LDS buffer.  The last
+    // compute is peeled so the loop body has no branches.
+    for (int kt = 0; kt + 1 < nk; ++kt) {
+      const int k1 = kbeg + (kt + 1) * BK;
+      load(k1);
+      compute(kt & 1);
+      store((kt + 1) & 1, k1);
+      __syncthreads();
+    }
+    compute((nk - 1) & 1);
   }
 
-  double2 *C = reinterpret_cast<double2 *>(p.C);
+  const bool split = gridDim.z > 1;
   const bool has_beta = (p.beta_re != 0.0 || p.beta_im != 0.0);
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -262,66 +326,185 @@ __global__ void __launch_bounds__(256)
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * WTM + i * 16 + lk + 4 * r;
         if (m >= p.M) continue;
-        double2 *c = C + m * p.scm + n * p.scn;
-        double xr = accr[i][j][r], xi = acci[i][j][r];
-        double vr = p.alpha_re * xr - p.alpha_im * xi;
-        double vi = p.alpha_re * xi + p.alpha_im * xr;
-        if (has_beta) {
-          double2 o = *c;
-          vr += p.beta_re * o.x - p.beta_im * o.y;
-          vi += p.beta_re * o.y + p.beta_im * o.x;
+        const int64_t off = (int64_t)m * p.scm + (int64_t)n * p.scn;
+        if constexpr (CPLX) {
+          const double xr = acc[0][i][j][r], xi = acc[1][i][j][r];
+          double vr = p.alpha_re * xr - p.alpha_im * xi;
+          double vi = p.alpha_re * xi + p.alpha_im * xr;
+          double2 *c = reinterpret_cast<double2 *>(p.C) + off;
+          if (split) {
+            atomicAdd(&c->x, vr);
+            atomicAdd(&c->y, vi);
+          } else {
+            if (has_beta) {
+              const double2 o = *c;
+              vr += p.beta_re * o.x - p.beta_im * o.y;
+              vi += p.beta_re * o.y + p.beta_im * o.x;
+            }
+            *c = make_double2(vr, vi);
+          }
+        } else {
+          double v = p.alpha_re * acc[0][i][j][r];
+          double *c = p.C + off;
+          if (split) {
+            atomicAdd(c, v);
+          } else {
+            if (has_beta) v += p.beta_re * (*c);
+            *c = v;
+          }
         }
-        *c = make_double2(vr, vi);
       }
     }
 }
 
-// ------------------------------------------------------------------ dispatch
-struct TileCfg {
-  int bm, bn;
-};
+// C[m, n] = 0 for the split-K accumulation (a kernel, not a memset, so the
+// sequence stays valid inside captured graphs).
+__global__ void k_zero_c(double *C, int M, int N, int64_t scm, int64_t scn, int cplx) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)M * N) return;
+  const int m = (int)(idx % M), n = (int)(idx / M);
+  const int64_t off = (int64_t)m * scm + (int64_t)n * scn;
+  if (cplx)
+    reinterpret_cast<double2 *>(C)[off] = make_double2(0.0, 0.0);
+  else
+    C[off] = 0.0;
+}
 
-// Pick the tile that minimises (tiles per CU, rounded up) x (tile area): the
-// kernels are MFMA-bound, so a CU's time is proportional to the C area it owns.
-static TileCfg pick_tile(int M, int N) {
-  const TileCfg cands[3] = {{64, 64}, {128, 64}, {128, 128}};
+// ------------------------------------------------------------------ dispatch
+
+struct Cand {
+  int bm, bn, bk, split;
+};
+// Candidate configurations the tuner measures (wave grid 2x2, 256 threads).
+static const Cand kCands[] = {
+    {64, 64, 16, 1}, {64, 64, 32, 1}, {128, 64, 16, 1}, {64, 128, 16, 1},
+    {128, 128, 16, 1}, {64, 64, 16, 2}, {64, 64, 32, 2}, {128, 64, 16, 2},
+};
+constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
+
+template <bool CPLX, bool AMC, bool BKC, bool VEC>
+static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
+  dim3 grid((p.M + c.bm - 1) / c.bm, (p.N + c.bn - 1) / c.bn, c.split);
+  int kc = (p.K + c.split - 1) / c.split;
+  p.kchunk = (kc + c.bk - 1) / c.bk * c.bk;
+#define MH_L(BM, BN, BK) \
+  hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC>), grid, dim3(256), 0, s, p)
+  if (c.bm == 64 && c.bn == 64 && c.bk == 16) MH_L(64, 64, 16);
+  else if (c.bm == 64 && c.bn == 64) MH_L(64, 64, 32);
+  else if (c.bm == 128 && c.bn == 64) MH_L(128, 64, 16);
+  else if (c.bm == 64 && c.bn == 128) MH_L(64, 128, 16);
+  else MH_L(128, 128, 16);
+#undef MH_L
+}
+
+template <bool CPLX, bool VEC>
+static void launch_layout(const Cand &c, const GemmP &p, hipStream_t s, bool amc, bool bkc) {
+  if (amc && bkc) launch_cfg<CPLX, true, true, VEC>(c, p, s);
+  else if (amc) launch_cfg<CPLX, true, false, VEC>(c, p, s);
+  else if (bkc) launch_cfg<CPLX, false, true, VEC>(c, p, s);
+  else launch_cfg<CPLX, false, false, VEC>(c, p, s);
+}
+
+// 16-byte (complex: 32-byte) pair loads need the pair's first element at an
+// even index of the contiguous dimension, an even extent along it, an even
+// stride along the other dimension and a 16-byte aligned base.
+static bool vec_ok(const GemmP &p, bool amc, bool bkc) {
+  auto al = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+  const bool a = al(p.A) && (amc ? (p.sam == 1 && p.M % 2 == 0 && (p.sak % 2 == 0 || p.K == 1))
+                                 : (p.sak == 1 && p.K % 2 == 0 && (p.sam % 2 == 0 || p.M == 1)));
+  const bool b = al(p.B) && (bkc ? (p.sbk == 1 && p.K % 2 == 0 && (p.sbn % 2 == 0 || p.N == 1))
+                                 : (p.sbn == 1 && p.N % 2 == 0 && (p.sbk % 2 == 0 || p.K == 1)));
+  return a && b;
+}
+
+static void launch_any(const Cand &c, const GemmP &p, hipStream_t s, bool cplx, bool amc, bool bkc) {
+  const bool vec = vec_ok(p, amc, bkc);
+  if (cplx) {
+    if (vec) launch_layout<true, true>(c, p, s, amc, bkc);
+    else launch_layout<true, false>(c, p, s, amc, bkc);
+  } else {
+    if (vec) launch_layout<false, true>(c, p, s, amc, bkc);
+    else launch_layout<false, false>(c, p, s, amc, bkc);
+  }
+}
+
+static bool cand_ok(const Cand &c, const GemmP &p, bool cplx) {
+  if (c.split > 1 && (p.beta_re != 0.0 || p.beta_im != 0.0)) return false;
+  if (c.split > 1 && p.K < 256 * c.split) return false;
+  if (cplx && c.bk == 32) return false;              // LDS: keep <= 80 KB
+  if (cplx && c.bm * c.bn > 64 * 128) return false;  // register pressure
+  return true;
+}
+
+// Heuristic when no measurement is available: minimise (tiles per CU, rounded
+// up) x (tile area) -- the kernels are MFMA-bound -- and split K in two when
+// the grid leaves CUs idle.
+static int heuristic(const GemmP &p, bool cplx) {
   const int ncu = 256;
-  TileCfg best = cands[0];
+  int best = 0;
   double best_cost = 1e300;
-  for (const auto &c : cands) {
-    int64_t tiles = (int64_t)((M + c.bm - 1) / c.bm) * ((N + c.bn - 1) / c.bn);
+  for (int i = 0; i < kNCand; ++i) {
+    const Cand &c = kCands[i];
+    if (!cand_ok(c, p, cplx) || c.bk != 16) continue;
+    int64_t tiles = (int64_t)((p.M + c.bm - 1) / c.bm) * ((p.N + c.bn - 1) / c.bn) * c.split;
     double rounds = (double)((tiles + ncu - 1) / ncu);
-    // Smaller tiles move more bytes per flop; weigh them slightly.
+    double area = (double)c.bm * c.bn / c.split;
     double penalty = (c.bm * c.bn == 64 * 64) ? 1.08 : (c.bm * c.bn == 128 * 64 ? 1.03 : 1.0);
-    double cost = rounds * c.bm * c.bn * penalty;
+    if (c.split > 1) penalty *= 1.05;
+    double cost = rounds * area * penalty;
     if (cost < best_cost) {
       best_cost = cost;
-      best = c;
+      best = i;
     }
   }
   return best;
 }
 
-template <template <int, int, int, int, bool, bool> class K, bool AMC, bool BKC>
-struct Launcher;
-
-template <int BM, int BN, bool AMC, bool BKC>
-static void launch_f64(const GemmP &p, dim3 grid, hipStream_t s, bool cplx) {
-  if (cplx)
-    hipLaunchKernelGGL((k_gemm_c64<BM, BN, 2, 2, AMC, BKC>), grid, dim3(256), 0, s, p);
-  else
-    hipLaunchKernelGGL((k_gemm_f64<BM, BN, 2, 2, AMC, BKC>), grid, dim3(256), 0, s, p);
+static int run(mh_handle h, const Cand &c, const GemmP &p, bool cplx, bool amc, bool bkc) {
+  if (c.split > 1) {
+    int64_t n = (int64_t)p.M * p.N;
+    hipLaunchKernelGGL(k_zero_c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                       p.C, p.M, p.N, p.scm, p.scn, cplx ? 1 : 0);
+  }
+  launch_any(c, p, h->stream, cplx, amc, bkc);
+  return check_launch(h, "gemm");
 }
 
-template <bool AMC, bool BKC>
-static void launch_tile(const TileCfg &t, const GemmP &p, hipStream_t s, bool cplx) {
-  dim3 grid((p.M + t.bm - 1) / t.bm, (p.N + t.bn - 1) / t.bn);
-  if (t.bm == 64 && t.bn == 64)
-    launch_f64<64, 64, AMC, BKC>(p, grid, s, cplx);
-  else if (t.bm == 128 && t.bn == 64)
-    launch_f64<128, 64, AMC, BKC>(p, grid, s, cplx);
-  else
-    launch_f64<128, 128, AMC, BKC>(p, grid, s, cplx);
+// Measure every admissible candidate on this shape (the output is recomputed
+// each time, so C ends up holding a correct result) and remember the fastest.
+static int tune(mh_handle h, const GemmP &p, bool cplx, bool amc, bool bkc, int *choice) {
+  hipEvent_t e0, e1;
+  MH_HIP(h, hipEventCreate(&e0));
+  MH_HIP(h, hipEventCreate(&e1));
+  float best = 1e30f;
+  int bi = heuristic(p, cplx);
+  for (int i = 0; i < kNCand; ++i) {
+    const Cand &c = kCands[i];
+    if (!cand_ok(c, p, cplx)) continue;
+    MH_TRY(run(h, c, p, cplx, amc, bkc));  // warm-up
+    MH_HIP(h, hipEventRecord(e0, h->stream));
+    for (int r = 0; r < 3; ++r) MH_TRY(run(h, c, p, cplx, amc, bkc));
+    MH_HIP(h, hipEventRecord(e1, h->stream));
+    MH_HIP(h, hipEventSynchronize(e1));
+    float ms = 0.f;
+    MH_HIP(h, hipEventElapsedTime(&ms, e0, e1));
+    ms /= 3.f;
+    if (h->tune_log)
+      fprintf(stderr, "[mh tune] %s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d: %.1f us (%.2f TFLOP/s)\n",
+              cplx ? "c64" : "f64", p.M, p.N, p.K, (int)amc, (int)bkc, c.bm, c.bn, c.bk, c.split,
+              ms * 1e3, 2.0 * p.M * (double)p.N * p.K * (cplx ? 4 : 1) / (ms * 1e-3) / 1e12);
+    if (ms < best) {
+      best = ms;
+      bi = i;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // The last measured run left C correct; re-run the winner so the final
+  // state does not depend on which candidate happened to be measured last.
+  MH_TRY(run(h, kCands[bi], p, cplx, amc, bkc));
+  *choice = bi;
+  return MH_OK;
 }
 
 int gemm_mixed(mh_handle h, int dtype, int prec, const GemmDesc &g);
@@ -333,6 +516,7 @@ int gemm(mh_handle h, int dtype, int prec, const GemmDesc &g) {
   p.M = g.M;
   p.N = g.N;
   p.K = g.K;
+  p.kchunk = g.K;
   p.A = (const double *)g.A;
   p.sam = g.sam;
   p.sak = g.sak;
@@ -350,20 +534,32 @@ int gemm(mh_handle h, int dtype, int prec, const GemmDesc &g) {
   const bool cplx = dtype == MH_C64;
   const bool amc = (g.sam == 1) || (g.M == 1);
   const bool bkc = (g.sbk == 1) || (g.K == 1);
-  TileCfg t = pick_tile(g.M, g.N);
-  double flops = 2.0 * g.M * (double)g.N * g.K * (cplx ? 4.0 : 1.0);
+  const double flops = 2.0 * g.M * (double)g.N * g.K * (cplx ? 4.0 : 1.0);
+
+  char key[160];
+  snprintf(key, sizeof(key), "f%d:%d:%d:%d:%d%d:%d", cplx ? 1 : 0, g.M, g.N, g.K, (int)amc,
+           (int)bkc, (g.beta_re != 0.0 || g.beta_im != 0.0) ? 1 : 0);
+  int choice = -1;
+  auto it = h->tune.find(key);
+  if (it != h->tune.end()) {
+    choice = it->second;
+  } else {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(h->stream, &cs);
+    // Tune shapes worth more than ~50 us of work; small ones use the heuristic.
+    if (h->autotune && cs == hipStreamCaptureStatusNone && flops > 2e9 && !h->profiling) {
+      MH_TRY(tune(h, p, cplx, amc, bkc, &choice));
+      h->tune[key] = choice;
+      return MH_OK;
+    }
+    choice = heuristic(p, cplx);
+    h->tune[key] = choice;
+  }
   int slot;
   prof_begin(h, MH_K_GEMM, flops, &slot);
-  if (amc && bkc)
-    launch_tile<true, true>(t, p, h->stream, cplx);
-  else if (amc && !bkc)
-    launch_tile<true, false>(t, p, h->stream, cplx);
-  else if (!amc && bkc)
-    launch_tile<false, true>(t, p, h->stream, cplx);
-  else
-    launch_tile<false, false>(t, p, h->stream, cplx);
+  int st = run(h, kCands[choice], p, cplx, amc, bkc);
   prof_end(h, slot);
-  return check_launch(h, "gemm");
+  return st;
 }
 
 }  // namespace mh

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/mpsheff.h"
@@ -41,6 +42,10 @@ struct mh_context {
   int64_t prof_n[MH_K_NCLASS] = {0, 0, 0};
   double prof_flops[MH_K_NCLASS] = {0, 0, 0};
   std::vector<hipEvent_t> event_pool;
+  // GEMM autotuner: shape key -> index of the fastest measured configuration.
+  std::unordered_map<std::string, int> tune;
+  int autotune = 1;
+  int tune_log = 0;
 };
 
 namespace mh {

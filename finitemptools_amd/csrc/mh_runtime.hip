@@ -1,5 +1,6 @@
 // Handle lifecycle, error reporting, workspace and launch profiling.
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 
 #include "mh_internal.h"
@@ -128,6 +129,9 @@ int mh_create(int device, mh_handle *out) {
   mh_handle h = new mh_context();
   h->device = device;
   h->stream = nullptr;
+  // MH_AUTOTUNE=0 disables per-shape GEMM tuning; MH_TUNE_LOG=1 prints it.
+  if (const char *s = getenv("MH_AUTOTUNE")) h->autotune = atoi(s) != 0;
+  if (const char *s = getenv("MH_TUNE_LOG")) h->tune_log = atoi(s) != 0;
   *out = h;
   return MH_OK;
 }

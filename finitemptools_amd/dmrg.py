@@ -1,0 +1,133 @@
+"""Two-site DMRG on the device: algorithms/dmrg.jl:79-144, the sweep
+dispatcher of dmrgexcited.jl:219-223, compute! (dmrg.jl:221-234) and
+ground_state! (:240).
+
+Per bond: two-site tensor (MFMA GEMM), lowest eigenpair of ac2_prime by the
+device-resident Lanczos (kmax = eig_maxiter, btol = eig_tol), FP64 truncated
+SVD, normalised singular values, split, overlap error, environment update.
+``prec`` selects the arithmetic of the H_eff and environment contractions;
+the mixed forms add FP64 re-orthogonalisation to the Lanczos recurrence.
+"""
+import time
+
+import numpy as np
+import torch
+
+from .derivatives import HeffOperator
+from .envs import environments
+from .krylov import simple_lanczos_solver
+from .linalg import as_matrix, cm_view, gemm, two_site
+from .truncation import MPSTruncation, stable_tsvd
+
+MAXITER = 100   # FiniteMPTools.jl:54 (Defaults.maxiter)
+TOL = 1e-12     # Defaults.tol
+EIG_TOL = 1e-12  # Defaults.dmrg_eig_tol
+
+
+class DMRG2:
+    def __init__(self, maxiter=MAXITER, tol=TOL, eig_maxiter=10, eig_tol=EIG_TOL,
+                 verbosity=0, trunc=None, prec="f64", reorth=None):
+        self.maxiter, self.tol = maxiter, tol
+        self.eig_maxiter, self.eig_tol = eig_maxiter, eig_tol
+        self.verbosity = verbosity
+        self.trunc = trunc or MPSTruncation()
+        self.prec = prec
+        self.reorth = (prec != "f64") if reorth is None else reorth
+        self.stats = {"matvec_flops": 0.0, "lanczos_s": 0.0, "svd_s": 0.0, "env_s": 0.0, "bonds": 0}
+
+
+def _scale_rows(v, s):
+    """s[k] * v[k, s2, b]"""
+    return v * s.to(v.dtype).reshape(-1, 1, 1)
+
+
+def _scale_cols(u, s):
+    """u[a, s1, k] * s[k]"""
+    return u * s.to(u.dtype).reshape(1, 1, -1)
+
+
+def _bond_step(m, alg, site):
+    mpo, mps, hs = m.mpo, m.mps, m.hstorage
+    two = two_site(mps[site], mps[site + 1])
+    op = HeffOperator(2, hs[site], hs[site + 2], mpo[site], mpo[site + 1], prec=alg.prec)
+    t0 = time.perf_counter()
+    ev, vecs, info = simple_lanczos_solver(op, two, "SR", alg.eig_maxiter, alg.eig_tol,
+                                           reorth=alg.reorth)
+    t1 = time.perf_counter()
+    alg.stats["lanczos_s"] += t1 - t0
+    alg.stats["matvec_flops"] += op.flops() * alg.eig_maxiter
+    u, s, v, err = stable_tsvd(vecs, alg.trunc)
+    s = s / torch.linalg.vector_norm(s)
+    alg.stats["svd_s"] += time.perf_counter() - t1
+    alg.stats["bonds"] += 1
+    return two, ev, u, s, v
+
+
+def _overlap_error(two, u, s, v):
+    """err_1 = <u s v, twositemps> (dmrg.jl:109-110); returns |1 - |err_1||."""
+    usv = gemm(as_matrix(_scale_cols(u, s), 2), as_matrix(v, 1))
+    e = torch.vdot(usv.t().reshape(-1), as_matrix(two, 2).t().reshape(-1)).item()
+    return abs(1.0 - abs(e))
+
+
+def leftsweep(m, alg):
+    """dmrg.jl:89-115"""
+    mps = m.mps
+    energies, delta = [], 0.0
+    for site in range(len(mps) - 2):
+        two, ev, u, s, v = _bond_step(m, alg, site)
+        energies.append(ev)
+        mps[site] = u
+        mps[site + 1] = _scale_rows(v, s)
+        delta = max(delta, _overlap_error(two, u, s, v))
+        t0 = time.perf_counter()
+        m.updateleft(site)
+        alg.stats["env_s"] += time.perf_counter() - t0
+        if alg.verbosity > 2:
+            print(f"bond {site}: E = {ev}")
+    return energies, delta
+
+
+def rightsweep(m, alg):
+    """dmrg.jl:117-144"""
+    mps = m.mps
+    energies, delta = [], 0.0
+    for site in range(len(mps) - 2, -1, -1):
+        two, ev, u, s, v = _bond_step(m, alg, site)
+        energies.append(ev)
+        mps[site] = _scale_cols(u, s)
+        mps[site + 1] = v
+        mps.s[site + 1] = s
+        delta = max(delta, _overlap_error(two, u, s, v))
+        t0 = time.perf_counter()
+        m.updateright(site + 1)
+        alg.stats["env_s"] += time.perf_counter() - t0
+        if alg.verbosity > 2:
+            print(f"bond {site}: E = {ev}")
+    return energies, delta
+
+
+def sweep(m, alg):
+    """leftsweep! then rightsweep! (dmrgexcited.jl:219-223)"""
+    e1, d1 = leftsweep(m, alg)
+    e2, d2 = rightsweep(m, alg)
+    return e1 + e2, max(d1, d2)
+
+
+def compute(m, alg, nsweeps=None):
+    """compute! (dmrg.jl:221-234); ``nsweeps`` caps the number of sweeps."""
+    all_e, it, delta = [], 0, 2 * alg.tol
+    maxit = alg.maxiter if nsweeps is None else nsweeps
+    while it < maxit and delta > alg.tol:
+        e, delta = sweep(m, alg)
+        all_e += e
+        it += 1
+        if alg.verbosity > 2:
+            print(f"finish the {it}-th sweep with error {delta}")
+    return all_e, delta
+
+
+def ground_state(state, h, alg=None, nsweeps=None):
+    """ground_state!(state, h, alg) (dmrg.jl:240); returns (energies, delta)."""
+    alg = alg or DMRG2()
+    return compute(environments(h, state, alg.prec), alg, nsweeps)

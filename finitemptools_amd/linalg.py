@@ -1,0 +1,74 @@
+"""Dense tensor glue for the sweeps: products through the engine's MFMA GEMM,
+and the FP64 truncated SVD / QR of the sweep step.
+
+The SVD and QR go through torch.linalg on the device (rocSOLVER); they stay in
+FP64 in every precision mode (auxiliary/stable_svd.jl, mps/orth.jl).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .engine import as_colmajor, colmajor_empty, dtype_code, engine, ptr
+
+
+def gemm(A, B, alpha=1.0, conjA=False, out=None, beta=0.0):
+    """C = alpha * op(A) @ B (+ beta C) for 2-D device tensors of any strides
+    with one unit stride each, through mh_gemm."""
+    M, K = A.shape
+    K2, N = B.shape
+    if K != K2:
+        raise _lib.DimensionMismatch(2, f"gemm: {tuple(A.shape)} x {tuple(B.shape)}")
+    if A.stride(0) != 1 and A.stride(1) != 1:
+        A = as_colmajor(A)
+    if B.stride(0) != 1 and B.stride(1) != 1:
+        B = as_colmajor(B)
+    C = out if out is not None else colmajor_empty((M, N), A.dtype, A.device)
+    e = engine(A.device)
+    h = e.bind_stream()
+    a = complex(alpha)
+    b = complex(beta)
+    e.check(e.lib.mh_gemm(h, dtype_code(A), 0, M, N, K, ptr(A), A.stride(0), A.stride(1),
+                          1 if conjA else 0, ptr(B), B.stride(0), B.stride(1), ptr(C),
+                          C.stride(0), C.stride(1), a.real, a.imag, b.real, b.imag), "gemm")
+    return C
+
+
+def as_matrix(t, nrow_legs):
+    """Column-major tensor -> the (prod first legs) x (prod remaining legs)
+    matrix view (TensorMap codomain/domain split)."""
+    t = as_colmajor(t)
+    m = int(np.prod(t.shape[:nrow_legs]))
+    n = int(np.prod(t.shape[nrow_legs:]))
+    return t.permute(*reversed(range(t.dim()))).reshape(n, m).t()
+
+
+def from_matrix(M, shape):
+    """(m x n) matrix (any layout) -> column-major tensor of the given legs."""
+    Mc = M.t().contiguous()  # C-order (n, m) == column-major (m, n)
+    return Mc.reshape(tuple(reversed(shape))).permute(*reversed(range(len(shape))))
+
+
+def cm_view(C, shape):
+    """A column-major matrix (strides (1, M)) seen as a column-major tensor
+    of the given legs (no copy)."""
+    if not (C.stride(0) == 1 and (C.shape[1] == 1 or C.stride(1) == C.shape[0])):
+        C = as_colmajor(C)
+    return C.t().reshape(tuple(reversed(shape))).permute(*reversed(range(len(shape))))
+
+
+def two_site(A1, A2):
+    """x[a, s1, s2, b] = A1[a, s1, k] A2[k, s2, b] (dmrg.jl:98)."""
+    Dl, d1, _ = A1.shape
+    _, d2, Dr = A2.shape
+    return cm_view(gemm(as_matrix(A1, 2), as_matrix(A2, 1)), (Dl, d1, d2, Dr))
+
+
+def svd(M):
+    """Thin SVD of a device matrix in FP64 (torch.linalg -> rocSOLVER)."""
+    return torch.linalg.svd(M, full_matrices=False)
+
+
+def qr(M):
+    return torch.linalg.qr(M, mode="reduced")

@@ -1,0 +1,52 @@
+"""Truncation scheme and truncated SVD of the sweep step.
+
+``MPSTruncation`` restates auxiliary/truncation.jl:8-47: drop singular values
+at or below eps * ||s||_2, then keep at most D; the error is the 2-norm of the
+dropped values.  ``stable_tsvd`` is auxiliary/stable_svd.jl:12-18 on a
+two-site tensor split as (a, s1) | (s2, b).
+"""
+import numpy as np
+import torch
+
+from .linalg import as_matrix, from_matrix
+
+DEFAULT_SVD_CUTOFF = 1.0e-8   # truncation.jl:49
+DEFAULT_NONSYMMETRIC_D = 100  # truncation.jl:51
+
+
+class MPSTruncation:
+    def __init__(self, D=DEFAULT_NONSYMMETRIC_D, eps=DEFAULT_SVD_CUTOFF, verbosity=0):
+        self.D, self.eps, self.verbosity = int(D), float(eps), verbosity
+
+    def truncate(self, s):
+        """s: descending host array -> (kept count, 2-norm error)."""
+        n = float(np.linalg.norm(s))
+        k1 = int(np.count_nonzero(s > self.eps * n))
+        err1 = float(np.linalg.norm(s[k1:]))
+        if k1 <= self.D:
+            return k1, err1
+        err2 = float(np.linalg.norm(s[self.D:k1]))
+        if self.verbosity > 0:
+            print(f"sum: {len(s)} -> {k1} -> {self.D}, maximum {self.D}, truncation error: "
+                  f"abs={np.hypot(err1, err2)}, rel={np.hypot(err1, err2) / n}.")
+        return self.D, float(np.hypot(err1, err2))
+
+
+def default_truncation():
+    return MPSTruncation(DEFAULT_NONSYMMETRIC_D, DEFAULT_SVD_CUTOFF)
+
+
+def stable_tsvd(x, trunc):
+    """x[a, s1, s2, b] -> u[a, s1, k], s[k] (host + device), v[k, s2, b], err."""
+    Dl, d1, d2, Dr = x.shape
+    M = as_matrix(x, 2)
+    try:
+        U, S, Vh = torch.linalg.svd(M, full_matrices=False)
+    except RuntimeError:
+        U, S, Vh = torch.linalg.svd(M, full_matrices=False, driver="gesvd")
+    s_host = S.detach().cpu().numpy()
+    k, err = trunc.truncate(s_host)
+    k = max(k, 1)
+    u = from_matrix(U[:, :k], (Dl, d1, k))
+    v = from_matrix(Vh[:k], (k, d2, Dr))
+    return u, S[:k], v, err

@@ -1,0 +1,30 @@
+"""Host-side logic of the engine package (no GPU): seeded initial states,
+truncation scheme and MPO shapes agree with the oracle."""
+import numpy as np
+
+from finitemptools_amd import models
+from finitemptools_amd.mps import random_site_tensors, virtual_dims
+from finitemptools_amd.truncation import MPSTruncation
+from oracle import dmrg as OD
+
+
+def test_random_mps_matches_oracle_stream():
+    a = random_site_tensors(9, 2, 8, seed=3)
+    b = OD.randommps(9, 2, 8, seed=3)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert virtual_dims(9, 2, 8) == OD.virtual_dims(9, 2, 8)
+
+
+def test_truncation_matches_oracle():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        s = np.sort(np.abs(rng.standard_normal(30)) ** 6)[::-1]
+        for D, eps in [(5, 1e-8), (40, 1e-3), (10, 1e-12)]:
+            assert MPSTruncation(D, eps).truncate(s) == OD.MPSTruncation(D, eps).truncate(s)
+
+
+def test_mpo_strict_boundaries():
+    for mpo in (models.heisenberg_xxz_mpo(5), models.hubbard_mpo(4), models.j1j2_mpo(6, 1.0, 0.3)):
+        assert mpo[0].shape[0] == 1 and mpo[-1].shape[2] == 1
+        for a, b in zip(mpo[:-1], mpo[1:]):
+            assert a.shape[2] == b.shape[0]
