@@ -27,6 +27,15 @@ HIPCC_FLAGS = [
 ]
 
 
+# MFMA kernels keep their accumulators in VGPRs: with the default heuristic the
+# compiler may pick AGPRs and then copy every accumulator AGPR<->VGPR on each
+# K-loop iteration.
+EXTRA_FLAGS = {
+    "mh_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+    "mh_gemm_mixed.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
+
+
 def lib_path():
     return os.path.join(HERE, LIB_NAME)
 
@@ -41,7 +50,7 @@ def build(verbose=False, jobs=None):
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [hipcc] + [f for f in HIPCC_FLAGS if f != "-shared"] + [
+        cmd = [hipcc] + [f for f in HIPCC_FLAGS if f != "-shared"] + EXTRA_FLAGS.get(src, []) + [
             "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))

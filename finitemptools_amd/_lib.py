@@ -74,6 +74,7 @@ SIGNATURES = {
     "mh_profile_enable": (c_int, [c_void_p, c_int]),
     "mh_profile_read": (c_int, [c_void_p, c_int, P(c_double), P(c_int64), P(c_double)]),
     "mh_profile_reset": (c_int, [c_void_p]),
+    "mh_probe_mfma_f64": (c_int, [c_void_p, P(c_double)]),
 }
 
 _LIB = None

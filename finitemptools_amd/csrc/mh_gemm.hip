@@ -28,6 +28,13 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 struct GemmP {
   int M, N, K;
   int kchunk;  // K extent per split (multiple of the tile depth)
+  // XCD-aware tile order (group > 0): the 1-D grid holds 8 * xcd_chunk
+  // blocks; blocks are dispatched round-robin over the 8 XCDs, so XCD x
+  // receives ids x, x+8, ... and is given the contiguous range
+  // [x*xcd_chunk, (x+1)*xcd_chunk) of a grouped tile order (group tile-rows
+  // swept column by column), i.e. a compact region of C whose operands its
+  // own L2 can hold.
+  int tiles_m, tiles_n, xcd_chunk, group;
   const double *A;
   int64_t sam, sak;
   const double *B;
@@ -94,7 +101,20 @@ __global__ void __launch_bounds__(256)
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  int tm = blockIdx.x, tn = blockIdx.y;
+  if (p.group > 0) {
+    const int id = blockIdx.x;
+    const int local = id >> 3;
+    const int t = (id & 7) * p.xcd_chunk + local;
+    if (local >= p.xcd_chunk || t >= p.tiles_m * p.tiles_n) return;  // whole block
+    const int span = p.group * p.tiles_n;
+    const int first = (t / span) * p.group;
+    const int gsz = min(p.tiles_m - first, p.group);
+    const int w = t % span;
+    tm = first + w % gsz;
+    tn = w / gsz;
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
   const int lr = lane & 15, lk = lane >> 4;
   const int kbeg = blockIdx.z * p.kchunk;
   const int kend = min(p.K, kbeg + p.kchunk);
@@ -373,18 +393,25 @@ __global__ void k_zero_c(double *C, int M, int N, int64_t scm, int64_t scn, int 
 // ------------------------------------------------------------------ dispatch
 
 struct Cand {
-  int bm, bn, bk, split;
+  int bm, bn, bk, split, group;
 };
 // Candidate configurations the tuner measures (wave grid 2x2, 256 threads).
 static const Cand kCands[] = {
-    {64, 64, 16, 1}, {64, 64, 32, 1}, {128, 64, 16, 1}, {64, 128, 16, 1},
-    {128, 128, 16, 1}, {64, 64, 16, 2}, {64, 64, 32, 2}, {128, 64, 16, 2},
+    {64, 64, 16, 1, 0}, {64, 64, 32, 1, 0}, {128, 64, 16, 1, 0}, {64, 128, 16, 1, 0},
+    {128, 128, 16, 1, 0}, {64, 64, 16, 2, 0}, {64, 64, 32, 2, 0}, {128, 64, 16, 2, 0},
+    {64, 64, 16, 1, 8}, {64, 64, 32, 1, 8}, {64, 64, 16, 2, 8}, {64, 64, 32, 2, 8},
+    {128, 64, 16, 1, 8}, {64, 128, 16, 1, 8},
 };
 constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
 
 template <bool CPLX, bool AMC, bool BKC, bool VEC>
 static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
-  dim3 grid((p.M + c.bm - 1) / c.bm, (p.N + c.bn - 1) / c.bn, c.split);
+  p.tiles_m = (p.M + c.bm - 1) / c.bm;
+  p.tiles_n = (p.N + c.bn - 1) / c.bn;
+  p.group = c.group;
+  p.xcd_chunk = (p.tiles_m * p.tiles_n + 7) / 8;
+  dim3 grid(p.tiles_m, p.tiles_n, c.split);
+  if (c.group > 0) grid = dim3(8 * p.xcd_chunk, 1, c.split);
   int kc = (p.K + c.split - 1) / c.split;
   p.kchunk = (kc + c.bk - 1) / c.bk * c.bk;
 #define MH_L(BM, BN, BK) \
@@ -393,7 +420,7 @@ static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
   else if (c.bm == 64 && c.bn == 64) MH_L(64, 64, 32);
   else if (c.bm == 128 && c.bn == 64) MH_L(128, 64, 16);
   else if (c.bm == 64 && c.bn == 128) MH_L(64, 128, 16);
-  else MH_L(128, 128, 16);
+  else if constexpr (!CPLX) MH_L(128, 128, 16);  // complex 128x128 is never admitted
 #undef MH_L
 }
 
@@ -445,7 +472,7 @@ static int heuristic(const GemmP &p, bool cplx) {
   double best_cost = 1e300;
   for (int i = 0; i < kNCand; ++i) {
     const Cand &c = kCands[i];
-    if (!cand_ok(c, p, cplx) || c.bk != 16) continue;
+    if (!cand_ok(c, p, cplx) || c.bk != 16 || c.group != 0) continue;
     int64_t tiles = (int64_t)((p.M + c.bm - 1) / c.bm) * ((p.N + c.bn - 1) / c.bn) * c.split;
     double rounds = (double)((tiles + ncu - 1) / ncu);
     double area = (double)c.bm * c.bn / c.split;
@@ -490,8 +517,8 @@ static int tune(mh_handle h, const GemmP &p, bool cplx, bool amc, bool bkc, int 
     MH_HIP(h, hipEventElapsedTime(&ms, e0, e1));
     ms /= 3.f;
     if (h->tune_log)
-      fprintf(stderr, "[mh tune] %s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d: %.1f us (%.2f TFLOP/s)\n",
-              cplx ? "c64" : "f64", p.M, p.N, p.K, (int)amc, (int)bkc, c.bm, c.bn, c.bk, c.split,
+      fprintf(stderr, "[mh tune] %s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d: %.1f us (%.2f TFLOP/s)\n",
+              cplx ? "c64" : "f64", p.M, p.N, p.K, (int)amc, (int)bkc, c.bm, c.bn, c.bk, c.split, c.group,
               ms * 1e3, 2.0 * p.M * (double)p.N * p.K * (cplx ? 4 : 1) / (ms * 1e-3) / 1e12);
     if (ms < best) {
       best = ms;
@@ -506,6 +533,50 @@ static int tune(mh_handle h, const GemmP &p, bool cplx, bool amc, bool bkc, int 
   *choice = bi;
   return MH_OK;
 }
+
+// Register-only FP64 MFMA throughput probe: four independent accumulator
+// chains per wave, no memory traffic in the loop.  Measures the matrix-core
+// rate this device actually sustains.
+__global__ void __launch_bounds__(256) k_mfma_probe(double *out, int iters) {
+  d4_t acc0 = {0, 0, 0, 0}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
+  const double a = 1.0 + 1e-6 * threadIdx.x, b = 1.0 - 1e-6 * threadIdx.x;
+  for (int i = 0; i < iters; ++i) {
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, a, acc2, 0, 0, 0);
+    acc3 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, b, acc3, 0, 0, 0);
+  }
+  const d4_t s = acc0 + acc1 + acc2 + acc3;
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s[0] + s[1] + s[2] + s[3];
+}
+
+}  // namespace mh
+
+extern "C" int mh_probe_mfma_f64(mh_handle h, double *tflops) {
+  using namespace mh;
+  if (!h || !tflops) return MH_ERR_ARG;
+  int ncu = 0;
+  MH_HIP(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+  const int blocks = ncu * 4, iters = 4000;
+  MH_TRY(ensure_red(h, (size_t)blocks * 256));
+  hipEvent_t e0, e1;
+  MH_HIP(h, hipEventCreate(&e0));
+  MH_HIP(h, hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_mfma_probe, dim3(blocks), dim3(256), 0, h->stream, h->red, iters);
+  MH_HIP(h, hipEventRecord(e0, h->stream));
+  hipLaunchKernelGGL(k_mfma_probe, dim3(blocks), dim3(256), 0, h->stream, h->red, iters);
+  MH_HIP(h, hipEventRecord(e1, h->stream));
+  MH_HIP(h, hipEventSynchronize(e1));
+  float ms = 0.f;
+  MH_HIP(h, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const double flops = (double)blocks * 4 /*waves*/ * iters * 4 /*chains*/ * 2.0 * 16 * 16 * 4;
+  *tflops = flops / (ms * 1e-3) / 1e12;
+  return check_launch(h, "mfma probe");
+}
+
+namespace mh {
 
 int gemm_mixed(mh_handle h, int dtype, int prec, const GemmDesc &g);
 

@@ -172,6 +172,9 @@ int mh_profile_enable(mh_handle h, int enable);
 int mh_profile_read(mh_handle h, int kclass, double *ms, int64_t *launches,
                     double *flops);
 int mh_profile_reset(mh_handle h);
+/* FP64 matrix-core throughput this device sustains on a register-only MFMA
+ * loop (no memory traffic), in TFLOP/s: the measured roofline ceiling. */
+int mh_probe_mfma_f64(mh_handle h, double *tflops);
 
 #ifdef __cplusplus
 }
