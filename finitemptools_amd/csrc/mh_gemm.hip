@@ -19,31 +19,11 @@
 // Complex FP64: four real MFMAs per 16x16x4 step on split re/im LDS planes.
 #include <type_traits>
 
-#include "mh_internal.h"
+#include "mh_gemm_common.h"
 
 namespace mh {
 
-typedef double d4_t __attribute__((ext_vector_type(4)));
 
-struct GemmP {
-  int M, N, K;
-  int kchunk;  // K extent per split (multiple of the tile depth)
-  // XCD-aware tile order (group > 0): the 1-D grid holds 8 * xcd_chunk
-  // blocks; blocks are dispatched round-robin over the 8 XCDs, so XCD x
-  // receives ids x, x+8, ... and is given the contiguous range
-  // [x*xcd_chunk, (x+1)*xcd_chunk) of a grouped tile order (group tile-rows
-  // swept column by column), i.e. a compact region of C whose operands its
-  // own L2 can hold.
-  int tiles_m, tiles_n, xcd_chunk, group;
-  const double *A;
-  int64_t sam, sak;
-  const double *B;
-  int64_t sbk, sbn;
-  double *C;
-  int64_t scm, scn;
-  double alpha_re, alpha_im, beta_re, beta_im;
-  int conjA;
-};
 
 // Operand staging.  One operand tile is BR rows ("r" = m for A, n for B) by
 // BK values of k.  An operand contiguous along r ("r-major": A m-contiguous,
@@ -101,19 +81,8 @@ __global__ void __launch_bounds__(256)
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
-  int tm = blockIdx.x, tn = blockIdx.y;
-  if (p.group > 0) {
-    const int id = blockIdx.x;
-    const int local = id >> 3;
-    const int t = (id & 7) * p.xcd_chunk + local;
-    if (local >= p.xcd_chunk || t >= p.tiles_m * p.tiles_n) return;  // whole block
-    const int span = p.group * p.tiles_n;
-    const int first = (t / span) * p.group;
-    const int gsz = min(p.tiles_m - first, p.group);
-    const int w = t % span;
-    tm = first + w % gsz;
-    tn = w / gsz;
-  }
+  int tm, tn;
+  if (!tile_of_block(p, tm, tn)) return;  // whole block: padding of the grouped grid
   const int m0 = tm * BM, n0 = tn * BN;
   const int lr = lane & 15, lk = lane >> 4;
   const int kbeg = blockIdx.z * p.kchunk;
@@ -377,24 +346,8 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// C[m, n] = 0 for the split-K accumulation (a kernel, not a memset, so the
-// sequence stays valid inside captured graphs).
-__global__ void k_zero_c(double *C, int M, int N, int64_t scm, int64_t scn, int cplx) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)M * N) return;
-  const int m = (int)(idx % M), n = (int)(idx / M);
-  const int64_t off = (int64_t)m * scm + (int64_t)n * scn;
-  if (cplx)
-    reinterpret_cast<double2 *>(C)[off] = make_double2(0.0, 0.0);
-  else
-    C[off] = 0.0;
-}
-
 // ------------------------------------------------------------------ dispatch
 
-struct Cand {
-  int bm, bn, bk, split, group;
-};
 // Candidate configurations the tuner measures (wave grid 2x2, 256 threads).
 static const Cand kCands[] = {
     {64, 64, 16, 1, 0}, {64, 64, 32, 1, 0}, {128, 64, 16, 1, 0}, {64, 128, 16, 1, 0},
@@ -406,14 +359,7 @@ constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
 
 template <bool CPLX, bool AMC, bool BKC, bool VEC>
 static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
-  p.tiles_m = (p.M + c.bm - 1) / c.bm;
-  p.tiles_n = (p.N + c.bn - 1) / c.bn;
-  p.group = c.group;
-  p.xcd_chunk = (p.tiles_m * p.tiles_n + 7) / 8;
-  dim3 grid(p.tiles_m, p.tiles_n, c.split);
-  if (c.group > 0) grid = dim3(8 * p.xcd_chunk, 1, c.split);
-  int kc = (p.K + c.split - 1) / c.split;
-  p.kchunk = (kc + c.bk - 1) / c.bk * c.bk;
+  dim3 grid = setup_grid(c, p);
 #define MH_L(BM, BN, BK) \
   hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC>), grid, dim3(256), 0, s, p)
   if (c.bm == 64 && c.bn == 64 && c.bk == 16) MH_L(64, 64, 16);
@@ -444,7 +390,7 @@ static bool vec_ok(const GemmP &p, bool amc, bool bkc) {
   return a && b;
 }
 
-static void launch_any(const Cand &c, const GemmP &p, hipStream_t s, bool cplx, bool amc, bool bkc) {
+static void launch_f64(const Cand &c, const GemmP &p, hipStream_t s, bool cplx, bool amc, bool bkc) {
   const bool vec = vec_ok(p, amc, bkc);
   if (cplx) {
     if (vec) launch_layout<true, true>(c, p, s, amc, bkc);
@@ -455,7 +401,7 @@ static void launch_any(const Cand &c, const GemmP &p, hipStream_t s, bool cplx, 
   }
 }
 
-static bool cand_ok(const Cand &c, const GemmP &p, bool cplx) {
+static bool cand_ok_f64(const Cand &c, const GemmP &p, bool cplx) {
   if (c.split > 1 && (p.beta_re != 0.0 || p.beta_im != 0.0)) return false;
   if (c.split > 1 && p.K < 256 * c.split) return false;
   if (cplx && c.bk == 32) return false;              // LDS: keep <= 80 KB
@@ -463,76 +409,7 @@ static bool cand_ok(const Cand &c, const GemmP &p, bool cplx) {
   return true;
 }
 
-// Heuristic when no measurement is available: minimise (tiles per CU, rounded
-// up) x (tile area) -- the kernels are MFMA-bound -- and split K in two when
-// the grid leaves CUs idle.
-static int heuristic(const GemmP &p, bool cplx) {
-  const int ncu = 256;
-  int best = 0;
-  double best_cost = 1e300;
-  for (int i = 0; i < kNCand; ++i) {
-    const Cand &c = kCands[i];
-    if (!cand_ok(c, p, cplx) || c.bk != 16 || c.group != 0) continue;
-    int64_t tiles = (int64_t)((p.M + c.bm - 1) / c.bm) * ((p.N + c.bn - 1) / c.bn) * c.split;
-    double rounds = (double)((tiles + ncu - 1) / ncu);
-    double area = (double)c.bm * c.bn / c.split;
-    double penalty = (c.bm * c.bn == 64 * 64) ? 1.08 : (c.bm * c.bn == 128 * 64 ? 1.03 : 1.0);
-    if (c.split > 1) penalty *= 1.05;
-    double cost = rounds * area * penalty;
-    if (cost < best_cost) {
-      best_cost = cost;
-      best = i;
-    }
-  }
-  return best;
-}
-
-static int run(mh_handle h, const Cand &c, const GemmP &p, bool cplx, bool amc, bool bkc) {
-  if (c.split > 1) {
-    int64_t n = (int64_t)p.M * p.N;
-    hipLaunchKernelGGL(k_zero_c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
-                       p.C, p.M, p.N, p.scm, p.scn, cplx ? 1 : 0);
-  }
-  launch_any(c, p, h->stream, cplx, amc, bkc);
-  return check_launch(h, "gemm");
-}
-
-// Measure every admissible candidate on this shape (the output is recomputed
-// each time, so C ends up holding a correct result) and remember the fastest.
-static int tune(mh_handle h, const GemmP &p, bool cplx, bool amc, bool bkc, int *choice) {
-  hipEvent_t e0, e1;
-  MH_HIP(h, hipEventCreate(&e0));
-  MH_HIP(h, hipEventCreate(&e1));
-  float best = 1e30f;
-  int bi = heuristic(p, cplx);
-  for (int i = 0; i < kNCand; ++i) {
-    const Cand &c = kCands[i];
-    if (!cand_ok(c, p, cplx)) continue;
-    MH_TRY(run(h, c, p, cplx, amc, bkc));  // warm-up
-    MH_HIP(h, hipEventRecord(e0, h->stream));
-    for (int r = 0; r < 3; ++r) MH_TRY(run(h, c, p, cplx, amc, bkc));
-    MH_HIP(h, hipEventRecord(e1, h->stream));
-    MH_HIP(h, hipEventSynchronize(e1));
-    float ms = 0.f;
-    MH_HIP(h, hipEventElapsedTime(&ms, e0, e1));
-    ms /= 3.f;
-    if (h->tune_log)
-      fprintf(stderr, "[mh tune] %s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d: %.1f us (%.2f TFLOP/s)\n",
-              cplx ? "c64" : "f64", p.M, p.N, p.K, (int)amc, (int)bkc, c.bm, c.bn, c.bk, c.split, c.group,
-              ms * 1e3, 2.0 * p.M * (double)p.N * p.K * (cplx ? 4 : 1) / (ms * 1e-3) / 1e12);
-    if (ms < best) {
-      best = ms;
-      bi = i;
-    }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  // The last measured run left C correct; re-run the winner so the final
-  // state does not depend on which candidate happened to be measured last.
-  MH_TRY(run(h, kCands[bi], p, cplx, amc, bkc));
-  *choice = bi;
-  return MH_OK;
-}
+static const GemmFamily kF64Family = {"f64", kCands, kNCand, 16, cand_ok_f64, launch_f64};
 
 // Register-only FP64 MFMA throughput probe: four independent accumulator
 // chains per wave, no memory traffic in the loop.  Measures the matrix-core
@@ -583,33 +460,94 @@ int gemm_mixed(mh_handle h, int dtype, int prec, const GemmDesc &g);
 int gemm(mh_handle h, int dtype, int prec, const GemmDesc &g) {
   if (g.M <= 0 || g.N <= 0) return MH_OK;
   if (prec != MH_PREC_F64) return gemm_mixed(h, dtype, prec, g);
-  GemmP p;
-  p.M = g.M;
-  p.N = g.N;
-  p.K = g.K;
-  p.kchunk = g.K;
-  p.A = (const double *)g.A;
-  p.sam = g.sam;
-  p.sak = g.sak;
-  p.B = (const double *)g.B;
-  p.sbk = g.sbk;
-  p.sbn = g.sbn;
-  p.C = (double *)g.C;
-  p.scm = g.scm;
-  p.scn = g.scn;
-  p.alpha_re = g.alpha_re;
-  p.alpha_im = g.alpha_im;
-  p.beta_re = g.beta_re;
-  p.beta_im = g.beta_im;
-  p.conjA = g.conjA;
-  const bool cplx = dtype == MH_C64;
-  const bool amc = (g.sam == 1) || (g.M == 1);
-  const bool bkc = (g.sbk == 1) || (g.K == 1);
-  const double flops = 2.0 * g.M * (double)g.N * g.K * (cplx ? 4.0 : 1.0);
+  GemmP p = make_params(g);
+  return gemm_dispatch(h, kF64Family, p, dtype == MH_C64, "f64");
+}
 
-  char key[160];
-  snprintf(key, sizeof(key), "f%d:%d:%d:%d:%d%d:%d", cplx ? 1 : 0, g.M, g.N, g.K, (int)amc,
-           (int)bkc, (g.beta_re != 0.0 || g.beta_im != 0.0) ? 1 : 0);
+// ------------------------------------------------------------ shared tuner
+
+static int run_cand(mh_handle h, const GemmFamily &fam, const Cand &c, const GemmP &p, bool cplx,
+                    bool amc, bool bkc) {
+  if (c.split > 1) {
+    int64_t n = (int64_t)p.M * p.N;
+    hipLaunchKernelGGL(k_zero_c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                       p.C, p.M, p.N, p.scm, p.scn, cplx ? 1 : 0, p.c_f32);
+  }
+  fam.launch(c, p, h->stream, cplx, amc, bkc);
+  return check_launch(h, "gemm");
+}
+
+// Heuristic when no measurement is available: minimise (tiles per CU, rounded
+// up) x (tile area) -- the kernels are MFMA-bound -- and split K in two when
+// the grid leaves CUs idle.
+static int heuristic(const GemmFamily &fam, const GemmP &p, bool cplx) {
+  const int ncu = 256;
+  int best = -1;
+  double best_cost = 1e300;
+  for (int i = 0; i < fam.n; ++i) {
+    const Cand &c = fam.cands[i];
+    if (!fam.ok(c, p, cplx) || c.bk != fam.default_bk || c.group != 0) continue;
+    int64_t tiles = (int64_t)((p.M + c.bm - 1) / c.bm) * ((p.N + c.bn - 1) / c.bn) * c.split;
+    double rounds = (double)((tiles + ncu - 1) / ncu);
+    double area = (double)c.bm * c.bn / c.split;
+    double penalty = (c.bm * c.bn == 64 * 64) ? 1.08 : (c.bm * c.bn == 128 * 64 ? 1.03 : 1.0);
+    if (c.split > 1) penalty *= 1.05;
+    double cost = rounds * area * penalty;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = i;
+    }
+  }
+  return best < 0 ? 0 : best;
+}
+
+// Measure every admissible candidate on this shape (the output is recomputed
+// each time, so C ends up holding a correct result) and remember the fastest.
+static int tune(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx, bool amc,
+                bool bkc, int *choice) {
+  hipEvent_t e0, e1;
+  MH_HIP(h, hipEventCreate(&e0));
+  MH_HIP(h, hipEventCreate(&e1));
+  float best = 1e30f;
+  int bi = heuristic(fam, p, cplx);
+  for (int i = 0; i < fam.n; ++i) {
+    const Cand &c = fam.cands[i];
+    if (!fam.ok(c, p, cplx)) continue;
+    MH_TRY(run_cand(h, fam, c, p, cplx, amc, bkc));  // warm-up
+    MH_HIP(h, hipEventRecord(e0, h->stream));
+    for (int r = 0; r < 3; ++r) MH_TRY(run_cand(h, fam, c, p, cplx, amc, bkc));
+    MH_HIP(h, hipEventRecord(e1, h->stream));
+    MH_HIP(h, hipEventSynchronize(e1));
+    float ms = 0.f;
+    MH_HIP(h, hipEventElapsedTime(&ms, e0, e1));
+    ms /= 3.f;
+    if (h->tune_log)
+      fprintf(stderr, "[mh tune] %s%s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d: %.1f us (%.2f TFLOP/s)\n",
+              fam.name, cplx ? "/c" : "", p.M, p.N, p.K, (int)amc, (int)bkc, c.bm, c.bn, c.bk,
+              c.split, c.group, ms * 1e3,
+              2.0 * p.M * (double)p.N * p.K * (cplx ? 4 : 1) / (ms * 1e-3) / 1e12);
+    if (ms < best) {
+      best = ms;
+      bi = i;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // Re-run the winner so the final state does not depend on which candidate
+  // happened to be measured last.
+  MH_TRY(run_cand(h, fam, fam.cands[bi], p, cplx, amc, bkc));
+  *choice = bi;
+  return MH_OK;
+}
+
+int gemm_dispatch(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx,
+                  const char *tag) {
+  const bool amc = (p.sam == 1) || (p.M == 1);
+  const bool bkc = (p.sbk == 1) || (p.K == 1);
+  const double flops = 2.0 * p.M * (double)p.N * p.K * (cplx ? 4.0 : 1.0);
+  char key[192];
+  snprintf(key, sizeof(key), "%s:%d:%d:%d:%d:%d%d:%d:%d", tag, cplx ? 1 : 0, p.M, p.N, p.K,
+           (int)amc, (int)bkc, (p.beta_re != 0.0 || p.beta_im != 0.0) ? 1 : 0, p.c_f32);
   int choice = -1;
   auto it = h->tune.find(key);
   if (it != h->tune.end()) {
@@ -619,16 +557,16 @@ int gemm(mh_handle h, int dtype, int prec, const GemmDesc &g) {
     (void)hipStreamIsCapturing(h->stream, &cs);
     // Tune shapes worth more than ~50 us of work; small ones use the heuristic.
     if (h->autotune && cs == hipStreamCaptureStatusNone && flops > 2e9 && !h->profiling) {
-      MH_TRY(tune(h, p, cplx, amc, bkc, &choice));
+      MH_TRY(tune(h, fam, p, cplx, amc, bkc, &choice));
       h->tune[key] = choice;
       return MH_OK;
     }
-    choice = heuristic(p, cplx);
+    choice = heuristic(fam, p, cplx);
     h->tune[key] = choice;
   }
   int slot;
   prof_begin(h, MH_K_GEMM, flops, &slot);
-  int st = run(h, kCands[choice], p, cplx, amc, bkc);
+  int st = run_cand(h, fam, fam.cands[choice], p, cplx, amc, bkc);
   prof_end(h, slot);
   return st;
 }

@@ -1,12 +1,490 @@
-// Mixed-precision strided GEMM (FP32 and split-BF16 on the matrix cores).
-#include "mh_internal.h"
+// Mixed-precision strided GEMM on the CDNA4 matrix cores.
+//
+// Operands are "packed" 32-bit words per real component:
+//   MH_PREC_F32    : the FP32 value (round to nearest even from FP64);
+//   MH_PREC_BF16X3 : a BF16 pair (hi, lo) with hi = bf16(x), lo = bf16(x - hi),
+//                    hi in the low 16 bits, lo in the high 16 bits.
+// Complex operands store (re, im) word pairs.  C is FP64 (or FP32 when
+// GemmP::c_f32), accumulation is FP32 in the matrix cores.
+//
+//   F32    : v_mfma_f32_16x16x4f32, one product per 16x16x4 step.
+//   BF16X3 : v_mfma_f32_16x16x32_bf16, three products per 16x16x32 step,
+//            hi*hi + hi*lo + lo*hi (the lo*lo term, 2^-16 relative, is dropped).
+//
+// Both operands are staged k-major in LDS ([r][k], r = m for A, n for B), so
+// a lane's fragment (4 consecutive k for FP32, 8 for BF16) is one 16-byte
+// LDS read.  Threads move pairs of adjacent k; an operand contiguous along r
+// is read with two coalesced 4-byte loads per pair, one contiguous along k
+// with one 8-byte load.  Tiling, K loop, split-K and the XCD-grouped order
+// are those of the FP64 family (mh_gemm.hip).
+#include <cstdint>
+
+#include "mh_gemm_common.h"
 
 namespace mh {
 
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef short s8_t __attribute__((ext_vector_type(8)));
+
+constexpr int PM_F32 = 1, PM_BF16 = 2;
+
+__device__ __forceinline__ uint32_t pack_word(double x, int pm) {
+  const float f = (float)x;
+  if (pm == PM_F32) return __builtin_bit_cast(uint32_t, f);
+  const __bf16 hi = (__bf16)f;
+  const __bf16 lo = (__bf16)(f - (float)hi);
+  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
+
+// dst[i] = pack(src[i]) over n doubles (a complex array is 2n doubles).
+template <int PM>
+__global__ void k_pack(const double *__restrict__ src, uint32_t *__restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = pack_word(src[i], PM);
+}
+
+template <int PM, bool CPLX, int BR, bool RMAJ>
+struct MxStage {
+  static constexpr int BK = 32;
+  static constexpr int NC = CPLX ? 2 : 1;        // components
+  static constexpr int NS = PM == PM_BF16 ? 2 : 1;  // planes per component (hi, lo)
+  static constexpr int PITCH = PM == PM_BF16 ? BK + 8 : BK + 4;  // LDS elements per row
+  static constexpr int NPAIR = BR * BK / 512;
+  static_assert(NPAIR >= 1, "tile");
+  __device__ static void coord(int t, int i, int &r, int &k) {
+    const int e = t + 256 * i;
+    if (RMAJ) {
+      r = e % BR;
+      k = 2 * (e / BR);
+    } else {
+      k = 2 * (e % (BK / 2));
+      r = e / (BK / 2);
+    }
+  }
+};
+
+template <int PM, bool CPLX, int BM, int BN, bool AMC, bool BKC, bool VEC, bool OUTF32>
+__global__ void __launch_bounds__(256)
+    k_gemm_mx(const GemmP p) {
+  constexpr int BK = 32;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int NC = CPLX ? 2 : 1;
+  constexpr int NS = PM == PM_BF16 ? 2 : 1;
+  using SA = MxStage<PM, CPLX, BM, AMC>;
+  using SB = MxStage<PM, CPLX, BN, !BKC>;
+  using E = typename std::conditional<PM == PM_BF16, uint16_t, float>::type;
+  constexpr int PITCH = SA::PITCH;
+  __shared__ __align__(16) E sA[NC * NS][2][BM * PITCH];
+  __shared__ __align__(16) E sB[NC * NS][2][BN * PITCH];
+
+  int tm, tn;
+  if (!tile_of_block(p, tm, tn)) return;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int kbeg = blockIdx.z * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+  const uint32_t *A = reinterpret_cast<const uint32_t *>(p.A);
+  const uint32_t *B = reinterpret_cast<const uint32_t *>(p.B);
+  const int64_t a_sr = p.sam, a_sk = p.sak, b_sr = p.sbn, b_sk = p.sbk;
+  const uint32_t conj_mask = p.conjA ? (PM == PM_BF16 ? 0x80008000u : 0x80000000u) : 0u;
+
+  // Per-pair clamped row offsets and row validity.
+  int64_t aoff[SA::NPAIR], boff[SB::NPAIR];
+  bool aok[SA::NPAIR], bok[SB::NPAIR];
+#pragma unroll
+  for (int i = 0; i < SA::NPAIR; ++i) {
+    int r, k;
+    SA::coord(tid, i, r, k);
+    aok[i] = m0 + r < p.M;
+    aoff[i] = (int64_t)min(m0 + r, p.M - 1) * a_sr;
+  }
+#pragma unroll
+  for (int i = 0; i < SB::NPAIR; ++i) {
+    int r, k;
+    SB::coord(tid, i, r, k);
+    bok[i] = n0 + r < p.N;
+    boff[i] = (int64_t)min(n0 + r, p.N - 1) * b_sr;
+  }
+
+  uint32_t ra[NC][SA::NPAIR][2], rb[NC][SB::NPAIR][2];
+
+  // Global -> registers (branch-free; k clamped into the matrix).
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      if (VEC && !AMC) {  // pair contiguous along k
+        const int kc = min(k0 + k, p.K - 2);
+        const int64_t off = aoff[i] + (int64_t)kc;
+        if constexpr (CPLX) {
+          const uint4 v = *reinterpret_cast<const uint4 *>(A + 2 * off);
+          ra[0][i][0] = v.x; ra[1][i][0] = v.y; ra[0][i][1] = v.z; ra[1][i][1] = v.w;
+        } else {
+          const uint2 v = *reinterpret_cast<const uint2 *>(A + off);
+          ra[0][i][0] = v.x; ra[0][i][1] = v.y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int kc = min(k0 + k + e, p.K - 1);
+          const int64_t off = aoff[i] + (int64_t)kc * a_sk;
+          if constexpr (CPLX) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(A + 2 * off);
+            ra[0][i][e] = v.x;
+            ra[1][i][e] = v.y;
+          } else {
+            ra[0][i][e] = A[off];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      if (VEC && BKC) {
+        const int kc = min(k0 + k, p.K - 2);
+        const int64_t off = boff[i] + (int64_t)kc;
+        if constexpr (CPLX) {
+          const uint4 v = *reinterpret_cast<const uint4 *>(B + 2 * off);
+          rb[0][i][0] = v.x; rb[1][i][0] = v.y; rb[0][i][1] = v.z; rb[1][i][1] = v.w;
+        } else {
+          const uint2 v = *reinterpret_cast<const uint2 *>(B + off);
+          rb[0][i][0] = v.x; rb[0][i][1] = v.y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int kc = min(k0 + k + e, p.K - 1);
+          const int64_t off = boff[i] + (int64_t)kc * b_sk;
+          if constexpr (CPLX) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(B + 2 * off);
+            rb[0][i][e] = v.x;
+            rb[1][i][e] = v.y;
+          } else {
+            rb[0][i][e] = B[off];
+          }
+        }
+      }
+    }
+  };
+
+  // Registers -> LDS (masked), k-major [r][k].
+  auto store = [&](int buf, int k0) {
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      const bool ok0 = aok[i] && (k0 + k < kend), ok1 = aok[i] && (k0 + k + 1 < kend);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const uint32_t cm = (c == 1) ? conj_mask : 0u;
+        const uint32_t w0 = ok0 ? (ra[c][i][0] ^ cm) : 0u, w1 = ok1 ? (ra[c][i][1] ^ cm) : 0u;
+        if constexpr (PM == PM_BF16) {
+          *reinterpret_cast<uint32_t *>(&sA[c * 2][buf][r * PITCH + k]) = (w0 & 0xffffu) | (w1 << 16);
+          *reinterpret_cast<uint32_t *>(&sA[c * 2 + 1][buf][r * PITCH + k]) = (w0 >> 16) | (w1 & 0xffff0000u);
+        } else {
+          *reinterpret_cast<uint2 *>(&sA[c][buf][r * PITCH + k]) = make_uint2(w0, w1);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      const bool ok0 = bok[i] && (k0 + k < kend), ok1 = bok[i] && (k0 + k + 1 < kend);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const uint32_t w0 = ok0 ? rb[c][i][0] : 0u, w1 = ok1 ? rb[c][i][1] : 0u;
+        if constexpr (PM == PM_BF16) {
+          *reinterpret_cast<uint32_t *>(&sB[c * 2][buf][r * PITCH + k]) = (w0 & 0xffffu) | (w1 << 16);
+          *reinterpret_cast<uint32_t *>(&sB[c * 2 + 1][buf][r * PITCH + k]) = (w0 >> 16) | (w1 & 0xffff0000u);
+        } else {
+          *reinterpret_cast<uint2 *>(&sB[c][buf][r * PITCH + k]) = make_uint2(w0, w1);
+        }
+      }
+    }
+  };
+
+  f4_t acc[NC][TM][TN];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[c][i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  auto compute = [&](int buf) {
+    if constexpr (PM == PM_F32) {
+#pragma unroll
+      for (int kb = 0; kb < BK; kb += 16) {
+        f4_t a[NC][TM], b[NC][TN];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+            a[c][i] = *reinterpret_cast<const f4_t *>(&sA[c][buf][(wm * WTM + i * 16 + lr) * PITCH + kb + 4 * lk]);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            b[c][j] = *reinterpret_cast<const f4_t *>(&sB[c][buf][(wn * WTN + j * 16 + lr) * PITCH + kb + 4 * lk]);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+              if constexpr (CPLX) {
+                acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
+                acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[1][i][s], b[1][j][s], acc[0][i][j], 0, 0, 0);
+                acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
+                acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][i][s], b[0][j][s], acc[1][i][j], 0, 0, 0);
+              } else {
+                acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
+              }
+            }
+      }
+    } else {
+      bf8_t ah[NC][TM], al[NC][TM], bh[NC][TN], bl[NC][TN];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          const int o = (wm * WTM + i * 16 + lr) * PITCH + 8 * lk;
+          ah[c][i] = *reinterpret_cast<const bf8_t *>(&sA[2 * c][buf][o]);
+          al[c][i] = *reinterpret_cast<const bf8_t *>(&sA[2 * c + 1][buf][o]);
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const int o = (wn * WTN + j * 16 + lr) * PITCH + 8 * lk;
+          bh[c][j] = *reinterpret_cast<const bf8_t *>(&sB[2 * c][buf][o]);
+          bl[c][j] = *reinterpret_cast<const bf8_t *>(&sB[2 * c + 1][buf][o]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][i], bh[0][j], acc[0][i][j], 0, 0, 0);
+          acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][i], bl[0][j], acc[0][i][j], 0, 0, 0);
+          acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0][i], bh[0][j], acc[0][i][j], 0, 0, 0);
+          if constexpr (CPLX) {
+            // re -= im(A) im(B): negate the A imaginary fragments (sign bits)
+            const bf8_t nah = __builtin_bit_cast(bf8_t, __builtin_bit_cast(s8_t, ah[1][i]) ^ (short)0x8000);
+            const bf8_t nal = __builtin_bit_cast(bf8_t, __builtin_bit_cast(s8_t, al[1][i]) ^ (short)0x8000);
+            acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nah, bh[1][j], acc[0][i][j], 0, 0, 0);
+            acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nah, bl[1][j], acc[0][i][j], 0, 0, 0);
+            acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nal, bh[1][j], acc[0][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][i], bh[1][j], acc[1][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0][i], bl[1][j], acc[1][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0][i], bh[1][j], acc[1][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[1][i], bh[0][j], acc[1][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[1][i], bl[0][j], acc[1][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[1][i], bh[0][j], acc[1][i][j], 0, 0, 0);
+          }
+        }
+    }
+  };
+
+  const int nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nk > 0) {
+    load(kbeg);
+    store(0, kbeg);
+    __syncthreads();
+    for (int kt = 0; kt + 1 < nk; ++kt) {
+      const int k1 = kbeg + (kt + 1) * BK;
+      load(k1);
+      compute(kt & 1);
+      store((kt + 1) & 1, k1);
+      __syncthreads();
+    }
+    compute((nk - 1) & 1);
+  }
+
+  // f32 MFMA accumulator layout: register r of lane l is C[4*(l/16) + r][l%16].
+  const bool split = gridDim.z > 1;
+  const bool has_beta = (p.beta_re != 0.0 || p.beta_im != 0.0);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn * WTN + j * 16 + lr;
+      if (n >= p.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * WTM + i * 16 + 4 * lk + r;
+        if (m >= p.M) continue;
+        const int64_t off = (int64_t)m * p.scm + (int64_t)n * p.scn;
+        const double xr = (double)acc[0][i][j][r];
+        const double xi = CPLX ? (double)acc[NC - 1][i][j][r] : 0.0;
+        double vr = p.alpha_re * xr - p.alpha_im * xi;
+        double vi = p.alpha_re * xi + p.alpha_im * xr;
+        if constexpr (OUTF32) {
+          float *cf = reinterpret_cast<float *>(p.C);
+          if constexpr (CPLX) {
+            float2 *c = reinterpret_cast<float2 *>(cf) + off;
+            if (split) {
+              atomicAdd(&c->x, (float)vr);
+              atomicAdd(&c->y, (float)vi);
+            } else {
+              if (has_beta) {
+                const float2 o = *c;
+                vr += p.beta_re * o.x - p.beta_im * o.y;
+                vi += p.beta_re * o.y + p.beta_im * o.x;
+              }
+              *c = make_float2((float)vr, (float)vi);
+            }
+          } else {
+            float *c = cf + off;
+            if (split) atomicAdd(c, (float)vr);
+            else *c = (float)(has_beta ? vr + p.beta_re * (*c) : vr);
+          }
+        } else {
+          if constexpr (CPLX) {
+            double2 *c = reinterpret_cast<double2 *>(p.C) + off;
+            if (split) {
+              atomicAdd(&c->x, vr);
+              atomicAdd(&c->y, vi);
+            } else {
+              if (has_beta) {
+                const double2 o = *c;
+                vr += p.beta_re * o.x - p.beta_im * o.y;
+                vi += p.beta_re * o.y + p.beta_im * o.x;
+              }
+              *c = make_double2(vr, vi);
+            }
+          } else {
+            double *c = p.C + off;
+            if (split) atomicAdd(c, vr);
+            else *c = has_beta ? vr + p.beta_re * (*c) : vr;
+          }
+        }
+      }
+    }
+}
+
+// ------------------------------------------------------------------ dispatch
+
+static const Cand kMxCands[] = {
+    {64, 64, 32, 1, 0}, {128, 64, 32, 1, 0}, {64, 128, 32, 1, 0}, {128, 128, 32, 1, 0},
+    {64, 64, 32, 2, 0}, {128, 64, 32, 2, 0}, {128, 128, 32, 2, 0},
+    {64, 64, 32, 1, 8}, {128, 64, 32, 1, 8}, {128, 128, 32, 1, 8},
+};
+constexpr int kNMx = sizeof(kMxCands) / sizeof(kMxCands[0]);
+
+template <int PM, bool CPLX, bool AMC, bool BKC, bool VEC>
+static void launch_mx_cfg(const Cand &c, GemmP p, hipStream_t s) {
+  dim3 grid = setup_grid(c, p);
+#define MH_L(BM, BN)                                                                      \
+  do {                                                                                    \
+    if (p.c_f32)                                                                          \
+      hipLaunchKernelGGL((k_gemm_mx<PM, CPLX, BM, BN, AMC, BKC, VEC, true>), grid, dim3(256), 0, s, p); \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_gemm_mx<PM, CPLX, BM, BN, AMC, BKC, VEC, false>), grid, dim3(256), 0, s, p); \
+  } while (0)
+  if (c.bm == 64 && c.bn == 64) MH_L(64, 64);
+  else if (c.bm == 128 && c.bn == 64) MH_L(128, 64);
+  else if (c.bm == 64 && c.bn == 128) MH_L(64, 128);
+  else if constexpr (!CPLX) MH_L(128, 128);
+#undef MH_L
+}
+
+// 8-byte pair loads along k need an even K, an even stride along rows and an
+// 8-byte (complex 16-byte) aligned base; only k-major operands use them.
+static bool mx_vec_ok(const GemmP &p, bool amc, bool bkc, bool cplx) {
+  const uintptr_t al = cplx ? 15 : 7;
+  const bool a = amc || (((uintptr_t)p.A & al) == 0 && p.K % 2 == 0 && (p.sam % 2 == 0 || p.M == 1));
+  const bool b = !bkc || (((uintptr_t)p.B & al) == 0 && p.K % 2 == 0 && (p.sbn % 2 == 0 || p.N == 1));
+  return a && b;
+}
+
+template <int PM, bool CPLX, bool VEC>
+static void launch_mx_layout(const Cand &c, const GemmP &p, hipStream_t s, bool amc, bool bkc) {
+  if (amc && bkc) launch_mx_cfg<PM, CPLX, true, true, VEC>(c, p, s);
+  else if (amc) launch_mx_cfg<PM, CPLX, true, false, VEC>(c, p, s);
+  else if (bkc) launch_mx_cfg<PM, CPLX, false, true, VEC>(c, p, s);
+  else launch_mx_cfg<PM, CPLX, false, false, VEC>(c, p, s);
+}
+
+template <int PM>
+static void launch_mx(const Cand &c, const GemmP &p, hipStream_t s, bool cplx, bool amc, bool bkc) {
+  const bool vec = mx_vec_ok(p, amc, bkc, cplx);
+  if (cplx) {
+    if (vec) launch_mx_layout<PM, true, true>(c, p, s, amc, bkc);
+    else launch_mx_layout<PM, true, false>(c, p, s, amc, bkc);
+  } else {
+    if (vec) launch_mx_layout<PM, false, true>(c, p, s, amc, bkc);
+    else launch_mx_layout<PM, false, false>(c, p, s, amc, bkc);
+  }
+}
+
+static bool cand_ok_mx(const Cand &c, const GemmP &p, bool cplx) {
+  if (c.split > 1 && (p.beta_re != 0.0 || p.beta_im != 0.0)) return false;
+  if (c.split > 1 && p.K < 512 * c.split) return false;
+  if (cplx && c.bm * c.bn > 64 * 128) return false;
+  return true;
+}
+
+static const GemmFamily kF32Family = {"f32", kMxCands, kNMx, 32, cand_ok_mx, launch_mx<PM_F32>};
+static const GemmFamily kBF16Family = {"bf16x3", kMxCands, kNMx, 32, cand_ok_mx, launch_mx<PM_BF16>};
+
+const GemmFamily &mixed_family(int prec) {
+  return prec == MH_PREC_F32 ? kF32Family : kBF16Family;
+}
+
+int pack_operand(mh_handle h, int prec, const double *src, int64_t ndouble, uint32_t *dst) {
+  if (ndouble <= 0) return MH_OK;
+  dim3 grid((unsigned)((ndouble + 255) / 256));
+  if (prec == MH_PREC_F32)
+    hipLaunchKernelGGL(k_pack<PM_F32>, grid, dim3(256), 0, h->stream, src, dst, ndouble);
+  else
+    hipLaunchKernelGGL(k_pack<PM_BF16>, grid, dim3(256), 0, h->stream, src, dst, ndouble);
+  return check_launch(h, "pack");
+}
+
+// Element span of a strided matrix: the largest offset reached plus one.
+static int64_t span(int rows, int cols, int64_t sr, int64_t sc) {
+  return (int64_t)(rows - 1) * sr + (int64_t)(cols - 1) * sc + 1;
+}
+
+// mh_gemm with a mixed precision: pack both FP64 operands (their whole
+// element span, so the strides carry over unchanged) into the handle's
+// packing buffer, then run the mixed family with an FP64 C.
 int gemm_mixed(mh_handle h, int dtype, int prec, const GemmDesc &g) {
-  (void)dtype;
-  (void)g;
-  return fail(h, MH_ERR_UNSUPPORTED, "precision %d is not built yet", prec);
+  if (prec != MH_PREC_F32 && prec != MH_PREC_BF16X3)
+    return fail(h, MH_ERR_DTYPE, "unknown precision %d", prec);
+  const bool cplx = dtype == MH_C64;
+  const int64_t ew = cplx ? 2 : 1;
+  const int64_t na = span(g.M, g.K, g.sam, g.sak) * ew;
+  const int64_t nb = span(g.K, g.N, g.sbk, g.sbn) * ew;
+  const size_t abytes = ((size_t)na * 4 + 255) & ~size_t(255);
+  const size_t bytes = abytes + (size_t)nb * 4;
+  if (h->pack_bytes < bytes) {
+    MH_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->pack) MH_HIP(h, hipFree(h->pack));
+    h->pack = nullptr;
+    h->pack_bytes = 0;
+    if (hipMalloc(&h->pack, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(h, MH_ERR_OOM, "packing buffer of %zu bytes", bytes);
+    }
+    h->pack_bytes = bytes;
+  }
+  uint32_t *pa = (uint32_t *)h->pack;
+  uint32_t *pb = (uint32_t *)((char *)h->pack + abytes);
+  MH_TRY(pack_operand(h, prec, (const double *)g.A, na, pa));
+  MH_TRY(pack_operand(h, prec, (const double *)g.B, nb, pb));
+  GemmP p = make_params(g);
+  p.A = (const double *)pa;
+  p.B = (const double *)pb;
+  return gemm_dispatch(h, mixed_family(prec), p, cplx, prec == MH_PREC_F32 ? "f32" : "bf16x3");
 }
 
 }  // namespace mh

@@ -29,6 +29,9 @@ struct mh_context {
   // Grow-only workspace for contraction intermediates.
   void *ws = nullptr;
   size_t ws_bytes = 0;
+  // Packed (FP32 / BF16-pair) operand copies for the mixed-precision GEMMs.
+  void *pack = nullptr;
+  size_t pack_bytes = 0;
   // Small device buffers: packed MPO mode matrices and reduction scratch.
   void *small = nullptr;
   size_t small_bytes = 0;

@@ -147,6 +147,7 @@ int mh_destroy(mh_handle h) {
   for (auto e : h->event_pool) (void)hipEventDestroy(e);
   if (h->ws) (void)hipFree(h->ws);
   if (h->small) (void)hipFree(h->small);
+  if (h->pack) (void)hipFree(h->pack);
   if (h->red) (void)hipFree(h->red);
   if (h->red_host) (void)hipHostFree(h->red_host);
   delete h;

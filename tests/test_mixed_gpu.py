@@ -1,0 +1,99 @@
+"""Mixed-precision forms of the contractions against the FP64 oracle.
+
+Tolerances (relative 2-norm error of the result):
+  FP32 operands, FP32 MFMA accumulation            : 1e-5
+  split-BF16 x3 (hi*hi + hi*lo + lo*hi), FP32 acc. : 1e-4
+The expected errors are ~2^-24*sqrt(K) and ~2^-16 respectively; the bounds
+leave an order of magnitude of headroom and still fail for a wrong layout,
+a dropped split term (error ~2^-8) or a missing conjugation (error ~1).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rand
+from oracle import contractions as C
+
+pytestmark = pytest.mark.gpu
+
+TOLS = {"f32": 1e-5, "bf16x3": 1e-4}
+PRECS = {"f32": 1, "bf16x3": 2}
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def dev(a):
+    from finitemptools_amd.engine import to_device
+    return to_device(a)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("layout", ["NN", "NT", "TN", "TT"])
+def test_mixed_gemm_layouts(gpu, prec, cplx, layout):
+    import torch
+    rng = np.random.default_rng(3)
+    M, N, K = 77, 130, 45
+    A, B, Cm = rand(rng, (M, K), cplx), rand(rng, (K, N), cplx), rand(rng, (M, N), cplx)
+    Ad = dev(A) if layout[0] == "N" else dev(np.ascontiguousarray(A.T)).t()
+    Bd = dev(B) if layout[1] == "N" else dev(np.ascontiguousarray(B.T)).t()
+    alpha, beta = (0.6 + 0.3j, -0.2 + 0.1j) if cplx else (0.6, -0.2)
+    h = gpu.bind_stream()
+    for conj in (0, 1):
+        Cd = dev(Cm)
+        st = gpu.lib.mh_gemm(h, 1 if cplx else 0, PRECS[prec], M, N, K, ctypes.c_void_p(Ad.data_ptr()),
+                             Ad.stride(0), Ad.stride(1), conj, ctypes.c_void_p(Bd.data_ptr()),
+                             Bd.stride(0), Bd.stride(1), ctypes.c_void_p(Cd.data_ptr()), Cd.stride(0),
+                             Cd.stride(1), complex(alpha).real, complex(alpha).imag,
+                             complex(beta).real, complex(beta).imag)
+        gpu.check(st)
+        torch.cuda.synchronize()
+        ref = alpha * (A.conj() if conj else A) @ B + beta * Cm
+        assert relerr(host(Cd), ref) < TOLS[prec]
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_mixed_heff_and_env(gpu, prec, cplx):
+    from finitemptools_amd import models
+    from finitemptools_amd.derivatives import ac2_prime, ac_prime, c_prime
+    from finitemptools_amd.transfer import updateleft, updateright
+    rng = np.random.default_rng(11)
+    W = models.heisenberg_xxz_mpo(4, Jzz=0.9, hz=0.1)[1]
+    W = W.astype(np.complex128) if cplx else W
+    Dl, Dr, w = 40, 36, W.shape[0]
+    L = rand(rng, (Dl, w, Dl), cplx) / np.sqrt(Dl * w)
+    R = rand(rng, (Dr, w, Dr), cplx) / np.sqrt(Dr * w)
+    x2 = rand(rng, (Dl, 2, 2, Dr), cplx)
+    x1 = rand(rng, (Dl, 2, Dr), cplx)
+    x0 = rand(rng, (Dl, Dr), cplx)
+    tol = TOLS[prec]
+    assert relerr(host(ac2_prime(dev(x2), dev(W), dev(W), dev(L), dev(R), prec=prec)),
+                  C.ac2_prime(x2, W, W, L, R)) < tol
+    assert relerr(host(ac_prime(dev(x1), dev(W), dev(L), dev(R), prec=prec)), C.ac_prime(x1, W, L, R)) < tol
+    assert relerr(host(c_prime(dev(x0), dev(L), dev(R), prec=prec)), C.c_prime(x0, L, R)) < tol
+    A, Bk = rand(rng, (Dl, 2, Dr), cplx), rand(rng, (Dl, 2, Dr), cplx)
+    assert relerr(host(updateleft(dev(L), dev(A), dev(W), dev(Bk), prec=prec)), C.updateleft(L, A, W, Bk)) < tol
+    assert relerr(host(updateright(dev(R), dev(A), dev(W), dev(Bk), prec=prec)), C.updateright(R, A, W, Bk)) < tol
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+def test_mixed_benchmark_size(gpu, prec):
+    """2-site H_eff at D = 512 (the benchmark shape) in the mixed forms."""
+    from finitemptools_amd import models
+    from finitemptools_amd.derivatives import HeffOperator
+    rng = np.random.default_rng(5)
+    W = models.heisenberg_xxz_mpo(4)[1]
+    D, w = 512, 5
+    L = rng.standard_normal((D, w, D)) / np.sqrt(D * w)
+    R = rng.standard_normal((D, w, D)) / np.sqrt(D * w)
+    x = rng.standard_normal((D, 2, 2, D))
+    op = HeffOperator(2, dev(L), dev(R), dev(W), dev(W), prec=prec)
+    assert relerr(host(op(dev(x))), C.ac2_prime(x, W, W, L, R)) < TOLS[prec]
