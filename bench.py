@@ -131,6 +131,13 @@ def main():
     ref = C.ac2_prime(xs[0], W, W, L, R)
     err2 = float(np.linalg.norm(r2["y0"] - ref) / np.linalg.norm(ref))
     r1 = gpu_run(1, D1, max(1, args.steps // 2), args.warmup)
+    mixed = {}
+    for prec in ("f32", "bf16x3"):
+        rm = gpu_run(2, D2, args.steps, args.warmup, prec=prec)
+        mixed[prec] = {"tflops_f64_equivalent": round(rm["tflops"], 3),
+                       "ms": round(rm["sec_per_matvec"] * 1e3, 4),
+                       "speedup_vs_f64": round(r2["sec_per_matvec"] / rm["sec_per_matvec"], 3),
+                       "rel_err_vs_f64_oracle": float(np.linalg.norm(rm["y0"] - ref) / np.linalg.norm(ref))}
     import ctypes
     from finitemptools_amd.engine import engine
     eng = engine(0)
@@ -159,6 +166,7 @@ def main():
         "ac2_prime_D512_f64": {"tflops": round(r2["tflops"], 3), "ms": round(r2["sec_per_matvec"] * 1e3, 4),
                                "rel_err_vs_oracle": err2},
         "ac_prime_D1024_f64": {"tflops": round(r1["tflops"], 3), "ms": round(r1["sec_per_matvec"] * 1e3, 4)},
+        "ac2_prime_D512_mixed": mixed,
         "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (both GEMM stages of ac2_prime)",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,

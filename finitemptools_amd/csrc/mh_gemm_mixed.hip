@@ -4,12 +4,16 @@
 //   MH_PREC_F32    : the FP32 value (round to nearest even from FP64);
 //   MH_PREC_BF16X3 : a BF16 pair (hi, lo) with hi = bf16(x), lo = bf16(x - hi),
 //                    hi in the low 16 bits, lo in the high 16 bits.
-// Complex operands store (re, im) word pairs.  C is FP64 (or FP32 when
-// GemmP::c_f32), accumulation is FP32 in the matrix cores.
+// Complex operands store (re, im) word pairs.  Accumulation is FP32 in the
+// matrix cores.
 //
 //   F32    : v_mfma_f32_16x16x4f32, one product per 16x16x4 step.
 //   BF16X3 : v_mfma_f32_16x16x32_bf16, three products per 16x16x32 step,
 //            hi*hi + hi*lo + lo*hi (the lo*lo term, 2^-16 relative, is dropped).
+//
+// C is FP64, or (GemmP::c_f32 = 1) "packed" words of the same precision as
+// the operands, so one GEMM's output can feed the next without a separate
+// conversion pass.
 //
 // Both operands are staged k-major in LDS ([r][k], r = m for A, n for B), so
 // a lane's fragment (4 consecutive k for FP32, 8 for BF16) is one 16-byte
@@ -31,6 +35,19 @@ constexpr int PM_F32 = 1, PM_BF16 = 2;
 
 __device__ __forceinline__ uint32_t pack_word(double x, int pm) {
   const float f = (float)x;
+  if (pm == PM_F32) return __builtin_bit_cast(uint32_t, f);
+  const __bf16 hi = (__bf16)f;
+  const __bf16 lo = (__bf16)(f - (float)hi);
+  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
+
+__device__ __forceinline__ float unpack_word(uint32_t w, int pm) {
+  if (pm == PM_F32) return __builtin_bit_cast(float, w);
+  return __builtin_bit_cast(float, w << 16) + __builtin_bit_cast(float, w & 0xffff0000u);
+}
+
+__device__ __forceinline__ uint32_t pack_word_f(float f, int pm) {
   if (pm == PM_F32) return __builtin_bit_cast(uint32_t, f);
   const __bf16 hi = (__bf16)f;
   const __bf16 lo = (__bf16)(f - (float)hi);
@@ -327,24 +344,21 @@ __global__ void __launch_bounds__(256)
         double vr = p.alpha_re * xr - p.alpha_im * xi;
         double vi = p.alpha_re * xi + p.alpha_im * xr;
         if constexpr (OUTF32) {
-          float *cf = reinterpret_cast<float *>(p.C);
-          if constexpr (CPLX) {
-            float2 *c = reinterpret_cast<float2 *>(cf) + off;
-            if (split) {
-              atomicAdd(&c->x, (float)vr);
-              atomicAdd(&c->y, (float)vi);
-            } else {
-              if (has_beta) {
-                const float2 o = *c;
-                vr += p.beta_re * o.x - p.beta_im * o.y;
-                vi += p.beta_re * o.y + p.beta_im * o.x;
-              }
-              *c = make_float2((float)vr, (float)vi);
-            }
+          // packed output words of this family's precision
+          uint32_t *cw = reinterpret_cast<uint32_t *>(p.C);
+          const int64_t ow = CPLX ? 2 * off : off;
+          if (has_beta) {
+            const double orr = unpack_word(cw[ow], PM);
+            const double oi = CPLX ? unpack_word(cw[ow + 1], PM) : 0.0;
+            vr += p.beta_re * orr - p.beta_im * oi;
+            vi += p.beta_re * oi + p.beta_im * orr;
+          }
+          if (split) {  // F32 family only (see cand_ok_mx)
+            atomicAdd(reinterpret_cast<float *>(cw + ow), (float)vr);
+            if (CPLX) atomicAdd(reinterpret_cast<float *>(cw + ow + 1), (float)vi);
           } else {
-            float *c = cf + off;
-            if (split) atomicAdd(c, (float)vr);
-            else *c = (float)(has_beta ? vr + p.beta_re * (*c) : vr);
+            cw[ow] = pack_word_f((float)vr, PM);
+            if (CPLX) cw[ow + 1] = pack_word_f((float)vi, PM);
           }
         } else {
           if constexpr (CPLX) {
@@ -425,15 +439,21 @@ static void launch_mx(const Cand &c, const GemmP &p, hipStream_t s, bool cplx, b
   }
 }
 
-static bool cand_ok_mx(const Cand &c, const GemmP &p, bool cplx) {
+static bool cand_ok_mx_f32(const Cand &c, const GemmP &p, bool cplx) {
   if (c.split > 1 && (p.beta_re != 0.0 || p.beta_im != 0.0)) return false;
   if (c.split > 1 && p.K < 512 * c.split) return false;
   if (cplx && c.bm * c.bn > 64 * 128) return false;
   return true;
 }
 
-static const GemmFamily kF32Family = {"f32", kMxCands, kNMx, 32, cand_ok_mx, launch_mx<PM_F32>};
-static const GemmFamily kBF16Family = {"bf16x3", kMxCands, kNMx, 32, cand_ok_mx, launch_mx<PM_BF16>};
+// BF16-pair output words cannot be accumulated atomically: no split-K there.
+static bool cand_ok_mx_bf16(const Cand &c, const GemmP &p, bool cplx) {
+  if (c.split > 1 && p.c_f32) return false;
+  return cand_ok_mx_f32(c, p, cplx);
+}
+
+static const GemmFamily kF32Family = {"f32", kMxCands, kNMx, 32, cand_ok_mx_f32, launch_mx<PM_F32>};
+static const GemmFamily kBF16Family = {"bf16x3", kMxCands, kNMx, 32, cand_ok_mx_bf16, launch_mx<PM_BF16>};
 
 const GemmFamily &mixed_family(int prec) {
   return prec == MH_PREC_F32 ? kF32Family : kBF16Family;
@@ -447,6 +467,117 @@ int pack_operand(mh_handle h, int prec, const double *src, int64_t ndouble, uint
   else
     hipLaunchKernelGGL(k_pack<PM_BF16>, grid, dim3(256), 0, h->stream, src, dst, ndouble);
   return check_launch(h, "pack");
+}
+
+// out[p, j, q] = sum_i in[p, i, q] Mt[i, j] on packed words (FP32 arithmetic);
+// the FP64 Mt chunk is rounded to FP32 in LDS.  Same thread mapping and
+// zero-row skipping as the FP64 mode product.
+template <int PM, bool CPLX, int JC>
+__global__ void __launch_bounds__(256)
+    k_mode_p(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+             const double *__restrict__ Mt, int P, int ni, int nj, int64_t PQ, int64_t sIq,
+             int64_t sOq) {
+  constexpr int EW = CPLX ? 2 : 1;
+  extern __shared__ float smem_f[];
+  float *sM = smem_f;                              // [ni][JC][EW]
+  int *nz = reinterpret_cast<int *>(sM + ni * JC * EW);
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.y * JC;
+  const int jn = min(JC, nj - j0);
+  for (int t = tid; t < ni * JC; t += 256) {
+    const int i = t / JC, jj = t % JC;
+#pragma unroll
+    for (int c = 0; c < EW; ++c)
+      sM[t * EW + c] = (jj < jn) ? (float)Mt[(i + (int64_t)ni * (j0 + jj)) * EW + c] : 0.f;
+  }
+  __syncthreads();
+  for (int i = tid; i < ni; i += 256) {
+    int f = 0;
+    for (int q = 0; q < JC * EW; ++q) f |= sM[i * JC * EW + q] != 0.f;
+    nz[i] = f;
+  }
+  __syncthreads();
+  const int64_t idx = (int64_t)blockIdx.x * 256 + tid;
+  if (idx >= PQ) return;
+  const int64_t pp = idx % P, q = idx / P;
+  float ar[JC], ai[JC];
+#pragma unroll
+  for (int jj = 0; jj < JC; ++jj) ar[jj] = ai[jj] = 0.f;
+  const int64_t base = pp + q * sIq;
+  for (int i = 0; i < ni; ++i) {
+    if (!nz[i]) continue;
+    const int64_t o = (base + (int64_t)i * P) * EW;
+    const float vr = unpack_word(in[o], PM);
+    const float vi = CPLX ? unpack_word(in[o + 1], PM) : 0.f;
+#pragma unroll
+    for (int jj = 0; jj < JC; ++jj) {
+      const float mr = sM[(i * JC + jj) * EW];
+      if constexpr (CPLX) {
+        const float mi = sM[(i * JC + jj) * EW + 1];
+        ar[jj] += vr * mr - vi * mi;
+        ai[jj] += vr * mi + vi * mr;
+      } else {
+        ar[jj] += vr * mr;
+      }
+    }
+  }
+  const int64_t ob = pp + q * sOq;
+#pragma unroll
+  for (int jj = 0; jj < JC; ++jj)
+    if (jj < jn) {
+      const int64_t o = (ob + (int64_t)(j0 + jj) * P) * EW;
+      out[o] = pack_word_f(ar[jj], PM);
+      if (CPLX) out[o + 1] = pack_word_f(ai[jj], PM);
+    }
+}
+
+template <int PM, bool CPLX, int JC>
+static void launch_mode_p(mh_handle h, int P, int ni, int nj, int64_t Q, const uint32_t *in,
+                          int64_t sIq, const double *Mt, uint32_t *out, int64_t sOq) {
+  const int64_t PQ = (int64_t)P * Q;
+  dim3 grid((unsigned)((PQ + 255) / 256), (nj + JC - 1) / JC);
+  const size_t smem = sizeof(float) * ni * JC * (CPLX ? 2 : 1) + sizeof(int) * ni;
+  hipLaunchKernelGGL((k_mode_p<PM, CPLX, JC>), grid, dim3(256), smem, h->stream, in, out, Mt, P,
+                     ni, nj, PQ, sIq, sOq);
+}
+
+template <int PM, bool CPLX>
+static void mode_p_dispatch(mh_handle h, int P, int ni, int nj, int64_t Q, const uint32_t *in,
+                            int64_t sIq, const double *Mt, uint32_t *out, int64_t sOq) {
+  if (nj <= 8) launch_mode_p<PM, CPLX, 8>(h, P, ni, nj, Q, in, sIq, Mt, out, sOq);
+  else if (nj <= 16) launch_mode_p<PM, CPLX, 16>(h, P, ni, nj, Q, in, sIq, Mt, out, sOq);
+  else launch_mode_p<PM, CPLX, 24>(h, P, ni, nj, Q, in, sIq, Mt, out, sOq);
+}
+
+int mode_product_packed(mh_handle h, int dtype, int prec, int P, int ni, int nj, int Q,
+                        const void *in, int64_t sIq, const void *Mt, void *out, int64_t sOq) {
+  if ((int64_t)P * Q <= 0) return MH_OK;
+  const bool c = dtype == MH_C64;
+  double flops = 2.0 * P * (double)Q * ni * nj * (c ? 4.0 : 1.0);
+  int slot;
+  prof_begin(h, MH_K_WSTAGE, flops, &slot);
+  const uint32_t *pin = (const uint32_t *)in;
+  uint32_t *pout = (uint32_t *)out;
+  const double *m = (const double *)Mt;
+  if (prec == MH_PREC_F32) {
+    if (c) mode_p_dispatch<PM_F32, true>(h, P, ni, nj, Q, pin, sIq, m, pout, sOq);
+    else mode_p_dispatch<PM_F32, false>(h, P, ni, nj, Q, pin, sIq, m, pout, sOq);
+  } else {
+    if (c) mode_p_dispatch<PM_BF16, true>(h, P, ni, nj, Q, pin, sIq, m, pout, sOq);
+    else mode_p_dispatch<PM_BF16, false>(h, P, ni, nj, Q, pin, sIq, m, pout, sOq);
+  }
+  prof_end(h, slot);
+  return check_launch(h, "mode_product_packed");
+}
+
+// GEMM on operands that are already packed; c_packed selects packed words
+// (1) or FP64 (0) for C.
+int gemm_packed(mh_handle h, int dtype, int prec, const GemmDesc &g, int c_packed) {
+  if (g.M <= 0 || g.N <= 0) return MH_OK;
+  GemmP p = make_params(g);
+  p.c_f32 = c_packed;
+  return gemm_dispatch(h, mixed_family(prec), p, dtype == MH_C64,
+                       prec == MH_PREC_F32 ? "f32" : "bf16x3");
 }
 
 // Element span of a strided matrix: the largest offset reached plus one.

@@ -202,12 +202,76 @@ static void heff_tmp_elems(const mh_heff_op &op, size_t *t1, size_t *t2) {
   }
 }
 
+// Mixed-precision workspace: packed (4-byte per real component) T1, T2, x,
+// the FP64 mode matrix, and packed copies of L and R.
+struct MixedLayout {
+  size_t t1, t2, xp, mt, lp, rp, total;
+  explicit MixedLayout(const mh_heff_op &op) {
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t e1, e2;
+    heff_tmp_elems(op, &e1, &e2);
+    const size_t w = 4 * (op.dtype == MH_C64 ? 2 : 1);
+    t1 = 0;
+    t2 = t1 + al(e1 * w);
+    xp = t2 + al(e2 * w);
+    mt = xp + al(heff_vec_len(op) * w);
+    lp = mt + al(mode_elems(op) * dtype_bytes(op.dtype));
+    rp = lp + al((size_t)op.Dl * op.wl * op.Dl * w);
+    total = rp + al((size_t)op.Dr * op.wr * op.Dr * w);
+  }
+};
+
 size_t heff_ws_bytes(const mh_heff_op &op) {
+  if (op.prec != MH_PREC_F64) return MixedLayout(op).total;
   size_t t1, t2;
   heff_tmp_elems(op, &t1, &t2);
   size_t b = dtype_bytes(op.dtype);
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   return al(t1 * b) + al(t2 * b) + al(mode_elems(op) * b);
+}
+
+// Mixed forms: pack L and R into their workspace slots (once per operator).
+int heff_prepare_mixed(mh_handle h, const mh_heff_op &op, void *ws) {
+  const MixedLayout lay(op);
+  const int64_t ew = op.dtype == MH_C64 ? 2 : 1;
+  char *base = (char *)ws;
+  MH_TRY(pack_operand(h, op.prec, (const double *)op.L, (int64_t)op.Dl * op.wl * op.Dl * ew,
+                      (uint32_t *)(base + lay.lp)));
+  return pack_operand(h, op.prec, (const double *)op.R, (int64_t)op.Dr * op.wr * op.Dr * ew,
+                      (uint32_t *)(base + lay.rp));
+}
+
+// Mixed forms of H_eff.x: x is packed, step 1 writes packed T1, the mode
+// product reads and writes packed words (FP32 arithmetic), step 3 writes the
+// FP64 result.  L, R must already be packed (heff_prepare_mixed).
+static int heff_apply_mixed(mh_handle h, const mh_heff_op &op, const void *x, void *y, void *ws,
+                            const void *Mt_prebuilt) {
+  const MixedLayout lay(op);
+  char *base = (char *)ws;
+  void *T1 = base + lay.t1, *T2 = base + lay.t2, *xp = base + lay.xp;
+  const void *Lp = base + lay.lp, *Rp = base + lay.rp;
+  void *Mt = Mt_prebuilt ? (void *)Mt_prebuilt : (void *)(base + lay.mt);
+  const int64_t ew = op.dtype == MH_C64 ? 2 : 1;
+  MH_TRY(pack_operand(h, op.prec, (const double *)x, (int64_t)heff_vec_len(op) * ew, (uint32_t *)xp));
+  const int Dl = op.Dl, Dr = op.Dr, wl = op.wl, wr = op.wr;
+  if (op.kind == MH_HEFF0) {
+    GemmDesc g1 = gd(Dl * wl, Dr, Dl, Lp, 1, (int64_t)Dl * wl, 0, xp, 1, Dl, T1, 1, (int64_t)Dl * wl);
+    MH_TRY(gemm_packed(h, op.dtype, op.prec, g1, 1));
+    GemmDesc g2 = gd(Dl, Dr, wl * Dr, T1, 1, Dl, 0, Rp, Dr, 1, y, 1, Dl);
+    return gemm_packed(h, op.dtype, op.prec, g2, 0);
+  }
+  const int d1 = op.d1;
+  const int ds = (op.kind == MH_HEFF2) ? d1 * op.d2 : d1;
+  const int ni = wl * ds, nj = ds * wr;
+  if (!Mt_prebuilt)
+    MH_TRY(build_mode_matrix(h, op.dtype, op.kind == MH_HEFF2 ? MODE_TWO : MODE_SITE, wl, op.wm,
+                             wr, d1, op.d2, op.W1, op.W2, Mt));
+  GemmDesc g1 = gd(Dl * wl, ds * Dr, Dl, Lp, 1, (int64_t)Dl * wl, 0, xp, 1, Dl, T1, 1, (int64_t)Dl * wl);
+  MH_TRY(gemm_packed(h, op.dtype, op.prec, g1, 1));
+  MH_TRY(mode_product_packed(h, op.dtype, op.prec, Dl, ni, nj, Dr, T1, (int64_t)Dl * ni, Mt, T2,
+                             (int64_t)Dl * nj));
+  GemmDesc g3 = gd(Dl * ds, Dr, wr * Dr, T2, 1, (int64_t)Dl * ds, 0, Rp, Dr, 1, y, 1, (int64_t)Dl * ds);
+  return gemm_packed(h, op.dtype, op.prec, g3, 0);
 }
 
 int heff_check(mh_handle h, const mh_heff_op &op) {
@@ -236,9 +300,14 @@ int heff_check(mh_handle h, const mh_heff_op &op) {
   return MH_OK;
 }
 
-// Workspace layout: [T1 | T2 | Mt]; Mt may be supplied prebuilt.
+// Workspace layout: [T1 | T2 | Mt]; Mt may be supplied prebuilt.  Mixed
+// precisions use MixedLayout; lr_ready says L and R are already packed.
 int heff_apply_ws(mh_handle h, const mh_heff_op &op, const void *x, void *y,
-                  void *ws, const void *Mt_prebuilt) {
+                  void *ws, const void *Mt_prebuilt, bool lr_ready) {
+  if (op.prec != MH_PREC_F64) {
+    if (!lr_ready) MH_TRY(heff_prepare_mixed(h, op, ws));
+    return heff_apply_mixed(h, op, x, y, ws, Mt_prebuilt);
+  }
   const size_t b = dtype_bytes(op.dtype);
   auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
   size_t t1, t2;
@@ -282,7 +351,7 @@ int heff_apply_ws(mh_handle h, const mh_heff_op &op, const void *x, void *y,
 int heff_apply(mh_handle h, const mh_heff_op &op, const void *x, void *y,
                void *mode_cache) {
   MH_TRY(ensure_ws(h, heff_ws_bytes(op)));
-  return heff_apply_ws(h, op, x, y, h->ws, mode_cache);
+  return heff_apply_ws(h, op, x, y, h->ws, mode_cache, false);
 }
 
 // ------------------------------------------------------- environment updates

@@ -92,6 +92,12 @@ struct GemmDesc {
   int conjA;
 };
 int gemm(mh_handle h, int dtype, int prec, const GemmDesc &g);
+// Mixed precisions on packed operands (4-byte words per real component: FP32,
+// or a BF16 hi/lo pair); c_packed = 1 writes packed words, 0 writes FP64.
+int gemm_packed(mh_handle h, int dtype, int prec, const GemmDesc &g, int c_packed);
+int pack_operand(mh_handle h, int prec, const double *src, int64_t ndouble, uint32_t *dst);
+int mode_product_packed(mh_handle h, int dtype, int prec, int P, int ni, int nj, int Q,
+                        const void *in, int64_t sIq, const void *Mt, void *out, int64_t sOq);
 
 // Mode-matrix kinds used to fold MPO tensors into a (ni x nj) matrix Mt so
 // that out[p, j, q] = sum_i in[p, i, q] Mt[i, j].

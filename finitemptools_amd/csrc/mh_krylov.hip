@@ -20,7 +20,8 @@
 namespace mh {
 
 int heff_apply_ws(mh_handle h, const mh_heff_op &op, const void *x, void *y,
-                  void *ws, const void *Mt_prebuilt);
+                  void *ws, const void *Mt_prebuilt, bool lr_ready);
+int heff_prepare_mixed(mh_handle h, const mh_heff_op &op, void *ws);
 int build_mode_matrix(mh_handle h, int dtype, int kind, int wl, int wm, int wr,
                       int d1, int d2, const void *W1, const void *W2, void *Mt);
 
@@ -422,6 +423,8 @@ struct Krylov {
     Mt = (char *)V + vb;
     MH_TRY(ensure_red(h, lay.total));
     st = h->red;
+    // Mixed forms: the environments are packed once for the whole solve.
+    if (op.prec != MH_PREC_F64) MH_TRY(heff_prepare_mixed(h, op, heff_ws));
     if (op.kind != MH_HEFF0) {
       int kind = op.kind == MH_HEFF2 ? MODE_TWO : MODE_SITE;
       MH_TRY(build_mode_matrix(h, op.dtype, kind, op.wl, op.wm, op.wr, op.d1, op.d2,
@@ -454,7 +457,7 @@ struct Krylov {
   // One Lanczos step i: w = H V_i, alpha_i, w -= ..., (reorth), beta_i, V_{i+1}.
   int step(int i, double btol, int reorth, bool write_next) {
     double *part = st + lay.o_part;
-    MH_TRY(heff_apply_ws(h, op, vec(i), w, heff_ws, op.kind == MH_HEFF0 ? nullptr : Mt));
+    MH_TRY(heff_apply_ws(h, op, vec(i), w, heff_ws, op.kind == MH_HEFF0 ? nullptr : Mt, true));
     int slot;
     prof_begin(h, MH_K_VEC, 0.0, &slot);
     if (cplx) hipLaunchKernelGGL(k_dot_part<true>, dim3(G), dim3(RB), 0, h->stream, vec(i), w, n, part);
