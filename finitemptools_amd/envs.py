@@ -4,6 +4,7 @@
 right environment (0-based sites).  ``environments(mpo, mps)`` builds every
 slot from the right (centre = first site), as ``init_hstorage_right``.
 """
+import numpy as np
 import torch
 
 from . import transfer
@@ -75,3 +76,13 @@ def expectation(mpo, mps):
 
 def mpo_to_device(mpo_host, dtype=torch.float64, device="cuda"):
     return [to_device(W, device, dtype) for W in mpo_host]
+
+
+def local_expectation(mps, op, site):
+    """<psi|op_site|psi> / <psi|psi> for a local d x d operator."""
+    from .models import site_operator_mpo
+    L, d = len(mps), mps[0].shape[1]
+    dt = np.complex128 if mps[0].is_complex() else np.float64
+    num = expectation(mpo_to_device(site_operator_mpo(L, site, op, dt), mps[0].dtype, mps[0].device), mps)
+    den = expectation(mpo_to_device(site_operator_mpo(L, site, np.eye(d), dt), mps[0].dtype, mps[0].device), mps)
+    return num / den

@@ -140,3 +140,14 @@ def hubbard_mpo(L, t=1.0, U=4.0, mu=0.0):
     strings = [(-t, ["cup+F", "cup"]), (-t, ["cdn+F", "cdn"]),
                (-t, ["Fcup", "cup+"]), (-t, ["Fcdn", "cdn+"])]
     return schur_mpo(L, ops, onsite, strings)
+
+
+def site_operator_mpo(L, i, op, dtype=np.float64):
+    """Bond-dimension-1 MPO of a single local operator op (d x d) on site i
+    (identity elsewhere): used for local expectation values."""
+    d = op.shape[0]
+    mpo = []
+    for k in range(L):
+        m = op if k == i else np.eye(d)
+        mpo.append(np.asarray(m, dtype=dtype).reshape(1, d, 1, d))
+    return mpo

@@ -91,3 +91,12 @@ def randommps(L, d, D, seed=0, cplx=False, device="cuda"):
     """Seeded random MPS, uploaded (not canonicalised)."""
     dt = torch.complex128 if cplx else torch.float64
     return FiniteMPS([to_device(a, device, dt) for a in random_site_tensors(L, d, D, seed, cplx)])
+
+
+def prodmps(vectors, dtype=torch.float64, device="cuda"):
+    """Product state from one local vector per site (mps/initializers.jl:89-96)."""
+    ts = []
+    for v in vectors:
+        a = np.asarray(v).reshape(1, -1, 1)
+        ts.append(to_device(a, device, dtype))
+    return FiniteMPS(ts)
