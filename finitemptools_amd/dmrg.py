@@ -131,3 +131,103 @@ def ground_state(state, h, alg=None, nsweeps=None):
     """ground_state!(state, h, alg) (dmrg.jl:240); returns (energies, delta)."""
     alg = alg or DMRG2()
     return compute(environments(h, state, alg.prec), alg, nsweeps)
+
+
+# ------------------------------------------------------------ one-site DMRG
+
+class DMRG1:
+    """dmrg.jl:8-15: one-site DMRG; the bond dimension is raised to D by
+    zero-padding (increase_bond!) at the start of every left sweep."""
+
+    def __init__(self, D=50, maxiter=MAXITER, tol=TOL, eig_maxiter=10, eig_tol=EIG_TOL,
+                 verbosity=0, prec="f64", reorth=None):
+        self.D, self.maxiter, self.tol = D, maxiter, tol
+        self.eig_maxiter, self.eig_tol, self.verbosity = eig_maxiter, eig_tol, verbosity
+        self.prec = prec
+        self.reorth = (prec != "f64") if reorth is None else reorth
+
+
+class DMRG1S:
+    """dmrg.jl:148-156: one-site DMRG with the optimal subspace expansion."""
+
+    def __init__(self, maxiter=MAXITER, tol=TOL, eig_maxiter=10, eig_tol=EIG_TOL,
+                 verbosity=0, trunc=None, prec="f64", reorth=None):
+        self.maxiter, self.tol = maxiter, tol
+        self.eig_maxiter, self.eig_tol, self.verbosity = eig_maxiter, eig_tol, verbosity
+        self.trunc = trunc or MPSTruncation()
+        self.prec = prec
+        self.reorth = (prec != "f64") if reorth is None else reorth
+
+
+def calc_galerkin(m, site, prec="f64"):
+    """dmrg.jl:17-24: || N_L^dag H_eff A || with N_L the left null space of
+    the current site tensor A (right null space when the left one is empty)."""
+    from .derivatives import ac_prime
+    from .expansion import leftnull, rightnull
+    A = m.mps[site]
+    y = ac_prime(A, m.mpo[site], m.hstorage[site], m.hstorage[site + 1], prec=prec)
+    NL = leftnull(A)
+    if NL.shape[1] > 0:
+        return torch.linalg.norm(gemm(NL.t(), as_matrix(y, 2), conjA=True)).item()
+    NR = rightnull(as_matrix(A, 1))
+    return torch.linalg.norm(gemm(as_matrix(y, 1), NR.conj().t().contiguous())).item()
+
+
+def _site_eig(m, alg, site):
+    op = HeffOperator(1, m.hstorage[site], m.hstorage[site + 1], m.mpo[site], prec=alg.prec)
+    ev, vecs, _ = simple_lanczos_solver(op, m.mps[site], "SR", alg.eig_maxiter, alg.eig_tol,
+                                        reorth=alg.reorth)
+    return ev, vecs
+
+
+def _leftsweep1(m, alg, expand):
+    """dmrg.jl:27-49 (DMRG1) and :158-185 (DMRG1S)"""
+    from .expansion import right_expansion
+    from .tdvp import increase_bond, leftorth, _mul_left
+    if not expand:
+        increase_bond(m, alg.D)
+    mps = m.mps
+    energies, delta = [], 0.0
+    for site in range(len(mps) - 1):
+        if expand:
+            right_expansion(m, site, alg.trunc, prec=alg.prec)
+        ev, vecs = _site_eig(m, alg, site)
+        energies.append(ev)
+        delta = max(delta, calc_galerkin(m, site, alg.prec))
+        Q, R = leftorth(vecs)
+        mps[site] = Q
+        mps[site + 1] = _mul_left(R, mps[site + 1])
+        m.updateleft(site)
+    return energies, delta
+
+
+def _rightsweep1(m, alg, expand):
+    """dmrg.jl:52-76 (DMRG1) and :187-215 (DMRG1S)"""
+    from .expansion import left_expansion
+    from .tdvp import rightorth, _mul_right
+    mps = m.mps
+    energies, delta = [], 0.0
+    for site in range(len(mps) - 1, 0, -1):
+        if expand:
+            left_expansion(m, site, alg.trunc, prec=alg.prec)
+        ev, vecs = _site_eig(m, alg, site)
+        energies.append(ev)
+        delta = max(delta, calc_galerkin(m, site, alg.prec))
+        Lf, Q = rightorth(vecs)
+        mps[site] = Q
+        mps[site - 1] = _mul_right(mps[site - 1], Lf)
+        m.updateright(site)
+    return energies, delta
+
+
+_sweep2 = sweep
+
+
+def sweep(m, alg):
+    """sweep! (dmrgexcited.jl:219-223): left sweep then right sweep."""
+    if isinstance(alg, DMRG2):
+        return _sweep2(m, alg)
+    expand = isinstance(alg, DMRG1S)
+    e1, d1 = _leftsweep1(m, alg, expand)
+    e2, d2 = _rightsweep1(m, alg, expand)
+    return e1 + e2, max(d1, d2)

@@ -36,7 +36,19 @@ def is_colmajor(t):
     return True
 
 
+def resolve(t):
+    """Materialise torch's lazy conjugate / negative views: the engine reads
+    raw memory, which for such a view still holds the unconjugated values
+    (``.contiguous()`` keeps the lazy bit on an already-contiguous view)."""
+    if t.is_conj():
+        t = t.resolve_conj()
+    if t.is_neg():
+        t = t.resolve_neg()
+    return t
+
+
 def as_colmajor(t):
+    t = resolve(t)
     if is_colmajor(t):
         return t
     n = t.dim()

@@ -10,7 +10,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import as_colmajor, colmajor_empty, dtype_code, engine, ptr
+from .engine import as_colmajor, colmajor_empty, dtype_code, engine, ptr, resolve
 
 
 def gemm(A, B, alpha=1.0, conjA=False, out=None, beta=0.0):
@@ -20,6 +20,7 @@ def gemm(A, B, alpha=1.0, conjA=False, out=None, beta=0.0):
     K2, N = B.shape
     if K != K2:
         raise _lib.DimensionMismatch(2, f"gemm: {tuple(A.shape)} x {tuple(B.shape)}")
+    A, B = resolve(A), resolve(B)
     if A.stride(0) != 1 and A.stride(1) != 1:
         A = as_colmajor(A)
     if B.stride(0) != 1 and B.stride(1) != 1:
@@ -46,7 +47,7 @@ def as_matrix(t, nrow_legs):
 
 def from_matrix(M, shape):
     """(m x n) matrix (any layout) -> column-major tensor of the given legs."""
-    Mc = M.t().contiguous()  # C-order (n, m) == column-major (m, n)
+    Mc = resolve(M.t().contiguous())  # C-order (n, m) == column-major (m, n)
     return Mc.reshape(tuple(reversed(shape))).permute(*reversed(range(len(shape))))
 
 

@@ -58,6 +58,40 @@ def test_majumdar_ghosh_dimer_energy(gpu):
     assert abs(energies[-1] - (-0.75 * L / 2)) < 1e-8
 
 
+@pytest.mark.parametrize("algname", ["DMRG1", "DMRG1S"])
+def test_one_site_dmrg_reaches_ed(gpu, algname):
+    """DMRG1 (zero-padded bond dimension) and DMRG1S (optimal subspace
+    expansion) against exact diagonalisation, as the reference tests DMRG1S."""
+    from finitemptools_amd import dmrg
+    from finitemptools_amd.envs import mpo_to_device
+    from finitemptools_amd.mps import randommps
+    from finitemptools_amd.truncation import MPSTruncation
+    L = 12
+    psi = randommps(L, 2, 4, seed=2).right_canonicalize()
+    if algname == "DMRG1":
+        alg = dmrg.DMRG1(D=32)
+    else:
+        alg = dmrg.DMRG1S(trunc=MPSTruncation(D=32, eps=1e-10))
+    energies, delta = dmrg.ground_state(psi, mpo_to_device(models.heisenberg_xxz_mpo(L)), alg, nsweeps=8)
+    assert abs(energies[-1] - ED["xxz_L12_J1_D1"]) < 1e-8
+    assert delta < 1e-4  # Galerkin error of the converged state
+
+
+def test_complex_canonical_form_preserves_state(gpu):
+    """Right-canonicalising a complex MPS on the device leaves the state
+    unchanged and sites 2..L right-isometric (exercises conjugated factors)."""
+    from finitemptools_amd.mps import randommps
+    L = 6
+    psi = randommps(L, 2, 4, seed=9, cplx=True)
+    before = OD.to_dense(psi.to_host())
+    psi.right_canonicalize(normalize=False)
+    after = OD.to_dense(psi.to_host())
+    assert np.allclose(after, before, rtol=1e-12, atol=1e-12 * np.abs(before).max())
+    for A in psi.to_host()[1:]:
+        M = A.reshape(A.shape[0], -1, order="F")
+        assert np.allclose(M @ M.conj().T, np.eye(A.shape[0]), atol=1e-12)
+
+
 def test_xxz_L20_D64_four_sweeps(gpu):
     """The reference CPU run configuration: XXZ (J = Delta = 1), L = 20, D = 64,
     four sweeps from a seeded random right-canonical MPS."""

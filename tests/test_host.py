@@ -28,3 +28,18 @@ def test_mpo_strict_boundaries():
         assert mpo[0].shape[0] == 1 and mpo[-1].shape[2] == 1
         for a, b in zip(mpo[:-1], mpo[1:]):
             assert a.shape[2] == b.shape[0]
+
+
+def test_conjugate_views_are_materialised_before_the_engine():
+    """torch's conj() is a lazy view; .contiguous() keeps the lazy bit on an
+    already-contiguous view, so raw pointers would see unconjugated values."""
+    import torch
+    from finitemptools_amd.engine import as_colmajor
+    from finitemptools_amd.linalg import from_matrix
+    r = torch.tensor([[1 + 2j, 3 + 4j], [5 + 6j, 7 + 8j]], dtype=torch.complex128)
+    for t in (as_colmajor(r.conj()), from_matrix(r.conj().t(), (2, 2))):
+        assert not t.is_conj()
+        raw = torch.view_as_real(t)  # would raise on a conj view
+        assert torch.equal(torch.view_as_complex(raw), t)
+    t = from_matrix(r.conj().t(), (2, 2))
+    assert torch.equal(t, r.conj().t())
