@@ -12,6 +12,7 @@ SOURCES = [
     "mh_gemm_mixed.hip",
     "mh_heff.hip",
     "mh_krylov.hip",
+    "mh_blocksparse.hip",
 ]
 ARCH = "gfx950"
 
@@ -33,6 +34,7 @@ HIPCC_FLAGS = [
 EXTRA_FLAGS = {
     "mh_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
     "mh_gemm_mixed.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+    "mh_blocksparse.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
 }
 
 

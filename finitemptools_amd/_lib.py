@@ -23,6 +23,7 @@ MH_F64, MH_C64 = 0, 1
 PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
 MH_HEFF0, MH_HEFF1, MH_HEFF2 = 0, 1, 2
 MH_K_GEMM, MH_K_WSTAGE, MH_K_VEC = 0, 1, 2
+MH_BS_X, MH_BS_Y, MH_BS_P1, MH_BS_P2, MH_BS_T1, MH_BS_T2 = range(6)
 
 
 class HeffOp(ctypes.Structure):
@@ -71,6 +72,16 @@ SIGNATURES = {
     "mh_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_int64,
                         c_int64, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                         c_int64, c_double, c_double, c_double, c_double]),
+    "mh_bs_plan_create": (c_int, [c_void_p, c_int, P(c_int64), P(c_void_p)]),
+    "mh_bs_plan_add_gemm": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
+                                    c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "mh_bs_plan_add_mode": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
+                                    c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "mh_bs_plan_flops": (c_int, [c_void_p, P(c_double)]),
+    "mh_bs_plan_destroy": (c_int, [c_void_p, c_void_p]),
+    "mh_bs_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mh_bs_lanczos_eig": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                  c_double, c_int, P(c_double), c_void_p, P(c_int), P(c_int)]),
     "mh_profile_enable": (c_int, [c_void_p, c_int]),
     "mh_profile_read": (c_int, [c_void_p, c_int, P(c_double), P(c_int64), P(c_double)]),
     "mh_profile_reset": (c_int, [c_void_p]),

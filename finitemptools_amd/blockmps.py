@@ -1,0 +1,507 @@
+"""Charge-conserving (block-sparse) finite MPS, environments and two-site DMRG
+on the device.
+
+Restates, for U(1)-type graded spaces, what FiniteMPTools does through
+TensorKit: ``prodmps`` / ``randommps`` (mps/initializers.jl:34-73), right
+canonical form (mps/orth.jl:40-49), ``ExpectationCache`` (envs/finiteenv.jl),
+the DMRG2 sweeps (algorithms/dmrg.jl:89-144) with ``stable_tsvd`` truncation
+applied globally across sectors (auxiliary/truncation.jl:27-47).
+
+Contractions (two-site tensor, H_eff, environment updates) run as grouped
+plans (blocksparse.py); the Lanczos recurrence stays on the device; the
+sector SVDs and QRs are FP64 torch.linalg calls on the device.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import blocksparse as bs
+from .blocksparse import (BlockEnv, BlockLayout, BlockTensor, EnvLayout, Space, qadd,
+                          qsub, qzero)
+from .truncation import MPSTruncation
+
+
+# ------------------------------------------------------------------ models
+
+def hubbard_charges():
+    """(n_up, n_dn) of the four local states of the dense spin-1/2 fermion
+    basis (models.spinal_fermion_ops_dense): read off the number operators."""
+    from .models import spinal_fermion_ops_dense
+    f = spinal_fermion_ops_dense()
+    return [(int(round(f["nup"][s, s])), int(round(f["ndn"][s, s]))) for s in range(4)]
+
+
+def sz_charges():
+    """2 S^z of the spin-1/2 basis (index 0 = up, models.spin_half_matrices)."""
+    from .models import spin_half_matrices
+    z = spin_half_matrices()["z"].real
+    return [(int(round(z[s, s])),) for s in range(2)]
+
+
+class BlockMPO:
+    """Host MPO tensors with their channel charges (one list per MPO bond)."""
+
+    def __init__(self, mpo, phys):
+        self.W = [np.asarray(W, dtype=np.float64) for W in mpo]
+        self.phys = [tuple(q) for q in phys]
+        self.qw = bs.mpo_channel_charges(self.W, self.phys)
+
+    def __len__(self):
+        return len(self.W)
+
+    def __getitem__(self, i):
+        return self.W[i]
+
+
+# ------------------------------------------------------------------ states
+
+class BlockMPS:
+    def __init__(self, sites, phys):
+        self.sites = list(sites)
+        self.phys = [tuple(q) for q in phys]
+        self.s = [None] * (len(sites) + 1)
+
+    def __len__(self):
+        return len(self.sites)
+
+    def __getitem__(self, i):
+        return self.sites[i]
+
+    def __setitem__(self, i, v):
+        self.sites[i] = v
+
+    def bond_space(self, i):
+        return self.sites[i].lspace if i < len(self.sites) else self.sites[-1].rspace
+
+    def bond_dimension(self):
+        return max(t.rspace.total for t in self.sites)
+
+    def total_charge(self):
+        (q, _), = self.sites[-1].rspace.sectors
+        return q
+
+    def to_dense(self):
+        """Full state vector (small L only), site 0 slowest as in the oracle."""
+        v = self.sites[0].to_dense()  # (1, d, D1)
+        v = v.reshape(-1, v.shape[-1])
+        for t in self.sites[1:]:
+            A = t.to_dense()
+            v = np.einsum("xa,asb->xsb", v, A).reshape(-1, A.shape[-1])
+        return v.reshape(-1)
+
+    def right_canonicalize(self, normalize=True):
+        """Sites 1..L-1 right-isometric by sector-wise LQ (mps/orth.jl:40-49)."""
+        for i in range(len(self) - 1, 0, -1):
+            A = self.sites[i]
+            lay = A.layout
+            new_dims, Ls, Qs = {}, {}, {}
+            for qa, Da in lay.lspace.sectors:
+                if qa not in lay.group:
+                    continue
+                G = A.group_matrix(qa)                 # Da x ncols
+                Q, R = torch.linalg.qr(G.t(), mode="reduced")   # ncols x k, k x Da
+                new_dims[qa] = Q.shape[1]
+                Ls[qa] = R.t()                         # Da x k
+                Qs[qa] = Q
+            ls = Space(new_dims.items())
+            nl = BlockLayout(ls, lay.phys, 1, lay.rspace)
+            data = torch.empty(nl.size, dtype=torch.float64, device=A.data.device)
+            for qa in ls.dim:
+                g0, ncols = nl.group[qa]
+                data[g0:g0 + ls.dim[qa] * ncols] = Qs[qa].contiguous().reshape(-1)
+            self.sites[i] = BlockTensor(nl, data)
+            # absorb L into site i-1 (blocks ending in qa)
+            P = self.sites[i - 1]
+            pl = BlockLayout(P.lspace, P.layout.phys, 1, ls)
+            pdata = torch.zeros(pl.size, dtype=torch.float64, device=P.data.device)
+            for b in pl.blocks:
+                ob = P.layout.index[(b.qa, b.p)]
+                blk = P.block(ob) @ Ls[b.qb]
+                pdata[b.off:b.off + b.rows * b.cols] = blk.t().reshape(-1)
+            self.sites[i - 1] = BlockTensor(pl, pdata)
+        if normalize:
+            A = self.sites[0]
+            A.data /= torch.linalg.vector_norm(A.data)
+        return self
+
+
+def _prune(spaces_left, spaces_right):
+    """Sectors present in both the left-reachable and right-reachable sets,
+    with the smaller of the two dimensions."""
+    out = []
+    for sl, sr in zip(spaces_left, spaces_right):
+        out.append(Space([(q, min(d, sr.dim[q])) for q, d in sl.sectors if q in sr]))
+    return out
+
+
+def max_virtual_spaces(L, phys, total, cap=None):
+    """Largest bond spaces compatible with the physical charges and a total
+    charge: sector q of bond i has the number of ways to reach it from the
+    left and to complete it to ``total`` on the right (capped at ``cap``)."""
+    phys = [tuple(q) for q in phys]
+    n = len(phys[0])
+    left = [Space([(qzero(n), 1)])]
+    for i in range(L):
+        acc = {}
+        for q, d in left[-1].sectors:
+            for p in phys:
+                qq = qadd(q, p)
+                acc[qq] = acc.get(qq, 0) + d
+        left.append(Space([(q, min(d, cap) if cap else d) for q, d in acc.items()]))
+    right = [Space([(tuple(total), 1)])]
+    for i in range(L):
+        acc = {}
+        for q, d in right[0].sectors:
+            for p in phys:
+                qq = qsub(q, p)
+                acc[qq] = acc.get(qq, 0) + d
+        right.insert(0, Space([(q, min(d, cap) if cap else d) for q, d in acc.items()]))
+    return _prune(left, right)
+
+
+def prodmps(phys, states, device="cuda"):
+    """Product state with local basis states ``states[i]`` (prodmps for
+    graded spaces, initializers.jl:34-56): every bond carries one sector of
+    dimension 1 and every site tensor one block equal to 1."""
+    phys = [tuple(q) for q in phys]
+    n = len(phys[0])
+    q = qzero(n)
+    sites = []
+    for s in states:
+        qn = qadd(q, phys[s])
+        lay = BlockLayout(Space([(q, 1)]), phys, 1, Space([(qn, 1)]))
+        data = torch.zeros(lay.size, dtype=torch.float64, device=device)
+        data[lay.index[(q, (s,))].off] = 1.0
+        sites.append(BlockTensor(lay, data))
+        q = qn
+    return BlockMPS(sites, phys)
+
+
+def randommps(L, phys, total, D, seed=0, device="cuda"):
+    """Random MPS in the sector ``total`` with bond dimension <= D
+    (randommps(T, physpaces; sector, D), initializers.jl:58-73): site tensors
+    of iid N(0,1) blocks on the largest allowed spaces, each reduced to D by a
+    truncated SVD from the left (numpy PCG64 stream)."""
+    gen = np.random.Generator(np.random.PCG64(seed))
+    vmax = max_virtual_spaces(L, phys, total)
+    trunc = MPSTruncation(D=D, eps=0.0)
+    sites = []
+    left = vmax[0]
+    for i in range(L):
+        acc = {}
+        for q, d in left.sectors:
+            for p in phys:
+                qq = qadd(q, tuple(p))
+                if qq in vmax[i + 1]:
+                    acc[qq] = acc.get(qq, 0) + d
+        right = Space([(q, min(d, vmax[i + 1].dim[q])) for q, d in acc.items()])
+        lay = BlockLayout(left, phys, 1, right)
+        A = BlockTensor.randn(lay, gen, device)
+        if i < L - 1:
+            U, S, _ = sector_svd_left(A, trunc)
+            A = U
+        sites.append(A)
+        left = A.rspace
+    return BlockMPS(sites, phys)
+
+
+def sector_svd_left(A, trunc):
+    """Left-isometric factor of a one-site tensor A[(a,s), b] by sector SVDs
+    with global truncation (used to build random states)."""
+    lay = A.layout
+    mats, rows = {}, {}
+    for qb, Db in lay.rspace.sectors:
+        blocks = lay.by_right.get(qb, [])
+        if not blocks:
+            continue
+        mats[qb] = torch.cat([A.block(b) for b in blocks], dim=0)
+        rows[qb] = blocks
+    svd = {}
+    for qb, M in mats.items():
+        svd[qb] = torch.linalg.svd(M, full_matrices=False)
+    keep = _global_keep(svd, trunc)
+    ns = Space([(qb, k) for qb, k in keep.items()])
+    nl = BlockLayout(lay.lspace, lay.phys, 1, ns)
+    data = torch.zeros(nl.size, dtype=torch.float64, device=A.data.device)
+    for qb, k in keep.items():
+        U = svd[qb][0][:, :k]
+        r = 0
+        for b in rows[qb]:
+            nb = nl.index[(b.qa, b.p)]
+            data[nb.off:nb.off + nb.rows * nb.cols] = U[r:r + b.rows].t().reshape(-1)
+            r += b.rows
+    return BlockTensor(nl, data), None, None
+
+
+def _global_keep(svd, trunc):
+    """Kept count per sector when the truncation acts on all singular values
+    together (truncation.jl:27-47 on the sector-merged spectrum)."""
+    keys = list(svd.keys())
+    vals = [svd[q][1].detach().cpu().numpy() for q in keys]
+    allv = np.concatenate(vals) if vals else np.zeros(0)
+    owner = np.concatenate([np.full(len(v), j) for j, v in enumerate(vals)]) if vals else np.zeros(0, int)
+    order = np.argsort(-allv, kind="stable")
+    k, _ = trunc.truncate(allv[order])
+    k = max(k, 1)
+    counts = np.bincount(owner[order[:k]], minlength=len(keys))
+    return {q: int(c) for q, c in zip(keys, counts) if c > 0}
+
+
+# ------------------------------------------------------------------ environments
+
+def boundary_env(space, nq, device):
+    (q, d), = space.sectors
+    lay = EnvLayout(space, [qzero(nq)])
+    return BlockEnv(lay, torch.ones(lay.size, dtype=torch.float64, device=device))
+
+
+class BlockExpectationCache:
+    """ExpectationCache (envs/finiteenv.jl:3-93) for block-sparse states:
+    hstorage[i] is the environment at bond i; plans are built per update."""
+
+    def __init__(self, mpo, mps, prec="f64"):
+        if len(mpo) != len(mps):
+            raise ValueError("MPO and MPS lengths differ")
+        self.mpo, self.mps, self.prec = mpo, mps, prec
+        self.hstorage = [None] * (len(mps) + 1)
+        self.env_time = 0.0
+        self.init_hstorage()
+
+    @property
+    def state(self):
+        return self.mps
+
+    @property
+    def h(self):
+        return self.mpo
+
+    @property
+    def env(self):
+        return self.hstorage
+
+    def init_hstorage(self, center=0):
+        L = len(self.mps)
+        dev = self.mps[0].data.device
+        nq = len(self.mps.phys[0])
+        self.hstorage[0] = boundary_env(self.mps[0].lspace, nq, dev)
+        self.hstorage[L] = boundary_env(self.mps[L - 1].rspace, nq, dev)
+        for i in range(L - 1, center, -1):
+            self.updateright(i)
+        for i in range(center):
+            self.updateleft(i)
+
+    def updateleft(self, site):
+        """hstorage[site+1] = updateleft(hstorage[site], A, W, A)"""
+        t0 = time.perf_counter()
+        A = self.mps[site]
+        old = self.hstorage[site]
+        new_lay = EnvLayout(A.rspace, self.mpo.qw[site + 1])
+        tab = bs.plan_updateleft(A.layout, old.layout, self.mpo[site], new_lay)
+        out = torch.zeros(new_lay.size, dtype=torch.float64, device=A.data.device)
+        bs.Plan(tab, self.prec).apply(A.data, old.data, A.data, out)
+        self.hstorage[site + 1] = BlockEnv(new_lay, out)
+        self.env_time += time.perf_counter() - t0
+
+    def updateright(self, site):
+        """hstorage[site] = updateright(hstorage[site+1], A, W, A)"""
+        t0 = time.perf_counter()
+        A = self.mps[site]
+        old = self.hstorage[site + 1]
+        new_lay = EnvLayout(A.lspace, self.mpo.qw[site])
+        tab = bs.plan_updateright(A.layout, old.layout, self.mpo[site], new_lay)
+        out = torch.zeros(new_lay.size, dtype=torch.float64, device=A.data.device)
+        bs.Plan(tab, self.prec).apply(A.data, old.data, A.data, out)
+        self.hstorage[site] = BlockEnv(new_lay, out)
+        self.env_time += time.perf_counter() - t0
+
+
+def expectation(mpo, mps):
+    """<psi|H|psi> by right-to-left environment updates."""
+    c = BlockExpectationCache(mpo, mps)
+    L = len(mps)
+    c.updateright(0)
+    return c.hstorage[0].data[0].item()
+
+
+# ------------------------------------------------------------------ two-site step
+
+class TwoSiteMap:
+    """Index maps between a two-site tensor's blocks and its matrices per
+    middle charge c: M_c[(qa, s1, i), (s2, qb, j)] = x[(qa, s1, s2, qb)][i, j]."""
+
+    def __init__(self, xl, device):
+        self.xl = xl
+        Sl, Sr, phys = xl.lspace, xl.rspace, xl.phys
+        rows, cols = {}, {}
+        for qa, Da in Sl.sectors:
+            for s1 in range(xl.d):
+                c = qadd(qa, phys[s1])
+                rows.setdefault(c, []).append((qa, s1, Da))
+        for qb, Db in Sr.sectors:
+            for s2 in range(xl.d):
+                c = qsub(qb, phys[s2])
+                cols.setdefault(c, []).append((s2, qb, Db))
+        self.charges = [c for c in sorted(rows) if c in cols]
+        self.rows = {c: rows[c] for c in self.charges}
+        self.cols = {c: cols[c] for c in self.charges}
+        self.idx = {}
+        for c in self.charges:
+            R = sum(r[2] for r in self.rows[c])
+            C = sum(r[2] for r in self.cols[c])
+            idx = np.empty((R, C), dtype=np.int64)
+            ro = 0
+            for qa, s1, Da in self.rows[c]:
+                co = 0
+                for s2, qb, Db in self.cols[c]:
+                    b = xl.index[(qa, (s1, s2))]
+                    idx[ro:ro + Da, co:co + Db] = b.off + np.arange(Da)[:, None] + Da * np.arange(Db)[None, :]
+                    co += Db
+                ro += Da
+            self.idx[c] = torch.as_tensor(idx, device=device)
+
+    def matrices(self, flat):
+        return {c: flat[self.idx[c]] for c in self.charges}
+
+
+def two_site_layout(A1, A2):
+    return BlockLayout(A1.lspace, A1.layout.phys, 2, A2.rspace)
+
+
+def two_site(A1, A2, xl=None):
+    """x = A1 * A2 over the middle bond (dmrg.jl:98) by one grouped launch."""
+    xl = xl or two_site_layout(A1, A2)
+    tab = bs.plan_two_site(A1.layout, A2.layout, xl)
+    out = torch.zeros(xl.size, dtype=torch.float64, device=A1.data.device)
+    bs.Plan(tab, "f64").apply(A1.data, A2.data, None, out)
+    return BlockTensor(xl, out)
+
+
+def split_two_site(tmap, mats, trunc, absorb):
+    """Sector SVDs of the two-site matrices, global truncation, normalised
+    singular values; returns (A1, A2, s, svd) with s absorbed into A2
+    ("right") or A1 ("left")."""
+    xl = tmap.xl
+    svd = {}
+    for c in tmap.charges:
+        M = mats[c]
+        try:
+            U, S, Vh = torch.linalg.svd(M, full_matrices=False)
+        except RuntimeError:
+            U, S, Vh = torch.linalg.svd(M, full_matrices=False, driver="gesvd")
+        svd[c] = (U, S, Vh)
+    keep = _global_keep(svd, trunc)
+    norm = float(np.sqrt(sum(float((svd[c][1][:k] ** 2).sum()) for c, k in keep.items())))
+    mid = Space(keep.items())
+    l1 = BlockLayout(xl.lspace, xl.phys, 1, mid)
+    l2 = BlockLayout(mid, xl.phys, 1, xl.rspace)
+    dev = mats[tmap.charges[0]].device
+    d1 = torch.zeros(l1.size, dtype=torch.float64, device=dev)
+    d2 = torch.zeros(l2.size, dtype=torch.float64, device=dev)
+    svals = {}
+    for c, k in keep.items():
+        U, S, Vh = svd[c]
+        s = S[:k] / norm
+        svals[c] = s
+        u = U[:, :k]
+        v = Vh[:k]
+        if absorb == "right":
+            v = v * s.reshape(-1, 1)
+        else:
+            u = u * s.reshape(1, -1)
+        ro = 0
+        for qa, s1, Da in tmap.rows[c]:
+            b = l1.index[(qa, (s1,))]
+            d1[b.off:b.off + Da * k] = u[ro:ro + Da].t().reshape(-1)
+            ro += Da
+        co = 0
+        for s2, qb, Db in tmap.cols[c]:
+            b = l2.index[(c, (s2,))]
+            d2[b.off:b.off + k * Db] = v[:, co:co + Db].t().reshape(-1)
+            co += Db
+    return BlockTensor(l1, d1), BlockTensor(l2, d2), svals, svd, keep, norm
+
+
+def _bond_step(m, alg, site, absorb):
+    mps, hs = m.mps, m.hstorage
+    A1, A2 = mps[site], mps[site + 1]
+    x = two_site(A1, A2)
+    L, R = hs[site], hs[site + 2]
+    t0 = time.perf_counter()
+    tab = bs.plan_heff2(x.layout, L.layout, R.layout, m.mpo[site], m.mpo[site + 1])
+    plan = bs.Plan(tab, alg.prec)
+    t1 = time.perf_counter()
+    ev, vec, info, nit = plan.lanczos(L.data, R.data, x.data, alg.eig_maxiter, alg.eig_tol,
+                                      reorth=alg.reorth)
+    t2 = time.perf_counter()
+    tmap = TwoSiteMap(x.layout, x.data.device)
+    mats = tmap.matrices(vec)
+    n1, n2, svals, svd, keep, norm = split_two_site(tmap, mats, alg.trunc, absorb)
+    # err_1 = <u s v, twositemps> (dmrg.jl:109-110) over the kept sectors
+    old = tmap.matrices(x.data)
+    e = 0.0
+    for c, k in keep.items():
+        U, S, Vh = svd[c]
+        usv = (U[:, :k] * (S[:k] / norm).reshape(1, -1)) @ Vh[:k]
+        e += float((usv * old[c]).sum())
+    t3 = time.perf_counter()
+    st = alg.stats
+    st["plan_s"] = st.get("plan_s", 0.0) + (t1 - t0)
+    st["lanczos_s"] += t2 - t1
+    st["svd_s"] += t3 - t2
+    st["matvec_flops"] += tab.flops() * nit
+    st["bonds"] += 1
+    return ev, n1, n2, svals, abs(1.0 - abs(e))
+
+
+def leftsweep(m, alg):
+    """dmrg.jl:89-115 on block-sparse tensors"""
+    mps = m.mps
+    energies, delta = [], 0.0
+    for site in range(len(mps) - 2):
+        ev, n1, n2, svals, err = _bond_step(m, alg, site, "right")
+        energies.append(ev)
+        mps[site], mps[site + 1] = n1, n2
+        delta = max(delta, err)
+        t0 = time.perf_counter()
+        m.updateleft(site)
+        alg.stats["env_s"] += time.perf_counter() - t0
+    return energies, delta
+
+
+def rightsweep(m, alg):
+    """dmrg.jl:117-144 on block-sparse tensors"""
+    mps = m.mps
+    energies, delta = [], 0.0
+    for site in range(len(mps) - 2, -1, -1):
+        ev, n1, n2, svals, err = _bond_step(m, alg, site, "left")
+        energies.append(ev)
+        mps[site], mps[site + 1] = n1, n2
+        mps.s[site + 1] = svals
+        delta = max(delta, err)
+        t0 = time.perf_counter()
+        m.updateright(site + 1)
+        alg.stats["env_s"] += time.perf_counter() - t0
+    return energies, delta
+
+
+def sweep(m, alg):
+    e1, d1 = leftsweep(m, alg)
+    e2, d2 = rightsweep(m, alg)
+    return e1 + e2, max(d1, d2)
+
+
+def compute(m, alg, nsweeps=None):
+    all_e, it, delta = [], 0, 2 * alg.tol
+    maxit = alg.maxiter if nsweeps is None else nsweeps
+    while it < maxit and delta > alg.tol:
+        e, delta = sweep(m, alg)
+        all_e += e
+        it += 1
+    return all_e, delta
+
+
+def ground_state(state, mpo, alg, nsweeps=None):
+    """ground_state!(state, h, alg) for a block-sparse state: returns
+    (energies, delta) and leaves the optimised state in ``state``."""
+    return compute(BlockExpectationCache(mpo, state, alg.prec), alg, nsweeps)

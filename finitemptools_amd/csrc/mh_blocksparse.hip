@@ -1,0 +1,759 @@
+// Block-sparse (abelian-symmetric) contractions: grouped GEMM and block
+// mode product, driven by a plan built on the host from the charge sectors.
+//
+// A symmetric tensor is stored as a list of dense column-major blocks in one
+// buffer.  A contraction such as the two-site H_eff becomes three stages:
+//   1. grouped GEMM         T1 blocks = environment blocks x vector blocks
+//   2. block mode product   T2 blocks = sum_c coef_c * T1 block_c
+//                           (the MPO entries that conserve the charges)
+//   3. grouped GEMM         output blocks = T2 blocks x environment blocks
+// A GEMM task is one output block C (M x N) = sum over its K-segments s of
+// opA_s (M x K_s) * opB_s (K_s x N), every operand addressed by an offset and
+// a leading dimension inside one of six buffers (vector in/out, two
+// environments or site tensors, two intermediates).  All 64x64 tiles of all
+// tasks of a stage run in ONE launch, so sectors of uneven size share the
+// machine; tiles are laid out so that each XCD works on a contiguous range of
+// the task list (L2 reuse of the operand rows a task shares).
+//
+// Precisions: FP64 (v_mfma_f64_16x16x4f64), and the mixed forms FP32 and
+// split-BF16 (hi*hi + hi*lo + lo*hi, v_mfma_f32_16x16x32_bf16) with FP32
+// accumulation.  The mixed kernels read the FP64 operands and round them while
+// staging to LDS, so the plan, the buffers and the Krylov vectors are the same
+// for every precision; only the matrix-core arithmetic changes.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mh_gemm_common.h"
+
+namespace mh {
+
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+struct GTask {      // one output block
+  int M, N, seg0, nseg;
+  int64_t c_off;    // element offset of C in its buffer
+  int ldc, pad;
+};
+struct GSeg {       // one K-segment of a task
+  int64_t a_off, b_off;
+  int K, lda, ldb, pad;
+};
+struct GTile {      // task < 0: padding entry of the XCD-ordered grid
+  int task, m0, n0, pad;
+};
+struct MChunk {     // up to 1024 consecutive elements of one destination block
+  int64_t dst_off;
+  int rows, ld_dst;   // destination block rows and leading dimension
+  int e0, ne;         // element range (column-major index within the block)
+  int c0, nc;         // contributions
+};
+struct MContrib {
+  int64_t src_off;
+  int ld_src, pad;
+  double coef;
+};
+
+constexpr int NSRC = 6;
+
+struct BsStage {
+  int kind;                  // 0 grouped GEMM, 1 block mode product
+  int a_src, b_src, c_src;   // mode: a_src = input, c_src = output
+  int amc, bkc;
+  int ntile, nchunk;
+  GTask *tasks;
+  GSeg *segs;
+  GTile *tiles;
+  MChunk *chunks;
+  MContrib *contribs;
+  double flops;
+};
+
+}  // namespace mh
+
+struct mh_bs_plan_s {
+  int prec = MH_PREC_F64;
+  int64_t size[mh::NSRC] = {0, 0, 0, 0, 0, 0};  // elements of each buffer
+  std::vector<mh::BsStage> stages;
+  std::vector<void *> allocs;
+  double flops = 0;
+};
+
+namespace mh {
+
+// ------------------------------------------------------------------ FP64
+// Staging as in the strided FP64 GEMM: an operand contiguous along its row
+// index r (A m-contiguous, B n-contiguous) goes to LDS as [k][r] (pitch
+// BR + 4), one contiguous along k as [r][k] (pitch BK + 2); each thread moves
+// pairs adjacent along the contiguous dimension.  MFMA sub-step s of lane
+// group lk uses k = 4*lk + s inside each 16-deep slice.
+template <int BR, int BK, bool RMAJ>
+struct GStage {
+  static constexpr int PITCH = RMAJ ? BR + 4 : BK + 2;
+  static constexpr int SIZE = RMAJ ? BK * PITCH : BR * PITCH;
+  static constexpr int NPAIR = BR * BK / 512;
+  __device__ static void coord(int t, int i, int &r, int &k) {
+    const int e = t + 256 * i;
+    if (RMAJ) {
+      r = 2 * (e % (BR / 2));
+      k = e / (BR / 2);
+    } else {
+      k = 2 * (e % (BK / 2));
+      r = e / (BK / 2);
+    }
+  }
+};
+
+template <int BM, int BN, bool AMC, bool BKC>
+__global__ void __launch_bounds__(256)
+    k_ggemm(const GTile *__restrict__ tiles, const GTask *__restrict__ tasks,
+            const GSeg *__restrict__ segs, const double *__restrict__ Abase,
+            const double *__restrict__ Bbase, double *__restrict__ Cbase) {
+  constexpr int BK = 16;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  using SA = GStage<BM, BK, AMC>;
+  using SB = GStage<BN, BK, !BKC>;
+  __shared__ __align__(16) double sA[2][SA::SIZE];
+  __shared__ __align__(16) double sB[2][SB::SIZE];
+  const GTile tl = tiles[blockIdx.x];
+  if (tl.task < 0) return;
+  const GTask t = tasks[tl.task];
+  const int m0 = tl.m0, n0 = tl.n0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  d4_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  for (int sg = 0; sg < t.nseg; ++sg) {
+    const GSeg g = segs[t.seg0 + sg];
+    const double *A = Abase + g.a_off;
+    const double *B = Bbase + g.b_off;
+    const int64_t a_sr = AMC ? 1 : g.lda, a_sk = AMC ? g.lda : 1;
+    const int64_t b_sr = BKC ? g.ldb : 1, b_sk = BKC ? 1 : g.ldb;
+    const int K = g.K;
+    // rows clamped into the block (their values are masked at the LDS store)
+    int64_t aoff[SA::NPAIR][2], boff[SB::NPAIR][2];
+    bool aok[SA::NPAIR][2], bok[SB::NPAIR][2];
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int gr = m0 + r + (AMC ? e : 0);
+        aok[i][e] = gr < t.M;
+        aoff[i][e] = (int64_t)min(gr, t.M - 1) * a_sr;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int gr = n0 + r + (BKC ? 0 : e);
+        bok[i][e] = gr < t.N;
+        boff[i][e] = (int64_t)min(gr, t.N - 1) * b_sr;
+      }
+    }
+    double ra[SA::NPAIR][2], rb[SB::NPAIR][2];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int i = 0; i < SA::NPAIR; ++i) {
+        int r, k;
+        SA::coord(tid, i, r, k);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int kc = min(k0 + k + (AMC ? 0 : e), K - 1);
+          ra[i][e] = A[aoff[i][e] + (int64_t)kc * a_sk];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < SB::NPAIR; ++i) {
+        int r, k;
+        SB::coord(tid, i, r, k);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int kc = min(k0 + k + (BKC ? e : 0), K - 1);
+          rb[i][e] = B[boff[i][e] + (int64_t)kc * b_sk];
+        }
+      }
+    };
+    auto store = [&](int buf, int k0) {
+#pragma unroll
+      for (int i = 0; i < SA::NPAIR; ++i) {
+        int r, k;
+        SA::coord(tid, i, r, k);
+        const bool ok0 = aok[i][0] && (k0 + k < K);
+        const bool ok1 = aok[i][1] && (k0 + k + (AMC ? 0 : 1) < K);
+        const int o = AMC ? k * SA::PITCH + r : r * SA::PITCH + k;
+        double2 v;
+        v.x = ok0 ? ra[i][0] : 0.0;
+        v.y = ok1 ? ra[i][1] : 0.0;
+        *reinterpret_cast<double2 *>(&sA[buf][o]) = v;
+      }
+#pragma unroll
+      for (int i = 0; i < SB::NPAIR; ++i) {
+        int r, k;
+        SB::coord(tid, i, r, k);
+        const bool ok0 = bok[i][0] && (k0 + k < K);
+        const bool ok1 = bok[i][1] && (k0 + k + (BKC ? 1 : 0) < K);
+        const int o = BKC ? r * SB::PITCH + k : k * SB::PITCH + r;
+        double2 v;
+        v.x = ok0 ? rb[i][0] : 0.0;
+        v.y = ok1 ? rb[i][1] : 0.0;
+        *reinterpret_cast<double2 *>(&sB[buf][o]) = v;
+      }
+    };
+    auto compute = [&](int buf) {
+      double a[TM][4], b[TN][4];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int r = wm * WTM + i * 16 + lr;
+        if (AMC) {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) a[i][s] = sA[buf][(4 * lk + s) * SA::PITCH + r];
+        } else {
+          const double2 *p = reinterpret_cast<const double2 *>(&sA[buf][r * SA::PITCH + 4 * lk]);
+          const double2 u = p[0], v = p[1];
+          a[i][0] = u.x; a[i][1] = u.y; a[i][2] = v.x; a[i][3] = v.y;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int r = wn * WTN + j * 16 + lr;
+        if (!BKC) {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) b[j][s] = sB[buf][(4 * lk + s) * SB::PITCH + r];
+        } else {
+          const double2 *p = reinterpret_cast<const double2 *>(&sB[buf][r * SB::PITCH + 4 * lk]);
+          const double2 u = p[0], v = p[1];
+          b[j][0] = u.x; b[j][1] = u.y; b[j][2] = v.x; b[j][3] = v.y;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+    };
+    const int nk = (K + BK - 1) / BK;
+    __syncthreads();  // the previous segment's last compute is done with LDS
+    load(0);
+    store(0, 0);
+    __syncthreads();
+    for (int kt = 0; kt + 1 < nk; ++kt) {
+      load((kt + 1) * BK);
+      compute(kt & 1);
+      store((kt + 1) & 1, (kt + 1) * BK);
+      __syncthreads();
+    }
+    compute((nk - 1) & 1);
+  }
+
+  // FP64 accumulator register r of lane l holds C[l/16 + 4r][l%16].
+  double *C = Cbase + t.c_off;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn * WTN + j * 16 + lr;
+      if (n >= t.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * WTM + i * 16 + lk + 4 * r;
+        if (m < t.M) C[m + (int64_t)t.ldc * n] = acc[i][j][r];
+      }
+    }
+}
+
+// ------------------------------------------------------------- mixed
+// Both operands are staged k-major ([r][k]) as 32-bit words: the FP32 value,
+// or a BF16 (hi, lo) pair split into two planes.  A lane's fragment (4
+// consecutive k for FP32, 8 for BF16) is one 16-byte LDS read.
+constexpr int PM_F32 = 1, PM_BF16 = 2;
+
+__device__ __forceinline__ uint32_t round_word(double x, int pm) {
+  const float f = (float)x;
+  if (pm == PM_F32) return __builtin_bit_cast(uint32_t, f);
+  const __bf16 hi = (__bf16)f;
+  const __bf16 lo = (__bf16)(f - (float)hi);
+  return (uint32_t)__builtin_bit_cast(uint16_t, hi) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
+
+template <int BR, bool RMAJ>
+struct XStage {
+  static constexpr int BK = 32;
+  static constexpr int NPAIR = BR * BK / 512;
+  __device__ static void coord(int t, int i, int &r, int &k) {
+    const int e = t + 256 * i;
+    if (RMAJ) {
+      r = e % BR;
+      k = 2 * (e / BR);
+    } else {
+      k = 2 * (e % (BK / 2));
+      r = e / (BK / 2);
+    }
+  }
+};
+
+template <int PM, int BM, int BN, bool AMC, bool BKC>
+__global__ void __launch_bounds__(256)
+    k_ggemm_mx(const GTile *__restrict__ tiles, const GTask *__restrict__ tasks,
+               const GSeg *__restrict__ segs, const double *__restrict__ Abase,
+               const double *__restrict__ Bbase, double *__restrict__ Cbase) {
+  constexpr int BK = 32;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int NS = PM == PM_BF16 ? 2 : 1;
+  constexpr int PITCH = PM == PM_BF16 ? BK + 8 : BK + 4;
+  using E = typename std::conditional<PM == PM_BF16, uint16_t, float>::type;
+  using SA = XStage<BM, AMC>;
+  using SB = XStage<BN, !BKC>;
+  __shared__ __align__(16) E sA[NS][2][BM * PITCH];
+  __shared__ __align__(16) E sB[NS][2][BN * PITCH];
+  const GTile tl = tiles[blockIdx.x];
+  if (tl.task < 0) return;
+  const GTask t = tasks[tl.task];
+  const int m0 = tl.m0, n0 = tl.n0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  f4_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  for (int sg = 0; sg < t.nseg; ++sg) {
+    const GSeg g = segs[t.seg0 + sg];
+    const double *A = Abase + g.a_off;
+    const double *B = Bbase + g.b_off;
+    const int64_t a_sr = AMC ? 1 : g.lda, a_sk = AMC ? g.lda : 1;
+    const int64_t b_sr = BKC ? g.ldb : 1, b_sk = BKC ? 1 : g.ldb;
+    const int K = g.K;
+    int64_t aoff[SA::NPAIR], boff[SB::NPAIR];
+    bool aok[SA::NPAIR], bok[SB::NPAIR];
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      aok[i] = m0 + r < t.M;
+      aoff[i] = (int64_t)min(m0 + r, t.M - 1) * a_sr;
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      bok[i] = n0 + r < t.N;
+      boff[i] = (int64_t)min(n0 + r, t.N - 1) * b_sr;
+    }
+    double ra[SA::NPAIR][2], rb[SB::NPAIR][2];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int i = 0; i < SA::NPAIR; ++i) {
+        int r, k;
+        SA::coord(tid, i, r, k);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) ra[i][e] = A[aoff[i] + (int64_t)min(k0 + k + e, K - 1) * a_sk];
+      }
+#pragma unroll
+      for (int i = 0; i < SB::NPAIR; ++i) {
+        int r, k;
+        SB::coord(tid, i, r, k);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) rb[i][e] = B[boff[i] + (int64_t)min(k0 + k + e, K - 1) * b_sk];
+      }
+    };
+    auto store = [&](int buf, int k0) {
+#pragma unroll
+      for (int i = 0; i < SA::NPAIR; ++i) {
+        int r, k;
+        SA::coord(tid, i, r, k);
+        const uint32_t w0 = (aok[i] && k0 + k < K) ? round_word(ra[i][0], PM) : 0u;
+        const uint32_t w1 = (aok[i] && k0 + k + 1 < K) ? round_word(ra[i][1], PM) : 0u;
+        if constexpr (PM == PM_BF16) {
+          *reinterpret_cast<uint32_t *>(&sA[0][buf][r * PITCH + k]) = (w0 & 0xffffu) | (w1 << 16);
+          *reinterpret_cast<uint32_t *>(&sA[1][buf][r * PITCH + k]) = (w0 >> 16) | (w1 & 0xffff0000u);
+        } else {
+          *reinterpret_cast<uint2 *>(&sA[0][buf][r * PITCH + k]) = make_uint2(w0, w1);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < SB::NPAIR; ++i) {
+        int r, k;
+        SB::coord(tid, i, r, k);
+        const uint32_t w0 = (bok[i] && k0 + k < K) ? round_word(rb[i][0], PM) : 0u;
+        const uint32_t w1 = (bok[i] && k0 + k + 1 < K) ? round_word(rb[i][1], PM) : 0u;
+        if constexpr (PM == PM_BF16) {
+          *reinterpret_cast<uint32_t *>(&sB[0][buf][r * PITCH + k]) = (w0 & 0xffffu) | (w1 << 16);
+          *reinterpret_cast<uint32_t *>(&sB[1][buf][r * PITCH + k]) = (w0 >> 16) | (w1 & 0xffff0000u);
+        } else {
+          *reinterpret_cast<uint2 *>(&sB[0][buf][r * PITCH + k]) = make_uint2(w0, w1);
+        }
+      }
+    };
+    auto compute = [&](int buf) {
+      if constexpr (PM == PM_F32) {
+#pragma unroll
+        for (int kb = 0; kb < BK; kb += 16) {
+          f4_t a[TM], b[TN];
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+            a[i] = *reinterpret_cast<const f4_t *>(&sA[0][buf][(wm * WTM + i * 16 + lr) * PITCH + kb + 4 * lk]);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            b[j] = *reinterpret_cast<const f4_t *>(&sB[0][buf][(wn * WTN + j * 16 + lr) * PITCH + kb + 4 * lk]);
+#pragma unroll
+          for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+              for (int j = 0; j < TN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+        }
+      } else {
+        bf8_t ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          const int o = (wm * WTM + i * 16 + lr) * PITCH + 8 * lk;
+          ah[i] = *reinterpret_cast<const bf8_t *>(&sA[0][buf][o]);
+          al[i] = *reinterpret_cast<const bf8_t *>(&sA[1][buf][o]);
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const int o = (wn * WTN + j * 16 + lr) * PITCH + 8 * lk;
+          bh[j] = *reinterpret_cast<const bf8_t *>(&sB[0][buf][o]);
+          bl[j] = *reinterpret_cast<const bf8_t *>(&sB[1][buf][o]);
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          }
+      }
+    };
+    const int nk = (K + BK - 1) / BK;
+    __syncthreads();
+    load(0);
+    store(0, 0);
+    __syncthreads();
+    for (int kt = 0; kt + 1 < nk; ++kt) {
+      load((kt + 1) * BK);
+      compute(kt & 1);
+      store((kt + 1) & 1, (kt + 1) * BK);
+      __syncthreads();
+    }
+    compute((nk - 1) & 1);
+  }
+
+  // FP32 accumulator register r of lane l holds C[4*(l/16) + r][l%16].
+  double *C = Cbase + t.c_off;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn * WTN + j * 16 + lr;
+      if (n >= t.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * WTM + i * 16 + 4 * lk + r;
+        if (m < t.M) C[m + (int64_t)t.ldc * n] = (double)acc[i][j][r];
+      }
+    }
+}
+
+// ---------------------------------------------------- block mode product
+// One workgroup per chunk: dst[e] = sum_c coef_c * src_c[e] over the chunk's
+// element range (column-major index e = r + rows * c within the block).
+__global__ void __launch_bounds__(256)
+    k_bmode(const MChunk *__restrict__ chunks, const MContrib *__restrict__ contribs,
+            const double *__restrict__ src, double *__restrict__ dst) {
+  const MChunk ch = chunks[blockIdx.x];
+  for (int e = ch.e0 + threadIdx.x; e < ch.e0 + ch.ne; e += 256) {
+    const int r = e % ch.rows, c = e / ch.rows;
+    double acc = 0.0;
+    for (int q = 0; q < ch.nc; ++q) {
+      const MContrib cb = contribs[ch.c0 + q];
+      acc += cb.coef * src[cb.src_off + r + (int64_t)cb.ld_src * c];
+    }
+    dst[ch.dst_off + r + (int64_t)ch.ld_dst * c] = acc;
+  }
+}
+
+template <int PM, bool AMC, bool BKC>
+static void launch_gg(const BsStage &st, const double *A, const double *B, double *C,
+                      hipStream_t s) {
+  if constexpr (PM == 0)
+    hipLaunchKernelGGL((k_ggemm<64, 64, AMC, BKC>), dim3(st.ntile), dim3(256), 0, s, st.tiles,
+                       st.tasks, st.segs, A, B, C);
+  else
+    hipLaunchKernelGGL((k_ggemm_mx<PM, 64, 64, AMC, BKC>), dim3(st.ntile), dim3(256), 0, s,
+                       st.tiles, st.tasks, st.segs, A, B, C);
+}
+
+template <int PM>
+static void launch_pm(const BsStage &st, const double *A, const double *B, double *C,
+                      hipStream_t s) {
+  if (st.amc && st.bkc) launch_gg<PM, true, true>(st, A, B, C, s);
+  else if (st.amc) launch_gg<PM, true, false>(st, A, B, C, s);
+  else if (st.bkc) launch_gg<PM, false, true>(st, A, B, C, s);
+  else launch_gg<PM, false, false>(st, A, B, C, s);
+}
+
+static int run_stage(mh_handle h, int prec, const BsStage &st, void *const *srcs) {
+  if (st.kind == 0) {
+    if (st.ntile == 0) return MH_OK;
+    const double *A = (const double *)srcs[st.a_src];
+    const double *B = (const double *)srcs[st.b_src];
+    double *C = (double *)srcs[st.c_src];
+    int slot;
+    prof_begin(h, MH_K_GEMM, st.flops, &slot);
+    if (prec == MH_PREC_F32) launch_pm<PM_F32>(st, A, B, C, h->stream);
+    else if (prec == MH_PREC_BF16X3) launch_pm<PM_BF16>(st, A, B, C, h->stream);
+    else launch_pm<0>(st, A, B, C, h->stream);
+    prof_end(h, slot);
+    return check_launch(h, "grouped gemm");
+  }
+  if (st.nchunk == 0) return MH_OK;
+  int slot;
+  prof_begin(h, MH_K_WSTAGE, st.flops, &slot);
+  hipLaunchKernelGGL(k_bmode, dim3(st.nchunk), dim3(256), 0, h->stream, st.chunks, st.contribs,
+                     (const double *)srcs[st.a_src], (double *)srcs[st.c_src]);
+  prof_end(h, slot);
+  return check_launch(h, "block mode product");
+}
+
+static size_t al256(int64_t elems) { return ((size_t)elems * 8 + 255) & ~size_t(255); }
+
+size_t bs_ws_bytes(const mh_bs_plan_s &p) {
+  return al256(p.size[MH_BS_T1]) + al256(p.size[MH_BS_T2]);
+}
+
+size_t bs_vec_len(const mh_bs_plan_s &p) { return (size_t)p.size[MH_BS_X]; }
+
+int bs_apply_ws(mh_handle h, const mh_bs_plan_s &p, const void *x, const void *p1,
+                const void *p2, void *y, void *ws) {
+  void *srcs[NSRC];
+  srcs[MH_BS_X] = const_cast<void *>(x);
+  srcs[MH_BS_Y] = y;
+  srcs[MH_BS_P1] = const_cast<void *>(p1);
+  srcs[MH_BS_P2] = const_cast<void *>(p2);
+  srcs[MH_BS_T1] = ws;
+  srcs[MH_BS_T2] = (char *)ws + al256(p.size[MH_BS_T1]);
+  for (const auto &st : p.stages) MH_TRY(run_stage(h, p.prec, st, srcs));
+  return MH_OK;
+}
+
+int bs_check_buffers(mh_handle h, const mh_bs_plan_s &p, const void *x, const void *p1,
+                     const void *p2, const void *y) {
+  const void *b[4] = {x, y, p1, p2};
+  static const char *names[4] = {"x", "y", "p1", "p2"};
+  for (int i = 0; i < 4; ++i)
+    if (p.size[i] > 0 && !b[i]) return fail(h, MH_ERR_ARG, "plan needs buffer %s", names[i]);
+  return MH_OK;
+}
+
+int bs_check_square(mh_handle h, const mh_bs_plan_s &p, const void *p1, const void *p2) {
+  if (p.size[MH_BS_X] <= 0 || p.size[MH_BS_X] != p.size[MH_BS_Y])
+    return fail(h, MH_ERR_SHAPE, "plan is not an operator (x has %lld, y %lld elements)",
+                (long long)p.size[MH_BS_X], (long long)p.size[MH_BS_Y]);
+  if ((p.size[MH_BS_P1] > 0 && !p1) || (p.size[MH_BS_P2] > 0 && !p2))
+    return fail(h, MH_ERR_ARG, "plan needs its environment buffers");
+  return MH_OK;
+}
+
+template <typename T>
+static int upload(mh_handle h, mh_bs_plan_s *p, const T *src, size_t n, T **dst) {
+  *dst = nullptr;
+  if (n == 0) return MH_OK;
+  void *d = nullptr;
+  if (hipMalloc(&d, n * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, MH_ERR_OOM, "plan allocation of %zu bytes", n * sizeof(T));
+  }
+  p->allocs.push_back(d);
+  MH_HIP(h, hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  *dst = (T *)d;
+  return MH_OK;
+}
+
+// Largest element offset touched by a (rows x cols) operand at `off` with
+// leading dimension ld, contiguous along rows (col-major) or along cols.
+static int64_t last_elem(int64_t off, int rows, int cols, int ld, bool contig_rows) {
+  return contig_rows ? off + (rows - 1) + (int64_t)(cols - 1) * ld
+                     : off + (int64_t)(rows - 1) * ld + (cols - 1);
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+extern "C" {
+
+int mh_bs_plan_create(mh_handle h, int prec, const int64_t *sizes, mh_bs_plan *out) {
+  if (!h || !out || !sizes) return MH_ERR_ARG;
+  *out = nullptr;
+  if (prec != MH_PREC_F64 && prec != MH_PREC_F32 && prec != MH_PREC_BF16X3)
+    return fail(h, MH_ERR_ARG, "unknown precision %d", prec);
+  for (int i = 0; i < NSRC; ++i)
+    if (sizes[i] < 0) return fail(h, MH_ERR_SHAPE, "negative buffer size");
+  auto *p = new mh_bs_plan_s();
+  p->prec = prec;
+  for (int i = 0; i < NSRC; ++i) p->size[i] = sizes[i];
+  *out = p;
+  return MH_OK;
+}
+
+int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int c_src,
+                        int amc, int bkc, int ntask, const int32_t *task_i,
+                        const int64_t *task_coff, int nseg, const int64_t *seg_off,
+                        const int32_t *seg_i) {
+  if (!h || !plan) return MH_ERR_ARG;
+  if (ntask < 0 || nseg < 0) return fail(h, MH_ERR_SHAPE, "negative task count");
+  if ((ntask > 0 && (!task_i || !task_coff)) || (nseg > 0 && (!seg_off || !seg_i)))
+    return fail(h, MH_ERR_ARG, "null task or segment table");
+  for (int s : {a_src, b_src, c_src})
+    if (s < 0 || s >= NSRC) return fail(h, MH_ERR_ARG, "bad operand buffer %d", s);
+  if (c_src == a_src || c_src == b_src)
+    return fail(h, MH_ERR_ARG, "gemm output aliases an input");
+  const int64_t asz = plan->size[a_src], bsz = plan->size[b_src], csz = plan->size[c_src];
+  std::vector<GSeg> segs(nseg);
+  for (int s = 0; s < nseg; ++s) {
+    segs[s] = GSeg{seg_off[2 * s], seg_off[2 * s + 1], seg_i[3 * s], seg_i[3 * s + 1],
+                   seg_i[3 * s + 2], 0};
+    if (segs[s].K <= 0 || segs[s].lda <= 0 || segs[s].ldb <= 0 || segs[s].a_off < 0 ||
+        segs[s].b_off < 0)
+      return fail(h, MH_ERR_SHAPE, "bad segment %d", s);
+  }
+  std::vector<GTask> tasks(ntask);
+  std::vector<GTile> logical;
+  double flops = 0;
+  for (int i = 0; i < ntask; ++i) {
+    // task_i rows: M, N, seg0, nseg, ldc
+    GTask t{task_i[5 * i], task_i[5 * i + 1], task_i[5 * i + 2], task_i[5 * i + 3],
+            task_coff[i], task_i[5 * i + 4], 0};
+    if (t.M <= 0 || t.N <= 0 || t.nseg <= 0 || t.seg0 < 0 || t.seg0 + t.nseg > nseg ||
+        t.ldc < t.M || t.c_off < 0 || last_elem(t.c_off, t.M, t.N, t.ldc, true) >= csz)
+      return fail(h, MH_ERR_SHAPE, "bad task %d (M %d N %d ldc %d)", i, t.M, t.N, t.ldc);
+    for (int s = t.seg0; s < t.seg0 + t.nseg; ++s) {
+      const GSeg &g = segs[s];
+      // A is M x K (m-contiguous when amc), B is K x N (k-contiguous when bkc)
+      if ((amc ? g.lda < t.M : g.lda < g.K) || (bkc ? g.ldb < g.K : g.ldb < t.N) ||
+          last_elem(g.a_off, t.M, g.K, g.lda, amc) >= asz ||
+          last_elem(g.b_off, g.K, t.N, g.ldb, bkc) >= bsz)
+        return fail(h, MH_ERR_SHAPE, "segment %d of task %d lies outside its buffers", s, i);
+      flops += 2.0 * t.M * (double)t.N * g.K;
+    }
+    tasks[i] = t;
+    for (int m0 = 0; m0 < t.M; m0 += 64)
+      for (int n0 = 0; n0 < t.N; n0 += 64) logical.push_back(GTile{i, m0, n0, 0});
+  }
+  // Blocks are dispatched round-robin over the 8 XCDs: block b runs on XCD
+  // b % 8.  XCD x gets the contiguous logical range [x*chunk, (x+1)*chunk).
+  const int nl = (int)logical.size();
+  const int chunk = (nl + 7) / 8;
+  std::vector<GTile> tiles((size_t)chunk * 8, GTile{-1, 0, 0, 0});
+  for (int b = 0; b < chunk * 8; ++b) {
+    const int l = (b % 8) * chunk + b / 8;
+    if (l < nl) tiles[b] = logical[l];
+  }
+  BsStage st{};
+  st.kind = 0;
+  st.a_src = a_src;
+  st.b_src = b_src;
+  st.c_src = c_src;
+  st.amc = amc ? 1 : 0;
+  st.bkc = bkc ? 1 : 0;
+  st.ntile = nl > 0 ? chunk * 8 : 0;
+  st.flops = flops;
+  MH_TRY(upload(h, plan, tasks.data(), tasks.size(), &st.tasks));
+  MH_TRY(upload(h, plan, segs.data(), segs.size(), &st.segs));
+  if (nl > 0) MH_TRY(upload(h, plan, tiles.data(), tiles.size(), &st.tiles));
+  MH_HIP(h, hipStreamSynchronize(h->stream));
+  plan->stages.push_back(st);
+  plan->flops += flops;
+  return MH_OK;
+}
+
+int mh_bs_plan_add_mode(mh_handle h, mh_bs_plan plan, int in_src, int out_src, int ndst,
+                        const int64_t *dst_off, const int32_t *dst_i, int ncontrib,
+                        const int64_t *src_off, const int32_t *src_ld, const double *coef) {
+  if (!h || !plan) return MH_ERR_ARG;
+  if (ndst < 0 || ncontrib < 0) return fail(h, MH_ERR_SHAPE, "negative count");
+  if ((ndst > 0 && (!dst_off || !dst_i)) || (ncontrib > 0 && (!src_off || !src_ld || !coef)))
+    return fail(h, MH_ERR_ARG, "null block or contribution table");
+  if (in_src < 0 || in_src >= NSRC || out_src < 0 || out_src >= NSRC || in_src == out_src)
+    return fail(h, MH_ERR_ARG, "bad mode-product buffers");
+  const int64_t isz = plan->size[in_src], osz = plan->size[out_src];
+  std::vector<MChunk> chunks;
+  double flops = 0;
+  for (int d = 0; d < ndst; ++d) {
+    // dst_i rows: rows, cols, ld_dst, c0, nc
+    const int rows = dst_i[5 * d], cols = dst_i[5 * d + 1], ld = dst_i[5 * d + 2];
+    const int c0 = dst_i[5 * d + 3], nc = dst_i[5 * d + 4];
+    if (rows <= 0 || cols <= 0 || ld < rows || c0 < 0 || nc < 0 || c0 + nc > ncontrib ||
+        dst_off[d] < 0 || last_elem(dst_off[d], rows, cols, ld, true) >= osz)
+      return fail(h, MH_ERR_SHAPE, "bad destination block %d", d);
+    for (int c = c0; c < c0 + nc; ++c)
+      if (src_ld[c] < rows || src_off[c] < 0 ||
+          last_elem(src_off[c], rows, cols, src_ld[c], true) >= isz)
+        return fail(h, MH_ERR_SHAPE, "contribution %d of block %d lies outside its buffer", c, d);
+    const int n = rows * cols;
+    for (int e0 = 0; e0 < n; e0 += 1024)
+      chunks.push_back(MChunk{dst_off[d], rows, ld, e0, std::min(1024, n - e0), c0, nc});
+    flops += 2.0 * n * nc;
+  }
+  std::vector<MContrib> cbs(ncontrib);
+  for (int c = 0; c < ncontrib; ++c) cbs[c] = MContrib{src_off[c], src_ld[c], 0, coef[c]};
+  BsStage st{};
+  st.kind = 1;
+  st.a_src = in_src;
+  st.c_src = out_src;
+  st.nchunk = (int)chunks.size();
+  st.flops = flops;
+  MH_TRY(upload(h, plan, chunks.data(), chunks.size(), &st.chunks));
+  MH_TRY(upload(h, plan, cbs.data(), cbs.size(), &st.contribs));
+  MH_HIP(h, hipStreamSynchronize(h->stream));
+  plan->stages.push_back(st);
+  plan->flops += flops;
+  return MH_OK;
+}
+
+int mh_bs_plan_destroy(mh_handle h, mh_bs_plan plan) {
+  if (!h || !plan) return MH_ERR_ARG;
+  (void)hipStreamSynchronize(h->stream);
+  for (void *a : plan->allocs) (void)hipFree(a);
+  delete plan;
+  return MH_OK;
+}
+
+int mh_bs_plan_flops(mh_bs_plan plan, double *flops) {
+  if (!plan || !flops) return MH_ERR_ARG;
+  *flops = plan->flops;
+  return MH_OK;
+}
+
+int mh_bs_apply(mh_handle h, mh_bs_plan plan, const void *x, const void *p1, const void *p2,
+                void *y) {
+  if (!h || !plan) return MH_ERR_ARG;
+  MH_TRY(bs_check_buffers(h, *plan, x, p1, p2, y));
+  MH_TRY(ensure_ws(h, bs_ws_bytes(*plan)));
+  return bs_apply_ws(h, *plan, x, p1, p2, y, h->ws);
+}
+
+}  // extern "C"

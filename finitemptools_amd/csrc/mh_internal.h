@@ -124,6 +124,14 @@ int vec_dot_dev(mh_handle h, int dtype, size_t n, const void *x, const void *y,
 int vec_nrm2sq_dev(mh_handle h, int dtype, size_t n, const void *x,
                    double *dres);
 
+// Block-sparse plans (mh_blocksparse.hip).
+size_t bs_ws_bytes(const mh_bs_plan_s &p);
+size_t bs_vec_len(const mh_bs_plan_s &p);
+int bs_apply_ws(mh_handle h, const mh_bs_plan_s &p, const void *x, const void *p1,
+                const void *p2, void *y, void *ws);
+// An operator plan (x and y of one size) with the environment buffers it needs.
+int bs_check_square(mh_handle h, const mh_bs_plan_s &p, const void *p1, const void *p2);
+
 inline size_t dtype_bytes(int dtype) { return dtype == MH_C64 ? 16 : 8; }
 
 }  // namespace mh

@@ -391,7 +391,9 @@ void sym_eig(int k, const double *A, double *evals, double *Z) {
 
 struct Krylov {
   mh_handle h;
-  const mh_heff_op &op;
+  mh_heff_op op;                     // dense operator (when bs is null)
+  const mh_bs_plan_s *bs = nullptr;  // block-sparse operator plan
+  const void *bp1 = nullptr, *bp2 = nullptr;
   bool cplx;
   size_t n, nd;  // scalars, doubles per vector
   int kmax;
@@ -408,10 +410,21 @@ struct Krylov {
     nd = cplx ? 2 * n : n;
     G = nblocks(nd);
   }
+  Krylov(mh_handle hh, const mh_bs_plan_s &p, const void *p1, const void *p2, int k)
+      : h(hh), op{}, bs(&p), bp1(p1), bp2(p2), cplx(false), kmax(k), lay(k) {
+    n = bs_vec_len(p);
+    nd = n;
+    G = nblocks(nd);
+  }
+
+  int apply(const double *x, double *y) {
+    if (bs) return bs_apply_ws(h, *bs, x, bp1, bp2, y, heff_ws);
+    return heff_apply_ws(h, op, x, y, heff_ws, op.kind == MH_HEFF0 ? nullptr : Mt, true);
+  }
 
   int setup() {
     auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
-    size_t hb = heff_ws_bytes(op);
+    size_t hb = bs ? bs_ws_bytes(*bs) : heff_ws_bytes(op);
     // V_0 .. V_{kmax-1} and w, packed at stride nd (the multi-vector kernels
     // index V_j at j * nd), then room for the MPO mode matrix.
     size_t vb = al(nd * sizeof(double) * (kmax + 1));
@@ -424,8 +437,8 @@ struct Krylov {
     MH_TRY(ensure_red(h, lay.total));
     st = h->red;
     // Mixed forms: the environments are packed once for the whole solve.
-    if (op.prec != MH_PREC_F64) MH_TRY(heff_prepare_mixed(h, op, heff_ws));
-    if (op.kind != MH_HEFF0) {
+    if (!bs && op.prec != MH_PREC_F64) MH_TRY(heff_prepare_mixed(h, op, heff_ws));
+    if (!bs && op.kind != MH_HEFF0) {
       int kind = op.kind == MH_HEFF2 ? MODE_TWO : MODE_SITE;
       MH_TRY(build_mode_matrix(h, op.dtype, kind, op.wl, op.wm, op.wr, op.d1, op.d2,
                                op.W1, op.W2, Mt));
@@ -457,7 +470,7 @@ struct Krylov {
   // One Lanczos step i: w = H V_i, alpha_i, w -= ..., (reorth), beta_i, V_{i+1}.
   int step(int i, double btol, int reorth, bool write_next) {
     double *part = st + lay.o_part;
-    MH_TRY(heff_apply_ws(h, op, vec(i), w, heff_ws, op.kind == MH_HEFF0 ? nullptr : Mt, true));
+    MH_TRY(apply(vec(i), w));
     int slot;
     prof_begin(h, MH_K_VEC, 0.0, &slot);
     if (cplx) hipLaunchKernelGGL(k_dot_part<true>, dim3(G), dim3(RB), 0, h->stream, vec(i), w, n, part);
@@ -512,6 +525,38 @@ struct Krylov {
     return MH_OK;
   }
 };
+
+// Lanczos eigensolve on a prepared Krylov driver (dense or block-sparse).
+static int lanczos_eig_run(mh_handle h, Krylov &K, const void *x0, int kmax, double btol,
+                           int reorth, double *eigval, void *xout, int *info, int *niter) {
+  MH_TRY(K.setup());
+  MH_TRY(K.start(x0));
+  for (int i = 0; i < kmax; ++i) MH_TRY(K.step(i, btol, reorth, i + 1 < kmax));
+  std::vector<double> hs;
+  MH_TRY(K.read_state(hs));
+  if (!(hs[3] > 0.0) || !std::isfinite(hs[3]))
+    return fail(h, MH_ERR_ARG, "initial vector has zero or non-finite norm");
+  int k = (int)hs[1];
+  bool converged = hs[0] != 0.0;
+  if (k < 1) k = 1;
+  std::vector<double> T((size_t)k * k, 0.0), ev(k), Z((size_t)k * k);
+  for (int i = 0; i < k; ++i) {
+    T[i + k * i] = hs[K.lay.o_alpha + i];
+    if (i + 1 < k) {
+      T[i + 1 + k * i] = hs[K.lay.o_beta + i];
+      T[i + k * (i + 1)] = hs[K.lay.o_beta + i];
+    }
+  }
+  sym_eig(k, T.data(), ev.data(), Z.data());
+  *eigval = ev[0];  // "SR": smallest
+  std::vector<double> c2(2 * k, 0.0);
+  for (int i = 0; i < k; ++i) c2[2 * i] = Z[i];
+  MH_TRY(K.combine(k, c2.data(), xout));
+  if (info) *info = converged ? 0 : -1;
+  if (niter) *niter = k;
+  if (!std::isfinite(*eigval)) return fail(h, MH_ERR_NOCONV, "non-finite Ritz value");
+  return MH_OK;
+}
 
 }  // namespace mh
 
@@ -580,33 +625,18 @@ int mh_lanczos_eig(mh_handle h, const mh_heff_op *op, const void *x0, int kmax,
   if (!x0 || !xout) return fail(h, MH_ERR_ARG, "null vector");
   if (kmax < 1 || kmax > 128) return fail(h, MH_ERR_ARG, "kmax must be in [1, 128]");
   Krylov K(h, *op, kmax);
-  MH_TRY(K.setup());
-  MH_TRY(K.start(x0));
-  for (int i = 0; i < kmax; ++i) MH_TRY(K.step(i, btol, reorth, i + 1 < kmax));
-  std::vector<double> hs;
-  MH_TRY(K.read_state(hs));
-  if (!(hs[3] > 0.0) || !std::isfinite(hs[3]))
-    return fail(h, MH_ERR_ARG, "initial vector has zero or non-finite norm");
-  int k = (int)hs[1];
-  bool converged = hs[0] != 0.0;
-  if (k < 1) k = 1;
-  std::vector<double> T((size_t)k * k, 0.0), ev(k), Z((size_t)k * k);
-  for (int i = 0; i < k; ++i) {
-    T[i + k * i] = hs[K.lay.o_alpha + i];
-    if (i + 1 < k) {
-      T[i + 1 + k * i] = hs[K.lay.o_beta + i];
-      T[i + k * (i + 1)] = hs[K.lay.o_beta + i];
-    }
-  }
-  sym_eig(k, T.data(), ev.data(), Z.data());
-  *eigval = ev[0];  // "SR": smallest
-  std::vector<double> c2(2 * k, 0.0);
-  for (int i = 0; i < k; ++i) c2[2 * i] = Z[i];
-  MH_TRY(K.combine(k, c2.data(), xout));
-  if (info) *info = converged ? 0 : -1;
-  if (niter) *niter = k;
-  if (!std::isfinite(*eigval)) return fail(h, MH_ERR_NOCONV, "non-finite Ritz value");
-  return MH_OK;
+  return lanczos_eig_run(h, K, x0, kmax, btol, reorth, eigval, xout, info, niter);
+}
+
+int mh_bs_lanczos_eig(mh_handle h, mh_bs_plan plan, const void *p1, const void *p2,
+                      const void *x0, int kmax, double btol, int reorth, double *eigval,
+                      void *xout, int *info, int *niter) {
+  if (!h || !plan || !eigval) return MH_ERR_ARG;
+  if (!x0 || !xout) return fail(h, MH_ERR_ARG, "null vector");
+  if (kmax < 1 || kmax > 128) return fail(h, MH_ERR_ARG, "kmax must be in [1, 128]");
+  MH_TRY(bs_check_square(h, *plan, p1, p2));
+  Krylov K(h, *plan, p1, p2, kmax);
+  return lanczos_eig_run(h, K, x0, kmax, btol, reorth, eigval, xout, info, niter);
 }
 
 int mh_lanczos_expm(mh_handle h, const mh_heff_op *op, const void *x0, double dt_re,

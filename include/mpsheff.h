@@ -163,6 +163,59 @@ int mh_gemm(mh_handle h, int dtype, int prec, int M, int N, int K,
             int64_t scn, double alpha_re, double alpha_im, double beta_re,
             double beta_im);
 
+/* ---- block-sparse (abelian-symmetric) contractions ----------------------- */
+/* A tensor with U(1)-type charge conservation is a list of dense column-major
+ * blocks in one device buffer (TensorKit GradedSpace tensors; the Hubbard
+ * U1xU1 site operators of utilities/fermion_siteops.jl:37-62).  A host
+ * planner that knows the charge sectors lowers one contraction to a plan of
+ * stages; each stage is ONE launch over all sectors:
+ *   grouped GEMM: tasks (one output block C, M x N, ldc) each summing
+ *     K-segments opA (M x K) * opB (K x N) addressed by (offset, ld);
+ *   block mode product: destination blocks, each a linear combination of
+ *     equally shaped source blocks (the MPO entries between the sectors).
+ * Plans replace TensorKit's per-sector loop behind
+ *   ac2_prime(x, h1, h2, hleft, hright)  algorithms/derivatives.jl:20-22,
+ *   ac_prime(x, h, hleft, hright)        algorithms/derivatives.jl:2-4,
+ *   updateleft / updateright             mpo/transfer.jl:26-34
+ * on graded tensors.  Arithmetic is FP64 storage with FP64 or mixed
+ * (MH_PREC_F32 / MH_PREC_BF16X3, FP32 accumulation) matrix-core products.
+ * Buffers: the operand table of every stage indexes six buffers.          */
+typedef struct mh_bs_plan_s *mh_bs_plan;
+enum mh_bs_buffer {
+  MH_BS_X = 0,   /* input vector (ket site tensor for environment updates) */
+  MH_BS_Y = 1,   /* output                                                 */
+  MH_BS_P1 = 2,  /* left environment (H_eff, updateleft) / old right env   */
+  MH_BS_P2 = 3,  /* right environment (H_eff) / bra site tensor (updates)  */
+  MH_BS_T1 = 4,  /* intermediates, held in the handle's workspace          */
+  MH_BS_T2 = 5
+};
+/* sizes[6]: elements of each buffer; every table entry added later is
+ * checked against them, so a plan never addresses outside its buffers. */
+int mh_bs_plan_create(mh_handle h, int prec, const int64_t *sizes, mh_bs_plan *plan);
+/* task_i[5*t] = {M, N, seg0, nseg, ldc}, task_coff[t] = offset of C;
+ * seg_off[2*s] = {offset of A, offset of B}, seg_i[3*s] = {K, lda, ldb}.
+ * amc: A is m-contiguous (A(m,k) at m + k*lda), else A(m,k) at m*lda + k;
+ * bkc: B is k-contiguous (B(k,n) at k + n*ldb), else B(k,n) at k*ldb + n. */
+int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int c_src,
+                        int amc, int bkc, int ntask, const int32_t *task_i,
+                        const int64_t *task_coff, int nseg, const int64_t *seg_off,
+                        const int32_t *seg_i);
+/* dst_i[5*b] = {rows, cols, ld, c0, nc}: block b at dst_off[b] is
+ * sum_{c in [c0, c0+nc)} coef[c] * (rows x cols block at src_off[c], ld src_ld[c]) */
+int mh_bs_plan_add_mode(mh_handle h, mh_bs_plan plan, int in_src, int out_src, int ndst,
+                        const int64_t *dst_off, const int32_t *dst_i, int ncontrib,
+                        const int64_t *src_off, const int32_t *src_ld, const double *coef);
+int mh_bs_plan_flops(mh_bs_plan plan, double *flops);  /* sum 2mnk + mode flops */
+int mh_bs_plan_destroy(mh_handle h, mh_bs_plan plan);
+/* run the plan: y = contraction(x; p1, p2) */
+int mh_bs_apply(mh_handle h, mh_bs_plan plan, const void *x, const void *p1, const void *p2,
+                void *y);
+/* mh_lanczos_eig with a block-sparse H_eff plan (x and y of equal size);
+ * the Lanczos vectors are the flat block buffers. */
+int mh_bs_lanczos_eig(mh_handle h, mh_bs_plan plan, const void *p1, const void *p2,
+                      const void *x0, int kmax, double btol, int reorth, double *eigval,
+                      void *xout, int *info, int *niter);
+
 /* ---- measurement --------------------------------------------------------- */
 enum mh_kernel_class { MH_K_GEMM = 0, MH_K_WSTAGE = 1, MH_K_VEC = 2, MH_K_NCLASS = 3 };
 /* enable = 1: bracket every launch with HIP events on the handle's stream */

@@ -85,6 +85,20 @@ def hubbard_hamiltonian(L, t=1.0, U=4.0, mu=0.0):
     return H
 
 
+def hubbard_sector_energy(L, nup, ndn, t=1.0, U=4.0):
+    """Lowest energy of the Hubbard chain with nup up and ndn down electrons:
+    the Kronecker Hamiltonian restricted to the basis states of that sector
+    (mode 2i counts up, 2i+1 down electrons; occupied = bit 1, mode 0 the
+    most significant bit)."""
+    H = hubbard_hamiltonian(L, t, U)
+    M = 2 * L
+    idx = np.arange(2 ** M)
+    bits = (idx[:, None] >> (M - 1 - np.arange(M))[None, :]) & 1
+    sel = np.nonzero((bits[:, 0::2].sum(1) == nup) & (bits[:, 1::2].sum(1) == ndn))[0]
+    Hs = H[sel][:, sel]
+    return ground_energy(Hs)[0]
+
+
 def ground_energy(H, k=1):
     if H.shape[0] <= 512:
         return np.linalg.eigvalsh(H.toarray())[:k]

@@ -23,6 +23,12 @@ CASES = {
     "hubbard_L6_t1_U4": (lambda: ed.hubbard_hamiltonian(6, 1.0, 4.0)),
 }
 
+# lowest energy inside one charge sector (n_up, n_dn)
+SECTOR_CASES = {
+    "hubbard_L6_t1_U4_N3_3": (6, 3, 3),
+    "hubbard_L8_t1_U4_N4_4": (8, 4, 4),
+}
+
 
 def main():
     out = {}
@@ -32,6 +38,10 @@ def main():
         e = float(ed.ground_energy(H)[0])
         out[name] = e
         print(f"{name}: {e:.15f} ({time.time() - t0:.1f} s)", flush=True)
+    for name, (L, nup, ndn) in SECTOR_CASES.items():
+        t0 = time.time()
+        out[name] = float(ed.hubbard_sector_energy(L, nup, ndn))
+        print(f"{name}: {out[name]:.15f} ({time.time() - t0:.1f} s)", flush=True)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ed_energies.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)

@@ -1,0 +1,194 @@
+"""Block-sparse plans on the device: grouped GEMM / block mode product against
+the numpy execution of the same tables and the dense oracle (1e-12), mixed
+precisions within their stated tolerances, the device Lanczos against the
+projected dense H_eff, and two-site DMRG on charge-conserving Hubbard and XXZ
+chains against exact diagonalisation in the charge sector (1e-8, as the
+reference's DMRG tests, test/runtests.jl:423-451)."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from finitemptools_amd import blockmps as bm
+from finitemptools_amd import blocksparse as bs
+from finitemptools_amd import models
+from oracle import contractions as OC
+from plan_exec import execute
+
+pytestmark = pytest.mark.gpu
+
+ED = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ed_energies.json")))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def rand_space(rng, charges, lo, hi):
+    return bs.Space([(q, int(rng.integers(lo, hi + 1))) for q in charges])
+
+
+def reachable(S, phys, n):
+    acc = {}
+    for q, _ in S.sectors:
+        st = [q]
+        for _ in range(n):
+            st = [bs.qadd(x, p) for x in st for p in phys]
+        for x in st:
+            acc[x] = 1
+    return list(acc)
+
+
+def heff_case(seed, lo, hi):
+    rng = np.random.default_rng(seed)
+    phys = bm.hubbard_charges()
+    mpo = bm.BlockMPO(models.hubbard_mpo(8, 1.0, 4.0), phys)
+    Sl = rand_space(rng, [(a, b) for a in range(1, 5) for b in range(1, 5)], lo, hi)
+    Sr = rand_space(rng, reachable(Sl, phys, 2), lo, hi)
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    Lay, Ray = bs.EnvLayout(Sl, mpo.qw[3]), bs.EnvLayout(Sr, mpo.qw[5])
+    x = rng.standard_normal(xl.size)
+    Le = rng.standard_normal(Lay.size) / np.sqrt(hi * 6)
+    Re = rng.standard_normal(Ray.size) / np.sqrt(hi * 6)
+    tab = bs.plan_heff2(xl, Lay, Ray, mpo[3], mpo[4])
+    return tab, xl, x, Le, Re
+
+
+def dev(a):
+    return torch.as_tensor(a, dtype=torch.float64).cuda()
+
+
+@pytest.mark.parametrize("lo,hi", [(1, 7), (20, 90)])
+def test_heff2_plan_device_matches_tables(gpu, lo, hi):
+    """Uneven sector sizes (1..7: every block smaller than one tile; 20..90:
+    ragged multi-tile blocks)."""
+    tab, xl, x, Le, Re = heff_case(1, lo, hi)
+    ref = execute(tab, [x, np.zeros(xl.size), Le, Re, None, None])[1]
+    y = torch.full((xl.size,), float("nan"), dtype=torch.float64, device="cuda")
+    plan = bs.Plan(tab, "f64")
+    plan.apply(dev(x), dev(Le), dev(Re), y)
+    assert rel(y.cpu().numpy(), ref) < 1e-12
+    assert abs(plan.flops() - tab.flops()) <= 1e-9 * tab.flops()
+
+
+def test_heff2_plan_device_matches_dense_oracle(gpu):
+    tab, xl, x, Le, Re = heff_case(2, 2, 9)
+    y = torch.zeros(xl.size, dtype=torch.float64, device="cuda")
+    bs.Plan(tab).apply(dev(x), dev(Le), dev(Re), y)
+    mpo = bm.BlockMPO(models.hubbard_mpo(8, 1.0, 4.0), bm.hubbard_charges())
+    Lay, Ray = bs.EnvLayout(xl.lspace, mpo.qw[3]), bs.EnvLayout(xl.rspace, mpo.qw[5])
+    xd = bs.BlockTensor(xl, torch.as_tensor(x)).to_dense()
+    ref = OC.ac2_prime(xd, mpo[3], mpo[4], bs.BlockEnv(Lay, torch.as_tensor(Le)).to_dense(),
+                       bs.BlockEnv(Ray, torch.as_tensor(Re)).to_dense())
+    got = bs.BlockTensor(xl, y.cpu()).to_dense()
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16x3", 1e-4)])
+def test_heff2_plan_mixed_precision(gpu, prec, tol):
+    tab, xl, x, Le, Re = heff_case(3, 20, 90)
+    ref = execute(tab, [x, np.zeros(xl.size), Le, Re, None, None])[1]
+    y = torch.zeros(xl.size, dtype=torch.float64, device="cuda")
+    bs.Plan(tab, prec).apply(dev(x), dev(Le), dev(Re), y)
+    err = rel(y.cpu().numpy(), ref)
+    assert err < tol
+    assert err > 1e-12  # the reduced-precision kernels did run
+
+
+def test_env_and_two_site_plans_device(gpu):
+    rng = np.random.default_rng(4)
+    phys = bm.hubbard_charges()
+    mpo = bm.BlockMPO(models.hubbard_mpo(8, 1.0, 4.0), phys)
+    Sl = rand_space(rng, [(a, b) for a in range(1, 4) for b in range(1, 4)], 3, 70)
+    Sr = rand_space(rng, reachable(Sl, phys, 1), 3, 70)
+    al = bs.BlockLayout(Sl, phys, 1, Sr)
+    A, B = rng.standard_normal(al.size), rng.standard_normal(al.size)
+    W = mpo[3]
+    Lay, Lnew = bs.EnvLayout(Sl, mpo.qw[3]), bs.EnvLayout(Sr, mpo.qw[4])
+    Le = rng.standard_normal(Lay.size)
+    tab = bs.plan_updateleft(al, Lay, W, Lnew)
+    ref = execute(tab, [B, np.zeros(Lnew.size), Le, A, None, None])[1]
+    y = torch.zeros(Lnew.size, dtype=torch.float64, device="cuda")
+    bs.Plan(tab).apply(dev(B), dev(Le), dev(A), y)
+    assert rel(y.cpu().numpy(), ref) < 1e-12
+    Ray, Rnew = bs.EnvLayout(Sr, mpo.qw[4]), bs.EnvLayout(Sl, mpo.qw[3])
+    Re = rng.standard_normal(Ray.size)
+    tab = bs.plan_updateright(al, Ray, W, Rnew)
+    ref = execute(tab, [B, np.zeros(Rnew.size), Re, A, None, None])[1]
+    y = torch.zeros(Rnew.size, dtype=torch.float64, device="cuda")
+    bs.Plan(tab).apply(dev(B), dev(Re), dev(A), y)
+    assert rel(y.cpu().numpy(), ref) < 1e-12
+    # two-site product over the bond Sr
+    Sr2 = rand_space(rng, reachable(Sr, phys, 1), 3, 70)
+    l2 = bs.BlockLayout(Sr, phys, 1, Sr2)
+    A2 = rng.standard_normal(l2.size)
+    xl = bs.BlockLayout(Sl, phys, 2, Sr2)
+    tab = bs.plan_two_site(al, l2, xl)
+    ref = execute(tab, [A, np.zeros(xl.size), A2, None, None, None])[1]
+    y = torch.zeros(xl.size, dtype=torch.float64, device="cuda")
+    bs.Plan(tab).apply(dev(A), dev(A2), None, y)
+    assert rel(y.cpu().numpy(), ref) < 1e-12
+
+
+def test_block_lanczos_matches_projected_dense_heff(gpu):
+    """Environments of a random half-filled L=6 state: the device Lanczos on
+    the block plan reaches the lowest eigenvalue of the dense H_eff restricted
+    to the charge-allowed entries."""
+    phys = bm.hubbard_charges()
+    mpo = bm.BlockMPO(models.hubbard_mpo(6, 1.0, 4.0), phys)
+    psi = bm.randommps(6, phys, (3, 3), D=8, seed=1).right_canonicalize()
+    env = bm.BlockExpectationCache(mpo, psi)
+    env.updateleft(0)
+    env.updateleft(1)
+    x = bm.two_site(psi[2], psi[3])
+    L, R = env.hstorage[2], env.hstorage[4]
+    tab = bs.plan_heff2(x.layout, L.layout, R.layout, mpo[2], mpo[3])
+    n = x.layout.size
+    ev, vec, info, nit = bs.Plan(tab).lanczos(L.data, R.data, x.data, min(n, 120), 1e-12,
+                                              reorth=True)
+    H = OC.dense_heff2(mpo[2], mpo[3], L.to_dense(), R.to_dense())
+    # positions of the block entries in the oracle's vectorisation
+    probe = bs.BlockTensor(x.layout, torch.arange(1, n + 1, dtype=torch.float64)).to_dense()
+    pos = np.argsort(OC.vec(probe))[-n:]
+    pos = pos[np.argsort(OC.vec(probe)[pos])]
+    Hs = H[np.ix_(pos, pos)]
+    assert np.allclose(Hs, Hs.T, atol=1e-10 * np.abs(Hs).max())
+    e0 = np.linalg.eigvalsh(Hs)[0]
+    assert abs(ev - e0) < 1e-9 * max(1.0, abs(e0))
+
+
+def run_block_dmrg(mpo_host, phys, psi, D, nsweeps, prec="f64", eps=1e-10):
+    from finitemptools_amd.dmrg import DMRG2
+    from finitemptools_amd.truncation import MPSTruncation
+    mpo = bm.BlockMPO(mpo_host, phys)
+    alg = DMRG2(trunc=MPSTruncation(D=D, eps=eps), prec=prec)
+    energies, delta = bm.ground_state(psi, mpo, alg, nsweeps=nsweeps)
+    return energies, delta, mpo, alg
+
+
+def test_block_dmrg2_hubbard_half_filling_from_product_state(gpu):
+    """Charge-density-wave product start (the runtests.jl:324-337 pattern):
+    empty and doubly occupied sites alternating, N_up = N_dn = 3."""
+    phys = bm.hubbard_charges()
+    empty, double = phys.index((0, 0)), phys.index((1, 1))
+    psi = bm.prodmps(phys, [empty, double] * 3)
+    energies, delta, mpo, _ = run_block_dmrg(models.hubbard_mpo(6, 1.0, 4.0), phys, psi, 64, 6)
+    assert abs(energies[-1] - ED["hubbard_L6_t1_U4_N3_3"]) < 1e-8
+    assert abs(bm.expectation(mpo, psi) - energies[-1]) < 1e-8
+
+
+def test_block_dmrg2_hubbard_L8_random_start(gpu):
+    phys = bm.hubbard_charges()
+    psi = bm.randommps(8, phys, (4, 4), D=16, seed=0).right_canonicalize()
+    energies, delta, _, _ = run_block_dmrg(models.hubbard_mpo(8, 1.0, 4.0), phys, psi, 256, 6)
+    assert abs(energies[-1] - ED["hubbard_L8_t1_U4_N4_4"]) < 1e-8
+
+
+def test_block_dmrg2_xxz_u1(gpu):
+    phys = bm.sz_charges()
+    up, dn = phys.index((1,)), phys.index((-1,))
+    psi = bm.prodmps(phys, [up, dn] * 6)
+    energies, delta, _, _ = run_block_dmrg(models.heisenberg_xxz_mpo(12), phys, psi, 64, 6)
+    assert abs(energies[-1] - ED["xxz_L12_J1_D1"]) < 1e-8
