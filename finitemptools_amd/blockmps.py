@@ -327,41 +327,49 @@ def expectation(mpo, mps):
 # ------------------------------------------------------------------ two-site step
 
 class TwoSiteMap:
-    """Index maps between a two-site tensor's blocks and its matrices per
-    middle charge c: M_c[(qa, s1, i), (s2, qb, j)] = x[(qa, s1, s2, qb)][i, j]."""
+    """Permutation between a two-site tensor's block buffer and its matrices
+    per middle charge c, M_c[(qa, s1, i), (s2, qb, j)] = x[(qa, s1, s2, qb)][i, j]
+    (the (a, s1) | (s2, b) split of stable_tsvd!, dmrg.jl:104): ``flat[perm]``
+    is the row-major concatenation of all M_c."""
 
-    def __init__(self, xl, device):
+    def __init__(self, xl):
         self.xl = xl
         Sl, Sr, phys = xl.lspace, xl.rspace, xl.phys
         rows, cols = {}, {}
         for qa, Da in Sl.sectors:
             for s1 in range(xl.d):
-                c = qadd(qa, phys[s1])
-                rows.setdefault(c, []).append((qa, s1, Da))
+                rows.setdefault(qadd(qa, phys[s1]), []).append((qa, s1, Da))
         for qb, Db in Sr.sectors:
             for s2 in range(xl.d):
-                c = qsub(qb, phys[s2])
-                cols.setdefault(c, []).append((s2, qb, Db))
+                cols.setdefault(qsub(qb, phys[s2]), []).append((s2, qb, Db))
         self.charges = [c for c in sorted(rows) if c in cols]
         self.rows = {c: rows[c] for c in self.charges}
         self.cols = {c: cols[c] for c in self.charges}
-        self.idx = {}
+        self.shape, self.base = {}, {}
+        perm = np.empty(xl.size, dtype=np.int64)
+        base = 0
         for c in self.charges:
             R = sum(r[2] for r in self.rows[c])
             C = sum(r[2] for r in self.cols[c])
-            idx = np.empty((R, C), dtype=np.int64)
+            self.shape[c], self.base[c] = (R, C), base
             ro = 0
             for qa, s1, Da in self.rows[c]:
                 co = 0
+                I = np.arange(Da)[:, None]
                 for s2, qb, Db in self.cols[c]:
                     b = xl.index[(qa, (s1, s2))]
-                    idx[ro:ro + Da, co:co + Db] = b.off + np.arange(Da)[:, None] + Da * np.arange(Db)[None, :]
+                    J = np.arange(Db)[None, :]
+                    perm[base + (ro + I) * C + (co + J)] = b.off + I + J * Da
                     co += Db
                 ro += Da
-            self.idx[c] = torch.as_tensor(idx, device=device)
+            base += R * C
+        assert base == xl.size
+        self.perm = perm
 
-    def matrices(self, flat):
-        return {c: flat[self.idx[c]] for c in self.charges}
+    def matrices(self, flat_host):
+        cat = flat_host[self.perm]
+        return {c: cat[self.base[c]:self.base[c] + self.shape[c][0] * self.shape[c][1]]
+                .reshape(self.shape[c]) for c in self.charges}
 
 
 def two_site_layout(A1, A2):
@@ -377,38 +385,133 @@ def two_site(A1, A2, xl=None):
     return BlockTensor(xl, out)
 
 
-def split_two_site(tmap, mats, trunc, absorb):
-    """Sector SVDs of the two-site matrices, global truncation, normalised
-    singular values; returns (A1, A2, s, svd) with s absorbed into A2
-    ("right") or A1 ("left")."""
+def _svd(M):
+    """stable_tsvd (auxiliary/stable_svd.jl:4-18): divide-and-conquer LAPACK
+    SVD, falling back to the QR-iteration driver if it fails."""
+    import scipy.linalg as sla
+    try:
+        return sla.svd(M, full_matrices=False, lapack_driver="gesdd", check_finite=False)
+    except (np.linalg.LinAlgError, ValueError):
+        return sla.svd(M, full_matrices=False, lapack_driver="gesvd", check_finite=False)
+
+
+def _sector_svds(mats):
+    """FP64 SVD of every sector matrix on the host cores (LAPACK threads
+    limited to one, sectors spread over a thread pool, largest first)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from threadpoolctl import threadpool_limits
+    keys = sorted(mats, key=lambda c: -mats[c].size)
+    workers = max(1, min(16, len(keys), (__import__("os").cpu_count() or 1)))
+    with threadpool_limits(1):
+        with ThreadPoolExecutor(workers) as ex:
+            res = list(ex.map(lambda c: _svd(mats[c]), keys))
+    return dict(zip(keys, res))
+
+
+def split_two_site(tmap, vec_host, trunc, absorb, device):
+    """Sector SVDs of the two-site matrices, truncation on the merged
+    spectrum (truncation.jl:27-47), normalised singular values (dmrg.jl:105);
+    s is absorbed into A2 ("right": u | s v) or A1 ("left": u s | v).
+    Returns (A1, A2, {c: s_c})."""
     xl = tmap.xl
-    svd = {}
-    for c in tmap.charges:
-        M = mats[c]
-        try:
-            U, S, Vh = torch.linalg.svd(M, full_matrices=False)
-        except RuntimeError:
-            U, S, Vh = torch.linalg.svd(M, full_matrices=False, driver="gesvd")
-        svd[c] = (U, S, Vh)
-    keep = _global_keep(svd, trunc)
+    svd = _sector_svds(tmap.matrices(vec_host))
+    keep = _global_keep_host(svd, trunc)
     norm = float(np.sqrt(sum(float((svd[c][1][:k] ** 2).sum()) for c, k in keep.items())))
     mid = Space(keep.items())
     l1 = BlockLayout(xl.lspace, xl.phys, 1, mid)
     l2 = BlockLayout(mid, xl.phys, 1, xl.rspace)
-    dev = mats[tmap.charges[0]].device
-    d1 = torch.zeros(l1.size, dtype=torch.float64, device=dev)
-    d2 = torch.zeros(l2.size, dtype=torch.float64, device=dev)
+    d1 = np.zeros(l1.size)
+    d2 = np.zeros(l2.size)
     svals = {}
     for c, k in keep.items():
         U, S, Vh = svd[c]
         s = S[:k] / norm
         svals[c] = s
-        u = U[:, :k]
-        v = Vh[:k]
+        u, v = U[:, :k], Vh[:k]
         if absorb == "right":
-            v = v * s.reshape(-1, 1)
+            v = v * s[:, None]
         else:
-            u = u * s.reshape(1, -1)
+            u = u * s[None, :]
+        ro = 0
+        for qa, s1, Da in tmap.rows[c]:
+            b = l1.index[(qa, (s1,))]
+            d1[b.off:b.off + Da * k] = u[ro:ro + Da].T.ravel()
+            ro += Da
+        co = 0
+        for s2, qb, Db in tmap.cols[c]:
+            b = l2.index[(c, (s2,))]
+            d2[b.off:b.off + k * Db] = v[:, co:co + Db].T.ravel()
+            co += Db
+    A1 = BlockTensor(l1, torch.from_numpy(d1).to(device))
+    A2 = BlockTensor(l2, torch.from_numpy(d2).to(device))
+    return A1, A2, svals
+
+
+def split_two_site_rho(tmap, vec, trunc, absorb, perm_dev=None):
+    """Device split by the sector reduced density matrices (White's original
+    DMRG step): for the left sweep rho_c = M_c M_c^T gives u and s^2, and
+    s v = u^T M_c is an exact projection; for the right sweep rho_c = M_c^T M_c
+    gives v and u s = M_c v^T.  The eigenproblems of one size class run as one
+    batched eigh; the truncation acts on the merged spectrum s = sqrt(lambda)
+    exactly as for the SVD.  Returns (A1, A2, {c: s_c})."""
+    xl = tmap.xl
+    dev = vec.device
+    perm = perm_dev if perm_dev is not None else torch.as_tensor(tmap.perm, device=dev)
+    cat = vec[perm]
+    left = absorb == "right"
+    mats, grams = {}, {}
+    for c in tmap.charges:
+        R, C = tmap.shape[c]
+        M = cat[tmap.base[c]:tmap.base[c] + R * C].view(R, C)
+        mats[c] = M
+        grams[c] = M @ M.t() if left else M.t() @ M
+    # batched eigh per size class (multiples of 32), zero padding
+    classes = {}
+    for c, G in grams.items():
+        classes.setdefault(-(-G.shape[0] // 32) * 32, []).append(c)
+    eig = {}
+    lam_host = {}
+    pending = []
+    for nb, cs in classes.items():
+        Gb = torch.zeros((len(cs), nb, nb), dtype=torch.float64, device=dev)
+        for j, c in enumerate(cs):
+            n = grams[c].shape[0]
+            Gb[j, :n, :n] = grams[c]
+        lam, V = torch.linalg.eigh(Gb)
+        pending.append((nb, cs, lam, V))
+    lam_all = torch.cat([p[2].reshape(-1) for p in pending]).cpu().numpy() if pending else None
+    pos = 0
+    for nb, cs, lam, V in pending:
+        lh = lam_all[pos:pos + lam.numel()].reshape(lam.shape)
+        pos += lam.numel()
+        for j, c in enumerate(cs):
+            R, C = tmap.shape[c]
+            r = min(R, C)
+            # descending; only the top min(R, C) can be nonzero
+            l = np.maximum(lh[j, ::-1][:r], 0.0)
+            lam_host[c] = np.sqrt(l)
+            eig[c] = (V, j, nb)
+    keep = _global_keep_host({c: (None, s) for c, s in lam_host.items()}, trunc)
+    keep = {c: k for c, k in keep.items() if lam_host[c][k - 1] > 0.0}
+    norm = float(np.sqrt(sum(float((lam_host[c][:k] ** 2).sum()) for c, k in keep.items())))
+    mid = Space(keep.items())
+    l1 = BlockLayout(xl.lspace, xl.phys, 1, mid)
+    l2 = BlockLayout(mid, xl.phys, 1, xl.rspace)
+    d1 = torch.zeros(l1.size, dtype=torch.float64, device=dev)
+    d2 = torch.zeros(l2.size, dtype=torch.float64, device=dev)
+    svals = {}
+    for c, k in keep.items():
+        V, j, nb = eig[c]
+        M = mats[c]
+        n = grams[c].shape[0]
+        Q = V[j, :n, nb - k:].flip(-1)          # top-k eigenvectors, descending
+        svals[c] = lam_host[c][:k] / norm
+        if left:
+            u = Q                                # R x k
+            v = (Q.t() @ M) / norm               # s v, normalised s
+        else:
+            v = Q.t()                            # k x C
+            u = (M @ Q) / norm                   # u s
         ro = 0
         for qa, s1, Da in tmap.rows[c]:
             b = l1.index[(qa, (s1,))]
@@ -419,31 +522,41 @@ def split_two_site(tmap, mats, trunc, absorb):
             b = l2.index[(c, (s2,))]
             d2[b.off:b.off + k * Db] = v[:, co:co + Db].t().reshape(-1)
             co += Db
-    return BlockTensor(l1, d1), BlockTensor(l2, d2), svals, svd, keep, norm
+    return BlockTensor(l1, d1), BlockTensor(l2, d2), svals
+
+
+def _global_keep_host(svd, trunc):
+    keys = list(svd.keys())
+    vals = [svd[q][1] for q in keys]
+    allv = np.concatenate(vals)
+    owner = np.concatenate([np.full(len(v), j) for j, v in enumerate(vals)])
+    order = np.argsort(-allv, kind="stable")
+    k, _ = trunc.truncate(allv[order])
+    k = max(k, 1)
+    counts = np.bincount(owner[order[:k]], minlength=len(keys))
+    return {q: int(c) for q, c in zip(keys, counts) if c > 0}
 
 
 def _bond_step(m, alg, site, absorb):
     mps, hs = m.mps, m.hstorage
     A1, A2 = mps[site], mps[site + 1]
+    dev = A1.data.device
+    t0 = time.perf_counter()
     x = two_site(A1, A2)
     L, R = hs[site], hs[site + 2]
-    t0 = time.perf_counter()
     tab = bs.plan_heff2(x.layout, L.layout, R.layout, m.mpo[site], m.mpo[site + 1])
     plan = bs.Plan(tab, alg.prec)
     t1 = time.perf_counter()
     ev, vec, info, nit = plan.lanczos(L.data, R.data, x.data, alg.eig_maxiter, alg.eig_tol,
                                       reorth=alg.reorth)
     t2 = time.perf_counter()
-    tmap = TwoSiteMap(x.layout, x.data.device)
-    mats = tmap.matrices(vec)
-    n1, n2, svals, svd, keep, norm = split_two_site(tmap, mats, alg.trunc, absorb)
-    # err_1 = <u s v, twositemps> (dmrg.jl:109-110) over the kept sectors
-    old = tmap.matrices(x.data)
-    e = 0.0
-    for c, k in keep.items():
-        U, S, Vh = svd[c]
-        usv = (U[:, :k] * (S[:k] / norm).reshape(1, -1)) @ Vh[:k]
-        e += float((usv * old[c]).sum())
+    tmap = TwoSiteMap(x.layout)
+    if getattr(alg, "split", "rho") == "svd":
+        n1, n2, svals = split_two_site(tmap, vec.cpu().numpy(), alg.trunc, absorb, dev)
+    else:
+        n1, n2, svals = split_two_site_rho(tmap, vec, alg.trunc, absorb)
+    # err_1 = <u s v, twositemps> (dmrg.jl:109-110)
+    e = torch.dot(two_site(n1, n2, x.layout).data, x.data).item()
     t3 = time.perf_counter()
     st = alg.stats
     st["plan_s"] = st.get("plan_s", 0.0) + (t1 - t0)
