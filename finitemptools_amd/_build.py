@@ -13,6 +13,7 @@ SOURCES = [
     "mh_heff.hip",
     "mh_krylov.hip",
     "mh_blocksparse.hip",
+    "mh_adjoint.hip",
 ]
 ARCH = "gfx950"
 

@@ -72,6 +72,8 @@ SIGNATURES = {
     "mh_gemm": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_int64,
                         c_int64, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                         c_int64, c_double, c_double, c_double, c_double]),
+    "mh_mpo_grad": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mh_bs_plan_create": (c_int, [c_void_p, c_int, P(c_int64), P(c_void_p)]),
     "mh_bs_plan_add_gemm": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int,
                                     c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),

@@ -104,13 +104,15 @@ def tfim_mpo(L, J=1.0, h=1.0, dtype=np.float64):
     return schur_mpo(L, ops, [(-h, "z")], [(-J, ["x", "x"])], dtype=dtype)
 
 
-def j1j2_mpo(L, J1=1.0, J2=0.0):
+def j1j2_mpo(L, J1=1.0, J2=0.0, keep_j2=False):
     """H = J1 sum S_i.S_{i+1} + J2 sum S_i.S_{i+2} with S = sigma/2; w = 8
-    (the J2 strings share their first channel with the J1 strings)."""
+    (the J2 strings share their first channel with the J1 strings).
+    keep_j2: keep the J2 channels at J2 = 0, so that the tensors are affine in
+    J2 with one structure (what a derivative with respect to J2 needs)."""
     p = spin_half_matrices()
     ops = {"+": p["+"].real, "-": p["-"].real, "z": p["z"].real, "I": np.eye(2)}
     strings = [(J1 / 2, ["+", "-"]), (J1 / 2, ["-", "+"]), (J1 / 4, ["z", "z"])]
-    if J2 != 0.0:
+    if J2 != 0.0 or keep_j2:
         strings += [(J2 / 2, ["+", "I", "-"]), (J2 / 2, ["-", "I", "+"]),
                     (J2 / 4, ["z", "I", "z"])]
     return schur_mpo(L, ops, [], strings)

@@ -163,6 +163,18 @@ int mh_gemm(mh_handle h, int dtype, int prec, int M, int N, int K,
             int64_t scn, double alpha_re, double alpha_im, double beta_re,
             double beta_im);
 
+/* ---- adjoint (differentiation) ------------------------------------------- */
+/* G[w,s',w',s] = sum L[a,w,a'] conj(psiA[a,s',b]) psiB[a',s,b'] R[b,w',b']:
+ * the pullback of ac_prime / updateleft / updateright onto the MPO tensor,
+ * <psiA| H_eff(W) |psiB> = sum G.W.  The reference has no differentiation
+ * (Project.toml:6-11 has no ChainRulesCore); this is the kernel behind the
+ * energy gradient dE/dtheta = sum_i <G_i, dW_i/dtheta> (see the callers of
+ * ac_prime, derivatives.jl:2-4, and of updateleft, mpo/transfer.jl:31-34).
+ * L: (Dl,wl,Dl), psiA/psiB: (Dl,d,Dr), R: (Dr,wr,Dr), G: (wl,d,wr,d).
+ * Needs w*d <= 64 on both sides and (w d)^2 <= 1024.  Deterministic.      */
+int mh_mpo_grad(mh_handle h, int dtype, int prec, int Dl, int d, int Dr, int wl, int wr,
+                const void *L, const void *psiA, const void *psiB, const void *R, void *G);
+
 /* ---- block-sparse (abelian-symmetric) contractions ----------------------- */
 /* A tensor with U(1)-type charge conservation is a list of dense column-major
  * blocks in one device buffer (TensorKit GradedSpace tensors; the Hubbard

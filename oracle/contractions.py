@@ -57,6 +57,16 @@ def updateleft(hold, psiA, hj, psiB):
     return t.transpose(0, 2, 1)
 
 
+def mpo_grad(hleft, psiA, psiB, hright):
+    """Derivative of <psiA| ac_prime(psiB, W, hleft, hright) > with respect to
+    the MPO tensor W[w, s', w', s] (no counterpart in the reference):
+    G = hleft[a,w,a'] conj(psiA[a,s',b]) psiB[a',s,b'] hright[b,w',b'].
+    Pinned by sum(G * W) == vdot(psiA, ac_prime(psiB, W, hleft, hright))."""
+    t = np.tensordot(hleft, psiB, axes=([2], [0]))             # [a, w, s, b']
+    u = np.tensordot(psiA.conj(), hright, axes=([2], [0]))     # [a, s', w', b']
+    return np.einsum("awsc,atvc->wtvs", t, u)
+
+
 def l_LL(dtype=np.float64):
     """Strict boundary env (mpo/arithmetics.jl:257-258): 1x1x1 isometry."""
     return np.ones((1, 1, 1), dtype=dtype)

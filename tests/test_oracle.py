@@ -197,3 +197,34 @@ def test_exponentiate_closed_form_reference():
     x = rand(rng, 18, True)
     y = sla.expm(-0.05j * H) @ x
     assert np.isclose(np.linalg.norm(y), np.linalg.norm(x))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_mpo_grad_is_the_derivative_of_the_heff_bilinear_form(cplx):
+    """G[w,s',w',s] = d <A| ac_prime(B, W) > / dW[w,s',w',s]: brute force over
+    unit MPO tensors (pins the index order), and linearity sum G.W."""
+    rng = np.random.default_rng(21)
+    Dl, d, Dr, wl, wr = 3, 2, 4, 3, 2
+    L, R = rand(rng, (Dl, wl, Dl), cplx), rand(rng, (Dr, wr, Dr), cplx)
+    A, B = rand(rng, (Dl, d, Dr), cplx), rand(rng, (Dl, d, Dr), cplx)
+    G = C.mpo_grad(L, A, B, R)
+    assert G.shape == (wl, d, wr, d)
+    for idx in np.ndindex(G.shape):
+        E = np.zeros((wl, d, wr, d))
+        E[idx] = 1.0
+        assert abs(G[idx] - np.vdot(A, C.ac_prime(B, E, L, R))) < 1e-12
+    W = rand(rng, (wl, d, wr, d), cplx)
+    assert abs(np.sum(G * W) - np.vdot(A, C.ac_prime(B, W, L, R))) < 1e-11
+
+
+def test_hubbard_sector_energy_minimum_is_the_ground_energy():
+    """The sector restriction partitions the Hilbert space: the lowest sector
+    energy is the full ground energy, and sector sizes add to 4^L."""
+    from math import comb
+    L = 4
+    es = [ed.hubbard_sector_energy(L, a, b) for a in range(L + 1) for b in range(L + 1)]
+    full = ed.ground_energy(ed.hubbard_hamiltonian(L))[0]
+    assert abs(min(es) - full) < 1e-12
+    assert sum(comb(L, a) * comb(L, b) for a in range(L + 1) for b in range(L + 1)) == 4 ** L
+    # one electron: the lowest free-particle band energy -2t cos(pi/(L+1))
+    assert abs(ed.hubbard_sector_energy(L, 1, 0) + 2 * np.cos(np.pi / (L + 1))) < 1e-12
