@@ -108,12 +108,122 @@ def cpu_baseline(kind, D, budget_s=12.0):
             "sample": f"{n} two-site H_eff matvecs at D={D} (numpy oracle, host BLAS), {sec * 1e3:.1f} ms each"}
 
 
+def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
+    """Charge-conserving two-site DMRG of the Hubbard chain (t = 1, U = 4,
+    half filling, U1 x U1): ramp from the charge-density-wave product state
+    through D = 256 .. 2048 (one sweep each), then (a) the grouped-GEMM
+    matvec on the middle bond in each precision and (b) one more sweep per
+    precision from the same D = 2048 state."""
+    import torch
+    from finitemptools_amd import blockmps as bm
+    from finitemptools_amd import blocksparse as bs
+    from finitemptools_amd import models
+    from finitemptools_amd.dmrg import DMRG2
+    from finitemptools_amd.truncation import MPSTruncation
+    phys = bm.hubbard_charges()
+    psi = bm.prodmps(phys, [phys.index((0, 0)), phys.index((1, 1))] * (L // 2))
+    mpo = bm.BlockMPO(models.hubbard_mpo(L, 1.0, 4.0), phys)
+    env = bm.BlockExpectationCache(mpo, psi)
+    t0 = time.perf_counter()
+    ramp_log = []
+    for D in ramp:
+        alg = DMRG2(trunc=MPSTruncation(D=D, eps=1e-10))
+        ts = time.perf_counter()
+        e, _ = bm.sweep(env, alg)
+        ramp_log.append({"D": D, "s": round(time.perf_counter() - ts, 2), "E_per_site": e[-1] / L})
+    ramp_s = time.perf_counter() - t0
+    # (a) matvec on the middle bond
+    site = L // 2 - 1
+    x = bm.two_site(psi[site], psi[site + 1])
+    Le, Re = env.hstorage[site], env.hstorage[site + 2]
+    tab = bs.plan_heff2(x.layout, Le.layout, Re.layout, mpo[site], mpo[site + 1])
+    mid = psi.bond_space(site + 1)
+    mv = {"sectors": len(mid.sectors), "max_sector": max(d for _, d in mid.sectors),
+          "gflop_per_matvec": round(tab.flops() / 1e9, 3),
+          "dense_equivalent_gflop": round((4.0 * 2048 ** 3 * 6 * 16 + 4.0 * 2048 ** 2 * 36 * 64) / 1e9, 1)}
+    yref = None
+    for prec in ("f64", "f32", "bf16x3"):
+        plan = bs.Plan(tab, prec)
+        y = torch.empty_like(x.data)
+        plan.apply(x.data, Le.data, Re.data, y)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(nmatvec):
+            plan.apply(x.data, Le.data, Re.data, y)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - ts) / nmatvec
+        ent = {"ms": round(sec * 1e3, 4), "tflops": round(tab.flops() / sec / 1e12, 3)}
+        if yref is None:
+            yref = y.clone()
+        else:
+            ent["rel_err_vs_f64"] = float(torch.linalg.vector_norm(y - yref) / torch.linalg.vector_norm(yref))
+        mv[prec] = ent
+    # (b) one sweep per precision from the same state
+    base_sites = [bm.BlockTensor(t.layout, t.data.clone()) for t in psi.sites]
+    base_env = [bm.BlockEnv(q.layout, q.data.clone()) for q in env.hstorage]
+    sweeps = {}
+    for prec in ("f64", "f32", "bf16x3"):
+        psi.sites = [bm.BlockTensor(t.layout, t.data.clone()) for t in base_sites]
+        env.hstorage = [bm.BlockEnv(q.layout, q.data.clone()) for q in base_env]
+        env.prec = prec
+        alg = DMRG2(trunc=MPSTruncation(D=ramp[-1], eps=1e-10), prec=prec)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        e, delta = bm.sweep(env, alg)
+        torch.cuda.synchronize()
+        st = alg.stats
+        sweeps[prec] = {"s": round(time.perf_counter() - ts, 3), "E_per_site": e[-1] / L,
+                        "lanczos_s": round(st["lanczos_s"], 3), "split_s": round(st.get("split_s", 0), 3),
+                        "plan_s": round(st.get("plan_s", 0), 3), "env_s": round(st["env_s"], 3),
+                        "lanczos_tflops": round(st["matvec_flops"] / max(st["lanczos_s"], 1e-9) / 1e12, 3)}
+    for prec in ("f32", "bf16x3"):
+        sweeps[prec]["dE_per_site_vs_f64"] = sweeps[prec]["E_per_site"] - sweeps["f64"]["E_per_site"]
+        sweeps[prec]["speedup_vs_f64"] = round(sweeps["f64"]["s"] / sweeps[prec]["s"], 3)
+    return {"workload": f"Hubbard L={L} t=1 U=4 half filling U1xU1, DMRG2 ramp {list(ramp)}",
+            "ramp_s": round(ramp_s, 2), "ramp": ramp_log, "matvec_middle_bond": mv,
+            "sweep_D2048": sweeps}
+
+
+def gradients_j1j2(L=40, D=256, J2=0.25):
+    """dE/dJ2 and dE/dA of E = <psi|H|psi>/<psi|psi> for the J1-J2 chain on a
+    random D = 256 MPS (the cost does not depend on convergence), with stored
+    and with recomputed environments."""
+    import torch
+    from finitemptools_amd import gradients as GR
+    from finitemptools_amd import models
+    from finitemptools_amd.envs import mpo_to_device
+    from finitemptools_amd.mps import randommps
+    psi = randommps(L, 2, D, seed=0).right_canonicalize()
+    sites = [psi[i] for i in range(L)]
+    mpo = mpo_to_device(models.j1j2_mpo(L, 1.0, J2, keep_j2=True))
+    W1 = mpo_to_device(models.j1j2_mpo(L, 1.0, 1.0, keep_j2=True))
+    W0 = mpo_to_device(models.j1j2_mpo(L, 1.0, 0.0, keep_j2=True))
+    dW = [a - b for a, b in zip(W1, W0)]
+    out = {"workload": f"J1-J2 L={L} D={D} J2={J2}, random right-canonical MPS"}
+    GR.energy_gradients(mpo, sites, mode="store")  # warm-up
+    for mode in ("store", "recompute"):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        res = GR.energy_gradients(mpo, sites, mode=mode)
+        torch.cuda.synchronize()
+        g = GR.dE_dtheta(dW, res)
+        out[mode] = {"s": round(time.perf_counter() - ts, 4), "dE_dJ2": g,
+                     "resident_env_MB": round(res.stats["resident_env_bytes"] / 2 ** 20, 2),
+                     "right_env_updates": res.stats["right_updates"]}
+    hf = GR.hellmann_feynman(mpo_to_device(models.j1j2_dJ2_mpo(L)), sites)
+    out["hellmann_feynman"] = hf
+    out["adjoint_minus_hf"] = out["store"]["dE_dJ2"] - hf
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-blocksparse", action="store_true")
+    ap.add_argument("--no-gradients", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the H_eff engine has no multi-GPU path "
@@ -175,6 +285,12 @@ def main():
                      "avg_launch_ms": round(avg_ms, 5), "launches": gn,
                      "wstage_avg_ms": round(wms / max(wn, 1), 5)},
     }
+    if not args.no_blocksparse:
+        print("block-sparse Hubbard ramp ...", file=sys.stderr, flush=True)
+        out["blocksparse_hubbard_L64_D2048"] = blocksparse_hubbard()
+    if not args.no_gradients:
+        print("gradients ...", file=sys.stderr, flush=True)
+        out["gradients_j1j2_L40_D256"] = gradients_j1j2()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(2, D2)
     print(json.dumps(out))

@@ -346,7 +346,7 @@ class TwoSiteMap:
         self.rows = {c: rows[c] for c in self.charges}
         self.cols = {c: cols[c] for c in self.charges}
         self.shape, self.base = {}, {}
-        perm = np.empty(xl.size, dtype=np.int64)
+        pieces = []
         base = 0
         for c in self.charges:
             R = sum(r[2] for r in self.rows[c])
@@ -355,21 +355,77 @@ class TwoSiteMap:
             ro = 0
             for qa, s1, Da in self.rows[c]:
                 co = 0
-                I = np.arange(Da)[:, None]
                 for s2, qb, Db in self.cols[c]:
                     b = xl.index[(qa, (s1, s2))]
-                    J = np.arange(Db)[None, :]
-                    perm[base + (ro + I) * C + (co + J)] = b.off + I + J * Da
+                    pieces.append((b.off, Da, Db, base, C, ro, co))
                     co += Db
                 ro += Da
             base += R * C
         assert base == xl.size
-        self.perm = perm
+        self.pieces = np.array(pieces, dtype=np.int64).reshape(-1, 7)
+        self._perm = None
+        self._dev = {}
+
+    @property
+    def perm(self):
+        """host permutation: flat[perm] is the row-major concatenation of all M_c"""
+        if self._perm is None:
+            blk, mat = piece_index_host(self.pieces)
+            self._perm = np.empty(self.xl.size, dtype=np.int64)
+            self._perm[mat] = blk
+        return self._perm
+
+    def dest(self, device):
+        """device index: cat[dest] = flat scatters the blocks into the matrices"""
+        key = str(device)
+        if key not in self._dev:
+            blk, mat = piece_index(self.pieces, device)
+            d = torch.empty(self.xl.size, dtype=torch.int64, device=device)
+            d[blk] = mat
+            self._dev[key] = d
+        return self._dev[key]
 
     def matrices(self, flat_host):
         cat = flat_host[self.perm]
         return {c: cat[self.base[c]:self.base[c] + self.shape[c][0] * self.shape[c][1]]
                 .reshape(self.shape[c]) for c in self.charges}
+
+
+def piece_index_host(pieces):
+    """Element indices of block <-> matrix pieces.  Row p of ``pieces`` =
+    (blk_off, nr, nc, mat_base, ncols, r0, c0): element (i, j) of the piece is
+    at blk_off + i + j*nr in a column-major block buffer and at
+    mat_base + (r0 + i)*ncols + c0 + j in a row-major matrix buffer.
+    Returns (blk_idx, mat_idx) over all pieces."""
+    if len(pieces) == 0:
+        z = np.zeros(0, dtype=np.int64)
+        return z, z
+    P = pieces
+    sizes = P[:, 1] * P[:, 2]
+    pid = np.repeat(np.arange(len(P)), sizes)
+    start = np.cumsum(sizes) - sizes
+    local = np.arange(int(sizes.sum())) - start[pid]
+    nr = P[pid, 1]
+    i, j = local % nr, local // nr
+    return P[pid, 0] + local, P[pid, 3] + (P[pid, 5] + i) * P[pid, 4] + P[pid, 6] + j
+
+
+def piece_index(pieces, device):
+    """piece_index_host computed on the device (one pass, no host loop)."""
+    if len(pieces) == 0:
+        z = torch.zeros(0, dtype=torch.int64, device=device)
+        return z, z
+    P = torch.as_tensor(pieces, device=device)
+    sizes = P[:, 1] * P[:, 2]
+    total = int((pieces[:, 1] * pieces[:, 2]).sum())
+    pid = torch.repeat_interleave(torch.arange(len(pieces), device=device), sizes,
+                                  output_size=total)
+    start = torch.cumsum(sizes, 0) - sizes
+    local = torch.arange(total, device=device) - start[pid]
+    Pp = P[pid]
+    nr = Pp[:, 1]
+    i, j = local % nr, local // nr
+    return Pp[:, 0] + local, Pp[:, 3] + (Pp[:, 5] + i) * Pp[:, 4] + Pp[:, 6] + j
 
 
 def two_site_layout(A1, A2):
@@ -447,7 +503,7 @@ def split_two_site(tmap, vec_host, trunc, absorb, device):
     return A1, A2, svals
 
 
-def split_two_site_rho(tmap, vec, trunc, absorb, perm_dev=None):
+def split_two_site_rho(tmap, vec, trunc, absorb):
     """Device split by the sector reduced density matrices (White's original
     DMRG step): for the left sweep rho_c = M_c M_c^T gives u and s^2, and
     s v = u^T M_c is an exact projection; for the right sweep rho_c = M_c^T M_c
@@ -456,8 +512,8 @@ def split_two_site_rho(tmap, vec, trunc, absorb, perm_dev=None):
     exactly as for the SVD.  Returns (A1, A2, {c: s_c})."""
     xl = tmap.xl
     dev = vec.device
-    perm = perm_dev if perm_dev is not None else torch.as_tensor(tmap.perm, device=dev)
-    cat = vec[perm]
+    cat = torch.empty_like(vec)
+    cat[tmap.dest(dev)] = vec
     left = absorb == "right"
     mats, grams = {}, {}
     for c in tmap.charges:
@@ -465,10 +521,17 @@ def split_two_site_rho(tmap, vec, trunc, absorb, perm_dev=None):
         M = cat[tmap.base[c]:tmap.base[c] + R * C].view(R, C)
         mats[c] = M
         grams[c] = M @ M.t() if left else M.t() @ M
-    # batched eigh per size class (multiples of 32), zero padding
+    # Batched eigh per size class with zero padding.  A batched call costs a
+    # fixed latency plus roughly n^3 per matrix, so the classes are
+    # geometric: a new class starts when a matrix is below 0.55 x the
+    # largest of the current one.
     classes = {}
-    for c, G in grams.items():
-        classes.setdefault(-(-G.shape[0] // 32) * 32, []).append(c)
+    cur = None
+    for c in sorted(grams, key=lambda c: -grams[c].shape[0]):
+        n = grams[c].shape[0]
+        if cur is None or n < 0.55 * cur:
+            cur = -(-n // 32) * 32
+        classes.setdefault(cur, []).append(c)
     eig = {}
     lam_host = {}
     pending = []
@@ -497,14 +560,13 @@ def split_two_site_rho(tmap, vec, trunc, absorb, perm_dev=None):
     mid = Space(keep.items())
     l1 = BlockLayout(xl.lspace, xl.phys, 1, mid)
     l2 = BlockLayout(mid, xl.phys, 1, xl.rspace)
-    d1 = torch.zeros(l1.size, dtype=torch.float64, device=dev)
-    d2 = torch.zeros(l2.size, dtype=torch.float64, device=dev)
-    svals = {}
+    svals, us, vs, upieces, vpieces = {}, [], [], [], []
+    ubase = vbase = 0
     for c, k in keep.items():
         V, j, nb = eig[c]
         M = mats[c]
-        n = grams[c].shape[0]
-        Q = V[j, :n, nb - k:].flip(-1)          # top-k eigenvectors, descending
+        R, C = tmap.shape[c]
+        Q = V[j, :grams[c].shape[0], nb - k:].flip(-1)   # top-k eigenvectors, descending
         svals[c] = lam_host[c][:k] / norm
         if left:
             u = Q                                # R x k
@@ -512,16 +574,24 @@ def split_two_site_rho(tmap, vec, trunc, absorb, perm_dev=None):
         else:
             v = Q.t()                            # k x C
             u = (M @ Q) / norm                   # u s
+        us.append(u.reshape(-1))
+        vs.append(v.reshape(-1))
         ro = 0
         for qa, s1, Da in tmap.rows[c]:
-            b = l1.index[(qa, (s1,))]
-            d1[b.off:b.off + Da * k] = u[ro:ro + Da].t().reshape(-1)
+            upieces.append((l1.index[(qa, (s1,))].off, Da, k, ubase, k, ro, 0))
             ro += Da
         co = 0
         for s2, qb, Db in tmap.cols[c]:
-            b = l2.index[(c, (s2,))]
-            d2[b.off:b.off + k * Db] = v[:, co:co + Db].t().reshape(-1)
+            vpieces.append((l2.index[(c, (s2,))].off, k, Db, vbase, C, 0, co))
             co += Db
+        ubase += R * k
+        vbase += k * C
+    d1 = torch.zeros(l1.size, dtype=torch.float64, device=dev)
+    d2 = torch.zeros(l2.size, dtype=torch.float64, device=dev)
+    blk, mat = piece_index(np.array(upieces, dtype=np.int64).reshape(-1, 7), dev)
+    d1[blk] = torch.cat(us)[mat]
+    blk, mat = piece_index(np.array(vpieces, dtype=np.int64).reshape(-1, 7), dev)
+    d2[blk] = torch.cat(vs)[mat]
     return BlockTensor(l1, d1), BlockTensor(l2, d2), svals
 
 
@@ -551,14 +621,19 @@ def _bond_step(m, alg, site, absorb):
                                       reorth=alg.reorth)
     t2 = time.perf_counter()
     tmap = TwoSiteMap(x.layout)
+    t2a = time.perf_counter()
     if getattr(alg, "split", "rho") == "svd":
         n1, n2, svals = split_two_site(tmap, vec.cpu().numpy(), alg.trunc, absorb, dev)
     else:
         n1, n2, svals = split_two_site_rho(tmap, vec, alg.trunc, absorb)
+    t2b = time.perf_counter()
     # err_1 = <u s v, twositemps> (dmrg.jl:109-110)
     e = torch.dot(two_site(n1, n2, x.layout).data, x.data).item()
     t3 = time.perf_counter()
     st = alg.stats
+    st["map_s"] = st.get("map_s", 0.0) + (t2a - t2)
+    st["split_s"] = st.get("split_s", 0.0) + (t2b - t2a)
+    st["overlap_s"] = st.get("overlap_s", 0.0) + (t3 - t2b)
     st["plan_s"] = st.get("plan_s", 0.0) + (t1 - t0)
     st["lanczos_s"] += t2 - t1
     st["svd_s"] += t3 - t2
