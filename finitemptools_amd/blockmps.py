@@ -8,8 +8,11 @@ the DMRG2 sweeps (algorithms/dmrg.jl:89-144) with ``stable_tsvd`` truncation
 applied globally across sectors (auxiliary/truncation.jl:27-47).
 
 Contractions (two-site tensor, H_eff, environment updates) run as grouped
-plans (blocksparse.py); the Lanczos recurrence stays on the device; the
-sector SVDs and QRs are FP64 torch.linalg calls on the device.
+plans (blocksparse.py); the Lanczos recurrence stays on the device.  The
+two-site split diagonalises the sector reduced density matrices with batched
+FP64 eigh on the device (``split_two_site_rho``, the default) or takes the
+sector SVDs with host LAPACK (``split_two_site``, ``alg.split = "svd"``); the
+sector QRs of the canonical form are FP64 torch.linalg calls on the device.
 """
 import time
 
