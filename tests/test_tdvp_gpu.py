@@ -69,3 +69,32 @@ def test_imaginary_time_tdvp2_reaches_ground_state(gpu):
     e = expectation(mpo, psi) / expectation(ident, psi)
     ED = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ed_energies.json")))
     assert abs(e - ED["xxz_L12_J1_D1"]) < 1e-6
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("bf16x3", 5e-5)])
+def test_mixed_precision_quench_trajectory(gpu, prec, tol):
+    """Complex FP32 / split-BF16 contractions (environments and H_eff) with
+    FP64 re-orthogonalised Krylov: the <sigma^z_i(t)> trajectory of the TFIM
+    quench stays within tol of the FP64 run at every step."""
+    import torch
+    from finitemptools_amd import tdvp
+    from finitemptools_amd.envs import environments, local_expectation, mpo_to_device
+    from finitemptools_amd.mps import prodmps
+    from finitemptools_amd.truncation import MPSTruncation
+    L, h, dt, nsteps = 10, 1.0, 0.05, 10
+    mpo = mpo_to_device(models.tfim_mpo(L, 1.0, h, dtype=np.complex128), torch.complex128)
+    traj = {}
+    for p in ("f64", prec):
+        psi = prodmps([np.ones(2) / np.sqrt(2)] * L, dtype=torch.complex128)
+        alg = tdvp.TDVP2(stepsize=-1j * dt, trunc=MPSTruncation(D=32, eps=1e-12), prec=p)
+        env = environments(mpo, psi, p)
+        rows = []
+        for _ in range(nsteps):
+            tdvp.sweep(env, alg)
+            rows.append([local_expectation(psi, ed.SZ, i).real for i in range(L)])
+        traj[p] = np.array(rows)
+    err = np.max(np.abs(traj[prec] - traj["f64"]))
+    print(f"{prec}: max |<sz>(mixed) - <sz>(f64)| over {nsteps} steps = {err:.3e}")
+    assert 0.0 < err < tol
+    ref = exact_sz(L, h, dt * nsteps, plus_state(L))
+    assert np.max(np.abs(traj[prec][-1] - ref)) < max(1e-4, 2 * tol)
