@@ -65,9 +65,6 @@ def gpu_run(kind, D, steps, warmup, prec="f64", profile=False):
         for x, y in zip(xd, yd):
             op.apply(x, out=y)
     torch.cuda.synchronize()
-    if profile:
-        eng.profile_reset()
-        eng.profile_enable(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         for x, y in zip(xd, yd):
@@ -76,6 +73,14 @@ def gpu_run(kind, D, steps, warmup, prec="f64", profile=False):
     t1 = time.perf_counter()
     prof = None
     if profile:
+        # a second pass of the same steps with HIP events around every launch
+        # (kept out of the timed pass above: the events add gaps)
+        eng.profile_reset()
+        eng.profile_enable(True)
+        for _ in range(steps):
+            for x, y in zip(xd, yd):
+                op.apply(x, out=y)
+        torch.cuda.synchronize()
         eng.profile_enable(False)
         prof = {k: eng.profile_read(c) for k, c in
                 (("gemm", _lib.MH_K_GEMM), ("wstage", _lib.MH_K_WSTAGE))}
@@ -106,6 +111,19 @@ def cpu_baseline(kind, D, budget_s=12.0):
     return {"value": round(flops(kind, D) / sec / 1e12, 4), "unit": "TFLOP/s", "cores": cores,
             "kind": "port",
             "sample": f"{n} two-site H_eff matvecs at D={D} (numpy oracle, host BLAS), {sec * 1e3:.1f} ms each"}
+
+
+def pmc_traffic():
+    """HBM bytes per GEMM launch of the timed D=512 matvec, as measured by the
+    two counter runs summarised in profiles/ (None if the summary is absent)."""
+    path = os.path.join(ROOT, "profiles", "heff_f64_ac2_D512_pmc_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
@@ -224,6 +242,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-blocksparse", action="store_true")
     ap.add_argument("--no-gradients", action="store_true")
+    ap.add_argument("--only", choices=("all", "blocksparse"), default="all",
+                    help="blocksparse: run only the block-sparse section (profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="first copy MiB of device memory three times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the H_eff engine has no multi-GPU path "
@@ -234,6 +256,20 @@ def main():
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
     from oracle import contractions as C
+
+    if args.copy_calibration:
+        # a device copy of known size: HBM counters of this kernel calibrate
+        # the counter readings of the others (profiling runs only)
+        n = args.copy_calibration * 2 ** 20 // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)   # elementwise: reads and writes MiB each
+        torch.cuda.synchronize()
+        del a, b
+    if args.only == "blocksparse":
+        print(json.dumps({"blocksparse_hubbard_L64_D2048": blocksparse_hubbard()}))
+        return
 
     D2, D1 = 512, 1024
     r2 = gpu_run(2, D2, args.steps, args.warmup, profile=True)
@@ -279,7 +315,10 @@ def main():
         "ac2_prime_D512_mixed": mixed,
         "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (both GEMM stages of ac2_prime)",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": pmc_traffic(),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                     "copy-kernel calibrated)",
+                     "algorithmic_bytes": 60.82e6,
                      "measured_peak": round(peak_meas.value, 2),
                      "frac_of_measured_peak": round(achieved / peak_meas.value, 4),
                      "avg_launch_ms": round(avg_ms, 5), "launches": gn,

@@ -354,6 +354,8 @@ static const Cand kCands[] = {
     {128, 128, 16, 1, 0}, {64, 64, 16, 2, 0}, {64, 64, 32, 2, 0}, {128, 64, 16, 2, 0},
     {64, 64, 16, 1, 8}, {64, 64, 32, 1, 8}, {64, 64, 16, 2, 8}, {64, 64, 32, 2, 8},
     {128, 64, 16, 1, 8}, {64, 128, 16, 1, 8},
+    // narrow tiles: more workgroups for GEMMs with few output tiles and long K
+    {32, 64, 16, 1, 0}, {64, 32, 16, 1, 0}, {32, 64, 32, 1, 0}, {64, 32, 32, 1, 0},
 };
 constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
 
@@ -362,7 +364,11 @@ static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
   dim3 grid = setup_grid(c, p);
 #define MH_L(BM, BN, BK) \
   hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC>), grid, dim3(256), 0, s, p)
-  if (c.bm == 64 && c.bn == 64 && c.bk == 16) MH_L(64, 64, 16);
+  if (c.bm == 32 && c.bk == 16) MH_L(32, 64, 16);
+  else if (c.bm == 32) MH_L(32, 64, 32);
+  else if (c.bn == 32 && c.bk == 16) MH_L(64, 32, 16);
+  else if (c.bn == 32) MH_L(64, 32, 32);
+  else if (c.bm == 64 && c.bn == 64 && c.bk == 16) MH_L(64, 64, 16);
   else if (c.bm == 64 && c.bn == 64) MH_L(64, 64, 32);
   else if (c.bm == 128 && c.bn == 64) MH_L(128, 64, 16);
   else if (c.bm == 64 && c.bn == 128) MH_L(64, 128, 16);
