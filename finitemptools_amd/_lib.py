@@ -65,6 +65,7 @@ SIGNATURES = {
                                P(c_double), c_void_p, P(c_int), P(c_int)]),
     "mh_lanczos_expm": (c_int, [c_void_p, P(HeffOp), c_void_p, c_double, c_double, c_int,
                                 c_double, c_void_p, P(c_int)]),
+    "mh_set_graphs": (c_int, [c_void_p, c_int]),
     "mh_dot": (c_int, [c_void_p, c_int, c_size_t, c_void_p, c_void_p, P(c_double)]),
     "mh_nrm2": (c_int, [c_void_p, c_int, c_size_t, c_void_p, P(c_double)]),
     "mh_axpby": (c_int, [c_void_p, c_int, c_size_t, c_double, c_double, c_void_p,
@@ -80,6 +81,8 @@ SIGNATURES = {
     "mh_bs_plan_add_mode": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p,
                                     c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "mh_bs_plan_flops": (c_int, [c_void_p, P(c_double)]),
+    "mh_bs_plan_variants": (c_int, [c_void_p, P(c_int)]),
+    "mh_bs_plan_force_variant": (c_int, [c_void_p, c_int]),
     "mh_bs_plan_destroy": (c_int, [c_void_p, c_void_p]),
     "mh_bs_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mh_bs_lanczos_eig": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,

@@ -380,6 +380,15 @@ class Plan:
         self.eng.check(self.lib.mh_bs_plan_flops(self.ptr, _lib.ctypes.byref(f)), "flops")
         return f.value
 
+    def variants(self):
+        n = _lib.c_int()
+        self.eng.check(self.lib.mh_bs_plan_variants(self.ptr, _lib.ctypes.byref(n)), "variants")
+        return n.value
+
+    def force_variant(self, v):
+        """fix the grouped-GEMM tile variant of every stage (-1: tune)"""
+        self.eng.check(self.lib.mh_bs_plan_force_variant(self.ptr, int(v)), "force_variant")
+
     def apply(self, x, p1, p2, y):
         h = self.eng.bind_stream()
         self.eng.check(self.lib.mh_bs_apply(h, self.ptr, _dp(x), _dp(p1), _dp(p2), _dp(y)),

@@ -10,10 +10,12 @@
 // A GEMM task is one output block C (M x N) = sum over its K-segments s of
 // opA_s (M x K_s) * opB_s (K_s x N), every operand addressed by an offset and
 // a leading dimension inside one of six buffers (vector in/out, two
-// environments or site tensors, two intermediates).  All 64x64 tiles of all
-// tasks of a stage run in ONE launch, so sectors of uneven size share the
-// machine; tiles are laid out so that each XCD works on a contiguous range of
-// the task list (L2 reuse of the operand rows a task shares).
+// environments or site tensors, two intermediates).  All tiles of all tasks
+// of a stage run in ONE launch, so sectors of uneven size share the machine;
+// tiles are laid out so that each XCD works on a contiguous range of the task
+// list (L2 reuse of the operand rows a task shares).  The tile shape (64x64,
+// 32x64, 128x64; K depth 16 or 32) is picked per shape class of the stage by
+// timing the variants on first use.
 //
 // Precisions: FP64 (v_mfma_f64_16x16x4f64), and the mixed forms FP32 and
 // split-BF16 (hi*hi + hi*lo + lo*hi, v_mfma_f32_16x16x32_bf16) with FP32
@@ -57,14 +59,29 @@ struct MContrib {
 
 constexpr int NSRC = 6;
 
+// Tile shapes of the grouped GEMM: every stage carries one tile list per
+// shape; the variants below pick a shape and a K depth.
+constexpr int NTS = 3;
+constexpr int kTileBM[NTS] = {64, 32, 128};
+constexpr int kTileBN[NTS] = {64, 64, 64};
+struct GgVariant {
+  int bm, bn, bk, list;
+};
+constexpr GgVariant kGgF64[] = {{64, 64, 16, 0}, {64, 64, 32, 0}, {32, 64, 16, 1}, {128, 64, 16, 2}};
+constexpr GgVariant kGgMx[] = {{64, 64, 32, 0}, {32, 64, 32, 1}, {128, 64, 32, 2}};
+constexpr int kNF64 = 4, kNMx = 3;
+
 struct BsStage {
   int kind;                  // 0 grouped GEMM, 1 block mode product
   int a_src, b_src, c_src;   // mode: a_src = input, c_src = output
   int amc, bkc;
-  int ntile, nchunk;
+  int nchunk;
+  int ntile_v[NTS];          // tiles per shape (0: nothing to do)
+  GTile *tiles_v[NTS];
+  mutable int choice;        // tuned variant (-1: not yet chosen)
+  uint64_t key;              // shape class for the tuner
   GTask *tasks;
   GSeg *segs;
-  GTile *tiles;
   MChunk *chunks;
   MContrib *contribs;
   double flops;
@@ -105,12 +122,11 @@ struct GStage {
   }
 };
 
-template <int BM, int BN, bool AMC, bool BKC>
+template <int BM, int BN, int BK, bool AMC, bool BKC>
 __global__ void __launch_bounds__(256)
     k_ggemm(const GTile *__restrict__ tiles, const GTask *__restrict__ tasks,
             const GSeg *__restrict__ segs, const double *__restrict__ Abase,
             const double *__restrict__ Bbase, double *__restrict__ Cbase) {
-  constexpr int BK = 16;
   constexpr int WTM = BM / 2, WTN = BN / 2;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   using SA = GStage<BM, BK, AMC>;
@@ -213,38 +229,41 @@ __global__ void __launch_bounds__(256)
       }
     };
     auto compute = [&](int buf) {
-      double a[TM][4], b[TN][4];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int r = wm * WTM + i * 16 + lr;
-        if (AMC) {
+      for (int kb = 0; kb < BK; kb += 16) {
+        double a[TM][4], b[TN][4];
 #pragma unroll
-          for (int s = 0; s < 4; ++s) a[i][s] = sA[buf][(4 * lk + s) * SA::PITCH + r];
-        } else {
-          const double2 *p = reinterpret_cast<const double2 *>(&sA[buf][r * SA::PITCH + 4 * lk]);
-          const double2 u = p[0], v = p[1];
-          a[i][0] = u.x; a[i][1] = u.y; a[i][2] = v.x; a[i][3] = v.y;
+        for (int i = 0; i < TM; ++i) {
+          const int r = wm * WTM + i * 16 + lr;
+          if (AMC) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) a[i][s] = sA[buf][(kb + 4 * lk + s) * SA::PITCH + r];
+          } else {
+            const double2 *p = reinterpret_cast<const double2 *>(&sA[buf][r * SA::PITCH + kb + 4 * lk]);
+            const double2 u = p[0], v = p[1];
+            a[i][0] = u.x; a[i][1] = u.y; a[i][2] = v.x; a[i][3] = v.y;
+          }
         }
-      }
 #pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        const int r = wn * WTN + j * 16 + lr;
-        if (!BKC) {
+        for (int j = 0; j < TN; ++j) {
+          const int r = wn * WTN + j * 16 + lr;
+          if (!BKC) {
 #pragma unroll
-          for (int s = 0; s < 4; ++s) b[j][s] = sB[buf][(4 * lk + s) * SB::PITCH + r];
-        } else {
-          const double2 *p = reinterpret_cast<const double2 *>(&sB[buf][r * SB::PITCH + 4 * lk]);
-          const double2 u = p[0], v = p[1];
-          b[j][0] = u.x; b[j][1] = u.y; b[j][2] = v.x; b[j][3] = v.y;
+            for (int s = 0; s < 4; ++s) b[j][s] = sB[buf][(kb + 4 * lk + s) * SB::PITCH + r];
+          } else {
+            const double2 *p = reinterpret_cast<const double2 *>(&sB[buf][r * SB::PITCH + kb + 4 * lk]);
+            const double2 u = p[0], v = p[1];
+            b[j][0] = u.x; b[j][1] = u.y; b[j][2] = v.x; b[j][3] = v.y;
+          }
         }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
       }
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
     };
     const int nk = (K + BK - 1) / BK;
     __syncthreads();  // the previous segment's last compute is done with LDS
@@ -495,37 +514,118 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// One launch of variant v of a grouped-GEMM stage.  The layout with neither
+// operand contiguous along its row index is served by the 64x64 FP64 /
+// 64x64 mixed variant only (no plan of this library produces it).
 template <int PM, bool AMC, bool BKC>
-static void launch_gg(const BsStage &st, const double *A, const double *B, double *C,
+static void launch_gg(const BsStage &st, int v, const double *A, const double *B, double *C,
                       hipStream_t s) {
-  if constexpr (PM == 0)
-    hipLaunchKernelGGL((k_ggemm<64, 64, AMC, BKC>), dim3(st.ntile), dim3(256), 0, s, st.tiles,
-                       st.tasks, st.segs, A, B, C);
-  else
-    hipLaunchKernelGGL((k_ggemm_mx<PM, 64, 64, AMC, BKC>), dim3(st.ntile), dim3(256), 0, s,
-                       st.tiles, st.tasks, st.segs, A, B, C);
+  const GgVariant g = PM == 0 ? kGgF64[v] : kGgMx[v];
+  const dim3 grid(st.ntile_v[g.list]);
+  const GTile *tl = st.tiles_v[g.list];
+#define MH_GG(BM, BN, BK)                                                                    \
+  hipLaunchKernelGGL((k_ggemm<BM, BN, BK, AMC, BKC>), grid, dim3(256), 0, s, tl, st.tasks, \
+                     st.segs, A, B, C)
+#define MH_GX(BM, BN)                                                                         \
+  hipLaunchKernelGGL((k_ggemm_mx<PM, BM, BN, AMC, BKC>), grid, dim3(256), 0, s, tl, st.tasks, \
+                     st.segs, A, B, C)
+  if constexpr (PM == 0) {
+    if constexpr (AMC || BKC) {
+      if (v == 1) MH_GG(64, 64, 32);
+      else if (v == 2) MH_GG(32, 64, 16);
+      else if (v == 3) MH_GG(128, 64, 16);
+      else MH_GG(64, 64, 16);
+    } else {
+      MH_GG(64, 64, 16);
+    }
+  } else {
+    if constexpr (AMC || BKC) {
+      if (v == 1) MH_GX(32, 64);
+      else if (v == 2) MH_GX(128, 64);
+      else MH_GX(64, 64);
+    } else {
+      MH_GX(64, 64);
+    }
+  }
+#undef MH_GG
+#undef MH_GX
 }
 
 template <int PM>
-static void launch_pm(const BsStage &st, const double *A, const double *B, double *C,
+static void launch_pm(const BsStage &st, int v, const double *A, const double *B, double *C,
                       hipStream_t s) {
-  if (st.amc && st.bkc) launch_gg<PM, true, true>(st, A, B, C, s);
-  else if (st.amc) launch_gg<PM, true, false>(st, A, B, C, s);
-  else if (st.bkc) launch_gg<PM, false, true>(st, A, B, C, s);
-  else launch_gg<PM, false, false>(st, A, B, C, s);
+  if (st.amc && st.bkc) launch_gg<PM, true, true>(st, v, A, B, C, s);
+  else if (st.amc) launch_gg<PM, true, false>(st, v, A, B, C, s);
+  else if (st.bkc) launch_gg<PM, false, true>(st, v, A, B, C, s);
+  else launch_gg<PM, false, false>(st, 0, A, B, C, s);
+}
+
+static void launch_stage(mh_handle h, int prec, const BsStage &st, int v, const double *A,
+                         const double *B, double *C) {
+  if (prec == MH_PREC_F32) launch_pm<PM_F32>(st, v, A, B, C, h->stream);
+  else if (prec == MH_PREC_BF16X3) launch_pm<PM_BF16>(st, v, A, B, C, h->stream);
+  else launch_pm<0>(st, v, A, B, C, h->stream);
+}
+
+// Tile-shape autotuner: the first run of a stage whose shape class is not yet
+// in the handle's table times every variant (twice each, the faster counts)
+// on the stage's own buffers and records the fastest; later stages of the
+// same class launch it directly.  Every variant computes the same result, so
+// the stage output is valid after tuning.  Not used while a stream is being
+// captured or while profiling (both need plain launches).
+static int pick_variant(mh_handle h, int prec, const BsStage &st, const double *A, const double *B,
+                        double *C) {
+  if (st.choice >= 0) return st.choice;
+  const uint64_t key = st.key ^ ((uint64_t)prec << 60);
+  auto it = h->bs_tune.find(key);
+  if (it != h->bs_tune.end()) return st.choice = it->second;
+  const int nv = (st.amc || st.bkc) ? (prec == MH_PREC_F64 ? kNF64 : kNMx) : 1;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(h->stream, &cs);
+  if (nv == 1 || !h->autotune || h->profiling || cs != hipStreamCaptureStatusNone)
+    return st.choice = 0;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return st.choice = 0;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return st.choice = 0;
+  }
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int v = 0; v < nv; ++v) {
+    for (int rep = 0; rep < 2; ++rep) {
+      (void)hipEventRecord(e0, h->stream);
+      launch_stage(h, prec, st, v, A, B, C);
+      (void)hipEventRecord(e1, h->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    if (h->tune_log)
+      fprintf(stderr, "[mh tune] grouped gemm class %016llx prec %d variant %d: %.1f us\n",
+              (unsigned long long)key, prec, v, 1e3 * best_ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->bs_tune[key] = best;
+  return st.choice = best;
 }
 
 static int run_stage(mh_handle h, int prec, const BsStage &st, void *const *srcs) {
   if (st.kind == 0) {
-    if (st.ntile == 0) return MH_OK;
+    if (st.ntile_v[0] == 0) return MH_OK;
     const double *A = (const double *)srcs[st.a_src];
     const double *B = (const double *)srcs[st.b_src];
     double *C = (double *)srcs[st.c_src];
+    const int v = pick_variant(h, prec, st, A, B, C);
+    MH_TRY(check_launch(h, "grouped gemm tuning"));
     int slot;
     prof_begin(h, MH_K_GEMM, st.flops, &slot);
-    if (prec == MH_PREC_F32) launch_pm<PM_F32>(st, A, B, C, h->stream);
-    else if (prec == MH_PREC_BF16X3) launch_pm<PM_BF16>(st, A, B, C, h->stream);
-    else launch_pm<0>(st, A, B, C, h->stream);
+    launch_stage(h, prec, st, v, A, B, C);
     prof_end(h, slot);
     return check_launch(h, "grouped gemm");
   }
@@ -660,17 +760,6 @@ int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int 
       flops += 2.0 * t.M * (double)t.N * g.K;
     }
     tasks[i] = t;
-    for (int m0 = 0; m0 < t.M; m0 += 64)
-      for (int n0 = 0; n0 < t.N; n0 += 64) logical.push_back(GTile{i, m0, n0, 0});
-  }
-  // Blocks are dispatched round-robin over the 8 XCDs: block b runs on XCD
-  // b % 8.  XCD x gets the contiguous logical range [x*chunk, (x+1)*chunk).
-  const int nl = (int)logical.size();
-  const int chunk = (nl + 7) / 8;
-  std::vector<GTile> tiles((size_t)chunk * 8, GTile{-1, 0, 0, 0});
-  for (int b = 0; b < chunk * 8; ++b) {
-    const int l = (b % 8) * chunk + b / 8;
-    if (l < nl) tiles[b] = logical[l];
   }
   BsStage st{};
   st.kind = 0;
@@ -679,11 +768,44 @@ int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int 
   st.c_src = c_src;
   st.amc = amc ? 1 : 0;
   st.bkc = bkc ? 1 : 0;
-  st.ntile = nl > 0 ? chunk * 8 : 0;
   st.flops = flops;
+  st.choice = -1;
+  std::vector<std::vector<GTile>> lists(NTS);
+  for (int sh = 0; sh < NTS; ++sh) {
+    const int bm = kTileBM[sh], bn = kTileBN[sh];
+    logical.clear();
+    for (int i = 0; i < ntask; ++i)
+      for (int m0 = 0; m0 < tasks[i].M; m0 += bm)
+        for (int n0 = 0; n0 < tasks[i].N; n0 += bn) logical.push_back(GTile{i, m0, n0, 0});
+    // Blocks are dispatched round-robin over the 8 XCDs: block b runs on XCD
+    // b % 8.  XCD x gets the contiguous logical range [x*chunk, (x+1)*chunk).
+    const int nl = (int)logical.size();
+    const int chunk = (nl + 7) / 8;
+    lists[sh].assign((size_t)chunk * 8, GTile{-1, 0, 0, 0});
+    for (int b = 0; b < chunk * 8; ++b) {
+      const int l = (b % 8) * chunk + b / 8;
+      if (l < nl) lists[sh][b] = logical[l];
+    }
+    st.ntile_v[sh] = nl > 0 ? chunk * 8 : 0;
+    st.tiles_v[sh] = nullptr;
+  }
+  // Shape class for the tuner: layout, and log2 buckets of the mean task
+  // extents and of the task count.
+  if (ntask > 0) {
+    double sm = 0, sn = 0, sk = 0;
+    for (int i = 0; i < ntask; ++i) {
+      sm += tasks[i].M;
+      sn += tasks[i].N;
+      for (int s = tasks[i].seg0; s < tasks[i].seg0 + tasks[i].nseg; ++s) sk += segs[s].K;
+    }
+    auto lb = [](double v) { return (uint64_t)std::max(0, (int)std::lround(std::log2(std::max(v, 1.0)))); };
+    st.key = (uint64_t)(st.amc | (st.bkc << 1)) | (lb(sm / ntask) << 2) | (lb(sn / ntask) << 12) |
+             (lb(sk / ntask) << 22) | (lb((double)ntask) << 32);
+  }
   MH_TRY(upload(h, plan, tasks.data(), tasks.size(), &st.tasks));
   MH_TRY(upload(h, plan, segs.data(), segs.size(), &st.segs));
-  if (nl > 0) MH_TRY(upload(h, plan, tiles.data(), tiles.size(), &st.tiles));
+  for (int sh = 0; sh < NTS; ++sh)
+    if (st.ntile_v[sh] > 0) MH_TRY(upload(h, plan, lists[sh].data(), lists[sh].size(), &st.tiles_v[sh]));
   MH_HIP(h, hipStreamSynchronize(h->stream));
   plan->stages.push_back(st);
   plan->flops += flops;
@@ -739,6 +861,21 @@ int mh_bs_plan_destroy(mh_handle h, mh_bs_plan plan) {
   (void)hipStreamSynchronize(h->stream);
   for (void *a : plan->allocs) (void)hipFree(a);
   delete plan;
+  return MH_OK;
+}
+
+int mh_bs_plan_variants(mh_bs_plan plan, int *n) {
+  if (!plan || !n) return MH_ERR_ARG;
+  *n = plan->prec == MH_PREC_F64 ? kNF64 : kNMx;
+  return MH_OK;
+}
+
+int mh_bs_plan_force_variant(mh_bs_plan plan, int variant) {
+  if (!plan) return MH_ERR_ARG;
+  const int nv = plan->prec == MH_PREC_F64 ? kNF64 : kNMx;
+  if (variant < -1 || variant >= nv) return MH_ERR_ARG;
+  for (auto &st : plan->stages)
+    if (st.kind == 0) st.choice = (st.amc || st.bkc) ? variant : (variant < 0 ? -1 : 0);
   return MH_OK;
 }
 

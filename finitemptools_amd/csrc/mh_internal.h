@@ -47,6 +47,12 @@ struct mh_context {
   std::vector<hipEvent_t> event_pool;
   // GEMM autotuner: shape key -> index of the fastest measured configuration.
   std::unordered_map<std::string, int> tune;
+  // Grouped-GEMM tuner: stage shape class -> fastest tile variant.
+  std::unordered_map<uint64_t, int> bs_tune;
+  // Krylov recurrences as hipGraphs: captured on a private stream (a
+  // non-blocking stream at the default priority), launched on `stream`.
+  int graphs = 1;
+  hipStream_t cap_stream = nullptr;
   int autotune = 1;
   int tune_log = 0;
 };

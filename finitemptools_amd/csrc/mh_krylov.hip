@@ -526,12 +526,80 @@ struct Krylov {
   }
 };
 
+// Lanczos steps [1, kmax) as one hipGraph.  The steps have no host
+// synchronisation (alpha, beta and the early-exit flag live in device
+// memory), so they are captured once on the handle's private stream and the
+// executable graph is launched on the handle's stream; a driver that repeats
+// the same steps (the sub-steps of exponentiate) launches it again.  Step 0
+// runs eagerly before, so tuners have fixed their choices (capture skips
+// tuning).  Without graphs, or if capture is refused, the steps run eagerly.
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  hipStream_t launched_on = nullptr;
+  ~StepGraph() {
+    if (exec) {
+      if (launched_on) (void)hipStreamSynchronize(launched_on);
+      (void)hipGraphExecDestroy(exec);
+    }
+  }
+};
+
+static int run_steps(mh_handle h, Krylov &K, int kmax, double btol, int reorth, StepGraph &g) {
+  if (kmax <= 1) return MH_OK;
+  if (g.exec) {
+    g.launched_on = h->stream;
+    MH_HIP(h, hipGraphLaunch(g.exec, h->stream));
+    return MH_OK;
+  }
+  auto eager = [&]() -> int {
+    for (int i = 1; i < kmax; ++i) MH_TRY(K.step(i, btol, reorth, i + 1 < kmax));
+    return MH_OK;
+  };
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(h->stream, &cs);
+  if (!h->graphs || h->profiling || cs != hipStreamCaptureStatusNone) return eager();
+  if (!h->cap_stream &&
+      hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    h->cap_stream = nullptr;
+    return eager();
+  }
+  hipStream_t saved = h->stream;
+  h->stream = h->cap_stream;
+  if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    h->stream = saved;
+    return eager();
+  }
+  int st = MH_OK;
+  for (int i = 1; i < kmax && st == MH_OK; ++i) st = K.step(i, btol, reorth, i + 1 < kmax);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  h->stream = saved;
+  hipError_t ei = hipErrorUnknown;
+  if (st == MH_OK && e == hipSuccess && graph)
+    ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    // nothing of the captured sequence ran: do it eagerly
+    (void)hipGetLastError();
+    g.exec = nullptr;
+    h->last_error.clear();
+    return eager();
+  }
+  g.launched_on = h->stream;
+  MH_HIP(h, hipGraphLaunch(g.exec, h->stream));
+  return MH_OK;
+}
+
 // Lanczos eigensolve on a prepared Krylov driver (dense or block-sparse).
 static int lanczos_eig_run(mh_handle h, Krylov &K, const void *x0, int kmax, double btol,
                            int reorth, double *eigval, void *xout, int *info, int *niter) {
   MH_TRY(K.setup());
   MH_TRY(K.start(x0));
-  for (int i = 0; i < kmax; ++i) MH_TRY(K.step(i, btol, reorth, i + 1 < kmax));
+  MH_TRY(K.step(0, btol, reorth, kmax > 1));
+  StepGraph g;
+  MH_TRY(run_steps(h, K, kmax, btol, reorth, g));
   std::vector<double> hs;
   MH_TRY(K.read_state(hs));
   if (!(hs[3] > 0.0) || !std::isfinite(hs[3]))
@@ -658,6 +726,7 @@ int mh_lanczos_expm(mh_handle h, const mh_heff_op *op, const void *x0, double dt
   }
   Krylov K(h, *op, kdim);
   MH_TRY(K.setup());
+  StepGraph graph;
   const size_t n = heff_vec_len(*op);
   const void *src = x0;
   std::complex<double> done_t(0.0, 0.0);
@@ -671,8 +740,10 @@ int mh_lanczos_expm(mh_handle h, const mh_heff_op *op, const void *x0, double dt
       hipLaunchKernelGGL(k_fill, dim3((K.lay.total + 255) / 256), dim3(256), 0, h->stream, K.st, K.lay.total, 0.0);
     }
     MH_TRY(K.start(src));
-    // Happy breakdown threshold relative to the vector norm 1.
-    for (int i = 0; i < kdim; ++i) MH_TRY(K.step(i, 1e-13, 1, i + 1 < kdim));
+    // Happy breakdown threshold relative to the vector norm 1.  Steps
+    // 1..kdim-1 are the same launches in every sub-step: one graph.
+    MH_TRY(K.step(0, 1e-13, 1, kdim > 1));
+    MH_TRY(run_steps(h, K, kdim, 1e-13, 1, graph));
     std::vector<double> hs;
     MH_TRY(K.read_state(hs));
     const double b0 = hs[3];

@@ -132,6 +132,8 @@ int mh_create(int device, mh_handle *out) {
   // MH_AUTOTUNE=0 disables per-shape GEMM tuning; MH_TUNE_LOG=1 prints it.
   if (const char *s = getenv("MH_AUTOTUNE")) h->autotune = atoi(s) != 0;
   if (const char *s = getenv("MH_TUNE_LOG")) h->tune_log = atoi(s) != 0;
+  // MH_GRAPHS=0: run the Krylov recurrences eagerly instead of as a hipGraph.
+  if (const char *s = getenv("MH_GRAPHS")) h->graphs = atoi(s) != 0;
   *out = h;
   return MH_OK;
 }
@@ -150,7 +152,14 @@ int mh_destroy(mh_handle h) {
   if (h->pack) (void)hipFree(h->pack);
   if (h->red) (void)hipFree(h->red);
   if (h->red_host) (void)hipHostFree(h->red_host);
+  if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
   delete h;
+  return MH_OK;
+}
+
+int mh_set_graphs(mh_handle h, int enable) {
+  if (!h) return MH_ERR_ARG;
+  h->graphs = enable ? 1 : 0;
   return MH_OK;
 }
 

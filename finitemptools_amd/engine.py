@@ -110,6 +110,10 @@ class Engine:
     def profile_reset(self):
         self.check(self.lib.mh_profile_reset(self.h))
 
+    def set_graphs(self, on=True):
+        """Krylov recurrences as hipGraphs (default) or eager launches."""
+        self.check(self.lib.mh_set_graphs(self.h, 1 if on else 0))
+
     def profile_read(self, kclass):
         ms = ctypes.c_double()
         n = ctypes.c_int64()

@@ -144,6 +144,13 @@ int mh_lanczos_expm(mh_handle h, const mh_heff_op *op, const void *x0,
                     double dt_re, double dt_im, int kdim, double tol,
                     void *xout, int *info);
 
+/* enable = 1 (default; environment MH_GRAPHS=0 turns it off): the Lanczos
+ * steps after the first of mh_lanczos_eig / mh_bs_lanczos_eig /
+ * mh_lanczos_expm are captured into one hipGraph and launched as a whole
+ * (exponentiate re-launches it for every sub-step).  Results are identical
+ * to the eager launches. */
+int mh_set_graphs(mh_handle h, int enable);
+
 /* ---- Krylov vector primitives (simple_lanczos.jl:28-38,50-63) ------------ */
 /* out[0] + i out[1] = <x, y>  (conjugate-linear in x, as Julia dot) */
 int mh_dot(mh_handle h, int dtype, size_t n, const void *x, const void *y,
@@ -218,6 +225,13 @@ int mh_bs_plan_add_mode(mh_handle h, mh_bs_plan plan, int in_src, int out_src, i
                         const int64_t *dst_off, const int32_t *dst_i, int ncontrib,
                         const int64_t *src_off, const int32_t *src_ld, const double *coef);
 int mh_bs_plan_flops(mh_bs_plan plan, double *flops);  /* sum 2mnk + mode flops */
+/* Grouped-GEMM tile variants (tile shape x K depth).  By default every GEMM
+ * stage picks its variant on first use by timing all of them, cached per
+ * shape class in the handle (MH_AUTOTUNE=0: always variant 0).
+ * force_variant fixes one variant for every GEMM stage (-1: back to tuning);
+ * used by the tests to check each variant. */
+int mh_bs_plan_variants(mh_bs_plan plan, int *n);
+int mh_bs_plan_force_variant(mh_bs_plan plan, int variant);
 int mh_bs_plan_destroy(mh_handle h, mh_bs_plan plan);
 /* run the plan: y = contraction(x; p1, p2) */
 int mh_bs_apply(mh_handle h, mh_bs_plan plan, const void *x, const void *p1, const void *p2,

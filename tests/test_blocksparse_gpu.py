@@ -73,6 +73,26 @@ def test_heff2_plan_device_matches_tables(gpu, lo, hi):
     assert abs(plan.flops() - tab.flops()) <= 1e-9 * tab.flops()
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5), ("bf16x3", 1e-4)])
+def test_every_tile_variant(gpu, prec, tol):
+    """Each grouped-GEMM tile variant (shape x K depth) the tuner may pick,
+    forced on every stage, against the numpy execution of the tables; ragged
+    sector sizes so that every shape has partial tiles."""
+    tab, xl, x, Le, Re = heff_case(5, 3, 150)
+    ref = execute(tab, [x, np.zeros(xl.size), Le, Re, None, None])[1]
+    plan = bs.Plan(tab, prec)
+    assert plan.variants() == (4 if prec == "f64" else 3)
+    for v in range(plan.variants()):
+        plan.force_variant(v)
+        y = torch.full((xl.size,), float("nan"), dtype=torch.float64, device="cuda")
+        plan.apply(dev(x), dev(Le), dev(Re), y)
+        assert rel(y.cpu().numpy(), ref) < tol, f"variant {v}"
+    plan.force_variant(-1)
+    y = torch.zeros(xl.size, dtype=torch.float64, device="cuda")
+    plan.apply(dev(x), dev(Le), dev(Re), y)   # tuned
+    assert rel(y.cpu().numpy(), ref) < tol
+
+
 def test_heff2_plan_device_matches_dense_oracle(gpu):
     tab, xl, x, Le, Re = heff_case(2, 2, 9)
     y = torch.zeros(xl.size, dtype=torch.float64, device="cuda")
