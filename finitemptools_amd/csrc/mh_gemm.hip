@@ -37,15 +37,15 @@ namespace mh {
 // lk = lane/16 uses k = 4*lk + s.  The four sub-steps still cover the slice
 // once, A and B agree on the order, and a k-major fragment then reads four
 // consecutive k of one row with two 16-byte LDS reads.
-template <bool CPLX, int BR, int BK, bool RMAJ>
+template <bool CPLX, int BR, int BK, bool RMAJ, int NT = 256>
 struct Stage {
   static constexpr int NP = CPLX ? 2 : 1;
   static constexpr int PITCH = RMAJ ? BR + 4 : BK + 2;
   static constexpr int SIZE = RMAJ ? BK * PITCH : BR * PITCH;
-  static constexpr int NPAIR = BR * BK / 512;
-  static_assert(NPAIR >= 1, "tile too small for 256 threads");
+  static constexpr int NPAIR = BR * BK / (2 * NT);
+  static_assert(NPAIR >= 1, "tile too small for the thread count");
   __device__ static void coord(int t, int i, int &r, int &k) {
-    const int e = t + 256 * i;
+    const int e = t + NT * i;
     if (RMAJ) {
       r = 2 * (e % (BR / 2));
       k = e / (BR / 2);
@@ -64,16 +64,18 @@ struct Stage {
 // gridDim.z == 2 the partial tiles are added with FP64 atomics onto a C that
 // was zeroed beforehand: 0 + a + b == 0 + b + a exactly, so the result does
 // not depend on the order the two adds land (only used for beta == 0).
-template <bool CPLX, int BM, int BN, int BK, bool AMC, bool BKC, bool VEC>
-__global__ void __launch_bounds__(256)
+// NW = 4 or 8 waves per block (2 x NW/2 wave grid): eight waves give two
+// waves per SIMD to GEMMs with few output tiles (one block per CU).
+template <bool CPLX, int BM, int BN, int BK, bool AMC, bool BKC, bool VEC, int NW = 4>
+__global__ void __launch_bounds__(64 * NW)
     k_gemm(const GemmP p) {
-  constexpr int WGM = 2, WGN = 2;
+  constexpr int WGM = 2, WGN = NW / 2;
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int EW = CPLX ? 2 : 1;  // doubles per element in global memory
-  using SA = Stage<CPLX, BM, BK, AMC>;
-  using SB = Stage<CPLX, BN, BK, !BKC>;
+  using SA = Stage<CPLX, BM, BK, AMC, 64 * NW>;
+  using SB = Stage<CPLX, BN, BK, !BKC, 64 * NW>;
   static_assert(TM >= 1 && TN >= 1 && BK % 16 == 0, "tile");
   __shared__ __align__(16) double sA[NP][2][SA::SIZE];
   __shared__ __align__(16) double sB[NP][2][SB::SIZE];
@@ -354,8 +356,10 @@ static const Cand kCands[] = {
     {128, 128, 16, 1, 0}, {64, 64, 16, 2, 0}, {64, 64, 32, 2, 0}, {128, 64, 16, 2, 0},
     {64, 64, 16, 1, 8}, {64, 64, 32, 1, 8}, {64, 64, 16, 2, 8}, {64, 64, 32, 2, 8},
     {128, 64, 16, 1, 8}, {64, 128, 16, 1, 8},
-    // narrow tiles: more workgroups for GEMMs with few output tiles and long K
-    {32, 64, 16, 1, 0}, {64, 32, 16, 1, 0}, {32, 64, 32, 1, 0}, {64, 32, 32, 1, 0},
+    // eight waves per block: two waves per SIMD when a GEMM has about one
+    // output tile per CU
+    {64, 64, 16, 1, 0, 8}, {64, 64, 32, 1, 0, 8}, {128, 64, 16, 1, 0, 8},
+    {64, 64, 16, 1, 8, 8}, {64, 64, 32, 2, 0, 8},
 };
 constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
 
@@ -364,16 +368,20 @@ static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
   dim3 grid = setup_grid(c, p);
 #define MH_L(BM, BN, BK) \
   hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC>), grid, dim3(256), 0, s, p)
-  if (c.bm == 32 && c.bk == 16) MH_L(32, 64, 16);
-  else if (c.bm == 32) MH_L(32, 64, 32);
-  else if (c.bn == 32 && c.bk == 16) MH_L(64, 32, 16);
-  else if (c.bn == 32) MH_L(64, 32, 32);
+#define MH_L8(BM, BN, BK) \
+  hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC, 8>), grid, dim3(512), 0, s, p)
+  if (c.waves == 8) {
+    if (c.bm == 128) MH_L8(128, 64, 16);
+    else if (c.bk == 32) MH_L8(64, 64, 32);
+    else MH_L8(64, 64, 16);
+  }
   else if (c.bm == 64 && c.bn == 64 && c.bk == 16) MH_L(64, 64, 16);
   else if (c.bm == 64 && c.bn == 64) MH_L(64, 64, 32);
   else if (c.bm == 128 && c.bn == 64) MH_L(128, 64, 16);
   else if (c.bm == 64 && c.bn == 128) MH_L(64, 128, 16);
   else if constexpr (!CPLX) MH_L(128, 128, 16);  // complex 128x128 is never admitted
 #undef MH_L
+#undef MH_L8
 }
 
 template <bool CPLX, bool VEC>
@@ -528,9 +536,9 @@ static int tune(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx, b
     MH_HIP(h, hipEventElapsedTime(&ms, e0, e1));
     ms /= 3.f;
     if (h->tune_log)
-      fprintf(stderr, "[mh tune] %s%s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d: %.1f us (%.2f TFLOP/s)\n",
+      fprintf(stderr, "[mh tune] %s%s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d waves=%d: %.1f us (%.2f TFLOP/s)\n",
               fam.name, cplx ? "/c" : "", p.M, p.N, p.K, (int)amc, (int)bkc, c.bm, c.bn, c.bk,
-              c.split, c.group, ms * 1e3,
+              c.split, c.group, c.waves ? c.waves : 4, ms * 1e3,
               2.0 * p.M * (double)p.N * p.K * (cplx ? 4 : 1) / (ms * 1e-3) / 1e12);
     if (ms < best) {
       best = ms;

@@ -37,6 +37,7 @@ struct GemmP {
 
 struct Cand {
   int bm, bn, bk, split, group;
+  int waves;  // waves per block; 0 means the family default (4)
 };
 
 struct GemmFamily {

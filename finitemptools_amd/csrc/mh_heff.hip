@@ -91,11 +91,23 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int jj = 0; jj < JC; ++jj) acc[jj] = Ar<T>::zero();
   const T *src = in + p + q * sIq;
-  for (int i = 0; i < ni; ++i) {
-    if (!nz[i]) continue;
-    const T v = src[(int64_t)i * P];
+  // Rows in groups of 8: the (independent) loads of a group are issued
+  // before its FMAs, so each thread has up to 8 loads in flight.  Rows whose
+  // Mt row is zero are neither loaded nor used (a uniform branch).
+  for (int i0 = 0; i0 < ni; i0 += 8) {
+    T v[8];
 #pragma unroll
-    for (int jj = 0; jj < JC; ++jj) acc[jj] = Ar<T>::fma(v, sM[i * JC + jj], acc[jj]);
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u;
+      v[u] = (i < ni && nz[i]) ? src[(int64_t)i * P] : Ar<T>::zero();
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u;
+      if (i >= ni || !nz[i]) continue;
+#pragma unroll
+      for (int jj = 0; jj < JC; ++jj) acc[jj] = Ar<T>::fma(v[u], sM[i * JC + jj], acc[jj]);
+    }
   }
   T *dst = out + p + q * sOq;
 #pragma unroll
