@@ -4,7 +4,8 @@ ground_state! (:240).
 
 Per bond: two-site tensor (MFMA GEMM), lowest eigenpair of ac2_prime by the
 device-resident Lanczos (kmax = eig_maxiter, btol = eig_tol), FP64 truncated
-SVD, normalised singular values, split, overlap error, environment update.
+split (reduced density matrix by default, or the SVD), normalised singular
+values, overlap error, environment update.
 ``prec`` selects the arithmetic of the H_eff and environment contractions;
 the mixed forms add FP64 re-orthogonalisation to the Lanczos recurrence.
 """
@@ -17,7 +18,7 @@ from .derivatives import HeffOperator
 from .envs import environments
 from .krylov import simple_lanczos_solver
 from .linalg import as_matrix, cm_view, gemm, two_site
-from .truncation import MPSTruncation, stable_tsvd
+from .truncation import MPSTruncation, split_rho, stable_tsvd
 
 MAXITER = 100   # FiniteMPTools.jl:54 (Defaults.maxiter)
 TOL = 1e-12     # Defaults.tol
@@ -26,13 +27,16 @@ EIG_TOL = 1e-12  # Defaults.dmrg_eig_tol
 
 class DMRG2:
     def __init__(self, maxiter=MAXITER, tol=TOL, eig_maxiter=10, eig_tol=EIG_TOL,
-                 verbosity=0, trunc=None, prec="f64", reorth=None):
+                 verbosity=0, trunc=None, prec="f64", reorth=None, split="rho"):
         self.maxiter, self.tol = maxiter, tol
         self.eig_maxiter, self.eig_tol = eig_maxiter, eig_tol
         self.verbosity = verbosity
         self.trunc = trunc or MPSTruncation()
         self.prec = prec
         self.reorth = (prec != "f64") if reorth is None else reorth
+        # "rho": reduced-density-matrix split (batched FP64 eigh on the
+        # device); "svd": stable_tsvd (the reference's tsvd)
+        self.split = split
         self.stats = {"matvec_flops": 0.0, "lanczos_s": 0.0, "svd_s": 0.0, "env_s": 0.0, "bonds": 0}
 
 
@@ -46,7 +50,10 @@ def _scale_cols(u, s):
     return u * s.to(u.dtype).reshape(1, 1, -1)
 
 
-def _bond_step(m, alg, site):
+def _bond_step(m, alg, site, absorb):
+    """Two-site tensor, lowest eigenpair, truncated split with the normalised
+    singular values absorbed into A2 (absorb="right": u | s v) or A1
+    ("left": u s | v).  Returns (two, ev, A1, A2, s)."""
     mpo, mps, hs = m.mpo, m.mps, m.hstorage
     two = two_site(mps[site], mps[site + 1])
     op = HeffOperator(2, hs[site], hs[site + 2], mpo[site], mpo[site + 1], prec=alg.prec)
@@ -56,16 +63,20 @@ def _bond_step(m, alg, site):
     t1 = time.perf_counter()
     alg.stats["lanczos_s"] += t1 - t0
     alg.stats["matvec_flops"] += op.flops() * alg.eig_maxiter
-    u, s, v, err = stable_tsvd(vecs, alg.trunc)
-    s = s / torch.linalg.vector_norm(s)
+    if getattr(alg, "split", "svd") == "rho":
+        A1, A2, s = split_rho(vecs, alg.trunc, absorb)
+    else:
+        u, s, v, err = stable_tsvd(vecs, alg.trunc)
+        s = s / torch.linalg.vector_norm(s)
+        A1, A2 = (u, _scale_rows(v, s)) if absorb == "right" else (_scale_cols(u, s), v)
     alg.stats["svd_s"] += time.perf_counter() - t1
     alg.stats["bonds"] += 1
-    return two, ev, u, s, v
+    return two, ev, A1, A2, s
 
 
-def _overlap_error(two, u, s, v):
+def _overlap_error(two, A1, A2):
     """err_1 = <u s v, twositemps> (dmrg.jl:109-110); returns |1 - |err_1||."""
-    usv = gemm(as_matrix(_scale_cols(u, s), 2), as_matrix(v, 1))
+    usv = gemm(as_matrix(A1, 2), as_matrix(A2, 1))
     e = torch.vdot(usv.t().reshape(-1), as_matrix(two, 2).t().reshape(-1)).item()
     return abs(1.0 - abs(e))
 
@@ -75,11 +86,11 @@ def leftsweep(m, alg):
     mps = m.mps
     energies, delta = [], 0.0
     for site in range(len(mps) - 2):
-        two, ev, u, s, v = _bond_step(m, alg, site)
+        two, ev, A1, A2, s = _bond_step(m, alg, site, "right")
         energies.append(ev)
-        mps[site] = u
-        mps[site + 1] = _scale_rows(v, s)
-        delta = max(delta, _overlap_error(two, u, s, v))
+        mps[site] = A1
+        mps[site + 1] = A2
+        delta = max(delta, _overlap_error(two, A1, A2))
         t0 = time.perf_counter()
         m.updateleft(site)
         alg.stats["env_s"] += time.perf_counter() - t0
@@ -93,12 +104,12 @@ def rightsweep(m, alg):
     mps = m.mps
     energies, delta = [], 0.0
     for site in range(len(mps) - 2, -1, -1):
-        two, ev, u, s, v = _bond_step(m, alg, site)
+        two, ev, A1, A2, s = _bond_step(m, alg, site, "left")
         energies.append(ev)
-        mps[site] = _scale_cols(u, s)
-        mps[site + 1] = v
+        mps[site] = A1
+        mps[site + 1] = A2
         mps.s[site + 1] = s
-        delta = max(delta, _overlap_error(two, u, s, v))
+        delta = max(delta, _overlap_error(two, A1, A2))
         t0 = time.perf_counter()
         m.updateright(site + 1)
         alg.stats["env_s"] += time.perf_counter() - t0

@@ -12,7 +12,7 @@ import torch
 from .derivatives import HeffOperator
 from .krylov import exponentiate
 from .linalg import as_matrix, cm_view, from_matrix, gemm, two_site
-from .truncation import default_truncation, stable_tsvd
+from .truncation import default_truncation, split_rho, stable_tsvd
 from . import transfer
 
 TDVP_EXP_TOL = 1.0e-8  # FiniteMPTools.jl:59 (Defaults.tdvp_exp_tol)
@@ -61,10 +61,23 @@ def _scale_rows(v, s):
 class TDVP2:
     """tdvp.jl:71-79 (exptol 1e-8, default truncation)."""
 
-    def __init__(self, stepsize, exptol=TDVP_EXP_TOL, trunc=None, prec="f64", kdim=12, verbosity=0):
+    def __init__(self, stepsize, exptol=TDVP_EXP_TOL, trunc=None, prec="f64", kdim=12, verbosity=0,
+                 split="rho"):
         self.stepsize, self.exptol = complex(stepsize), exptol
         self.trunc = trunc or default_truncation()
         self.prec, self.kdim, self.verbosity = prec, kdim, verbosity
+        self.split = split  # "rho": reduced-density-matrix split; "svd": stable_tsvd
+
+
+def _split2(two, alg, absorb):
+    """Truncated split of the evolved two-site tensor with the (unnormalised)
+    singular values absorbed into A2 ("right") or A1 ("left")."""
+    if getattr(alg, "split", "svd") == "rho":
+        return split_rho(two, alg.trunc, absorb, normalize=False)
+    u, s, v, _ = stable_tsvd(two, alg.trunc)
+    if absorb == "right":
+        return u, _scale_rows(v, s), s
+    return _scale_cols(u, s), v, s
 
 
 class TDVP1:
@@ -89,9 +102,7 @@ def _leftsweep2(m, alg):
         dt = _dt(alg, two)
         op2 = HeffOperator(2, hs[site], hs[site + 2], mpo[site], mpo[site + 1], prec=alg.prec)
         two = _exp(op2, dt / 2, two, alg)
-        u, s, v, _ = stable_tsvd(two, alg.trunc)
-        mps[site] = u
-        sitemps = _scale_rows(v, s)
+        mps[site], sitemps, s = _split2(two, alg, "right")
         hs[site + 1] = transfer.updateleft(hs[site], mps[site], mpo[site], mps[site], prec=alg.prec)
         op1 = HeffOperator(1, hs[site + 1], hs[site + 2], mpo[site + 1], prec=alg.prec)
         mps[site + 1] = _exp(op1, -dt / 2, sitemps, alg)
@@ -100,9 +111,7 @@ def _leftsweep2(m, alg):
     dt = _dt(alg, two)
     op2 = HeffOperator(2, hs[site], hs[site + 2], mpo[site], mpo[site + 1], prec=alg.prec)
     two = _exp(op2, dt, two, alg)
-    u, s, v, _ = stable_tsvd(two, alg.trunc)
-    mps[site] = _scale_cols(u, s)
-    mps[site + 1] = v
+    mps[site], mps[site + 1], s = _split2(two, alg, "left")
     hs[site + 1] = transfer.updateright(hs[site + 2], mps[site + 1], mpo[site + 1], mps[site + 1], prec=alg.prec)
     mps.s[site + 1] = s
 
@@ -118,9 +127,7 @@ def _rightsweep2(m, alg):
         two = two_site(mps[site], mps[site + 1])
         op2 = HeffOperator(2, hs[site], hs[site + 2], mpo[site], mpo[site + 1], prec=alg.prec)
         two = _exp(op2, dt / 2, two, alg)
-        u, s, v, _ = stable_tsvd(two, alg.trunc)
-        mps[site] = _scale_cols(u, s)
-        mps[site + 1] = v
+        mps[site], mps[site + 1], s = _split2(two, alg, "left")
         hs[site + 1] = transfer.updateright(hs[site + 2], mps[site + 1], mpo[site + 1], mps[site + 1], prec=alg.prec)
         mps.s[site + 1] = s
 

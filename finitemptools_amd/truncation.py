@@ -36,6 +36,38 @@ def default_truncation():
     return MPSTruncation(DEFAULT_NONSYMMETRIC_D, DEFAULT_SVD_CUTOFF)
 
 
+def split_rho(x, trunc, absorb, normalize=True):
+    """Truncated split of a two-site tensor x[a,s1,s2,b] through the reduced
+    density matrix (White's formulation of the DMRG step), FP64 on the
+    device: the left sweep diagonalises rho = M M^dag (M = x as (a,s1)|(s2,b))
+    for u and s^2 and forms s v = u^dag M exactly; the right sweep
+    diagonalises M^dag M for v and forms u s = M v^dag.  The kept values and
+    the truncated tensor u s v are those of the SVD (truncation.jl:27-47 on
+    s = sqrt(lambda)), with s normalised as dmrg.jl:105 does (normalize=False
+    keeps the raw values, as the TDVP sweeps need).  Returns (A1, A2, s)
+    with s absorbed into A2 (absorb="right") or A1 ("left")."""
+    Dl, d1, d2, Dr = x.shape
+    M = as_matrix(x, 2)
+    left = absorb == "right"
+    G = M @ M.conj().t() if left else M.conj().t() @ M
+    lam, V = torch.linalg.eigh(G)
+    r = min(M.shape)
+    lam_h = np.maximum(lam.detach().cpu().numpy()[::-1][:r], 0.0)
+    s_h = np.sqrt(lam_h)
+    k, _ = trunc.truncate(s_h)
+    k = max(1, min(k, int(np.count_nonzero(s_h > 0)) or 1))
+    norm = float(np.linalg.norm(s_h[:k])) if normalize else 1.0
+    Q = V[:, G.shape[0] - k:].flip(-1)
+    s = torch.as_tensor(s_h[:k] / norm, device=x.device)  # normalize=False: the raw values
+    if left:
+        A1 = from_matrix(Q, (Dl, d1, k))
+        A2 = from_matrix((Q.conj().t() @ M) / norm, (k, d2, Dr))
+    else:
+        A2 = from_matrix(Q.conj().t(), (k, d2, Dr))
+        A1 = from_matrix((M @ Q) / norm, (Dl, d1, k))
+    return A1, A2, s
+
+
 def stable_tsvd(x, trunc):
     """x[a, s1, s2, b] -> u[a, s1, k], s[k] (host + device), v[k, s2, b], err."""
     Dl, d1, d2, Dr = x.shape

@@ -1,6 +1,7 @@
 """Host-side logic of the engine package (no GPU): seeded initial states,
 truncation scheme and MPO shapes agree with the oracle."""
 import numpy as np
+import pytest
 
 from finitemptools_amd import models
 from finitemptools_amd.mps import random_site_tensors, virtual_dims
@@ -43,3 +44,34 @@ def test_conjugate_views_are_materialised_before_the_engine():
         assert torch.equal(torch.view_as_complex(raw), t)
     t = from_matrix(r.conj().t(), (2, 2))
     assert torch.equal(t, r.conj().t())
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("absorb", ["left", "right"])
+def test_density_matrix_split_equals_truncated_svd(cplx, absorb):
+    """split_rho keeps the singular values of stable_tsvd (normalised) and the
+    same truncated two-site tensor u s v; the non-absorbing factor is an
+    isometry."""
+    import torch
+    from finitemptools_amd.linalg import as_matrix
+    from finitemptools_amd.truncation import MPSTruncation, split_rho, stable_tsvd
+    rng = np.random.default_rng(7)
+    Dl, d, Dr = 6, 2, 5
+    x = rng.standard_normal((Dl, d, d, Dr))
+    if cplx:
+        x = x + 1j * rng.standard_normal((Dl, d, d, Dr))
+    # a decaying spectrum, as a DMRG two-site tensor has
+    M = x.reshape(Dl * d, d * Dr, order="F")
+    U, S, Vh = np.linalg.svd(M, full_matrices=False)
+    M = (U * (0.5 ** np.arange(len(S)))) @ Vh
+    xt = torch.as_tensor(M.reshape(Dl, d, d, Dr, order="F"))
+    tr = MPSTruncation(D=7, eps=1e-10)
+    A1, A2, s = split_rho(xt, tr, absorb)
+    u, s_ref, v, _ = stable_tsvd(xt, tr)
+    s_ref = s_ref / torch.linalg.vector_norm(s_ref)
+    assert np.allclose(s.numpy(), s_ref.numpy(), atol=1e-12)
+    rec = (as_matrix(A1, 2) @ as_matrix(A2, 1)).numpy()
+    ref = (as_matrix(u, 2) @ torch.diag(s_ref.to(u.dtype)) @ as_matrix(v, 1)).numpy()
+    assert np.allclose(rec, ref, atol=1e-12)
+    iso = as_matrix(A1, 2) if absorb == "right" else as_matrix(A2, 1).conj().t()
+    assert np.allclose((iso.conj().t() @ iso).numpy(), np.eye(iso.shape[1]), atol=1e-12)
