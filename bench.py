@@ -202,6 +202,85 @@ def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
             "sweep_D2048": sweeps}
 
 
+def xxz_dmrg_L20_D64(cpu=True):
+    """The reference's CPU run configuration: XXZ (J = Delta = 1), L = 20,
+    two-site DMRG to D = 64, 4 sweeps from a seeded random right-canonical
+    MPS; energy and per-sweep times on the device, and the oracle's numpy
+    sweep of the same configuration on the host cores."""
+    import json as _json
+    import torch
+    from finitemptools_amd import dmrg, models
+    from finitemptools_amd.envs import environments, mpo_to_device
+    from finitemptools_amd.mps import randommps
+    from finitemptools_amd.truncation import MPSTruncation
+    L, D = 20, 64
+    ed = _json.load(open(os.path.join(ROOT, "tests", "golden", "ed_energies.json")))["xxz_L20_J1_D1"]
+    psi = randommps(L, 2, D, seed=0).right_canonicalize()
+    env = environments(mpo_to_device(models.heisenberg_xxz_mpo(L)), psi)
+    alg = dmrg.DMRG2(trunc=MPSTruncation(D=D, eps=1e-8))
+    sweeps = []
+    for _ in range(4):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        e, delta = dmrg.sweep(env, alg)
+        torch.cuda.synchronize()
+        sweeps.append(round(time.perf_counter() - ts, 4))
+    out = {"workload": "XXZ L=20 J=Delta=1, DMRG2 D=64, 4 sweeps, seed 0",
+           "E": e[-1], "E_minus_ED": e[-1] - ed, "sweep_s": sweeps,
+           "lanczos_s": round(alg.stats["lanczos_s"], 4), "split_s": round(alg.stats["svd_s"], 4),
+           "env_s": round(alg.stats["env_s"], 4)}
+    if cpu:
+        from oracle import dmrg as OD
+        psi_h = OD.right_canonicalize(OD.randommps(L, 2, D, seed=0))
+        envh = OD.ExpectationCache(models.heisenberg_xxz_mpo(L), psi_h)
+        algh = OD.DMRG2(trunc=OD.MPSTruncation(D=D, eps=1e-8))
+        ts = time.perf_counter()
+        eh, _ = OD.compute(envh, algh, nsweeps=1)
+        out["cpu_oracle_first_sweep_s"] = round(time.perf_counter() - ts, 3)
+        out["cpu_oracle_first_sweep_E"] = eh[-1]
+    return out
+
+
+def gradients_converged_j1j2(L=40, D=256, J2=0.25, nsweeps=5, h=1e-5):
+    """dE/dJ2 at a converged DMRG2 ground state of the J1-J2 chain: adjoint
+    kernels vs Hellmann-Feynman vs the central finite difference of ground
+    energies (two more DMRG runs at J2 +- h)."""
+    import torch
+    from finitemptools_amd import dmrg, models
+    from finitemptools_amd import gradients as GR
+    from finitemptools_amd.envs import environments, mpo_to_device
+    from finitemptools_amd.mps import randommps
+    from finitemptools_amd.truncation import MPSTruncation
+
+    def ground(j2):
+        psi = randommps(L, 2, 32, seed=1).right_canonicalize()
+        mpo = mpo_to_device(models.j1j2_mpo(L, 1.0, j2, keep_j2=True))
+        alg = dmrg.DMRG2(trunc=MPSTruncation(D=D, eps=1e-12))
+        e, _ = dmrg.compute(environments(mpo, psi), alg, nsweeps)
+        return e[-1], psi, mpo
+
+    ts = time.perf_counter()
+    E0, psi, mpo = ground(J2)
+    dmrg_s = time.perf_counter() - ts
+    sites = [psi[i] for i in range(L)]
+    W1 = mpo_to_device(models.j1j2_mpo(L, 1.0, 1.0, keep_j2=True))
+    W0 = mpo_to_device(models.j1j2_mpo(L, 1.0, 0.0, keep_j2=True))
+    dW = [a - b for a, b in zip(W1, W0)]
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    res = GR.energy_gradients(mpo, sites, want_dA=True)
+    torch.cuda.synchronize()
+    grad_s = time.perf_counter() - ts
+    g = GR.dE_dtheta(dW, res)
+    hf = GR.hellmann_feynman(mpo_to_device(models.j1j2_dJ2_mpo(L)), sites)
+    fd = (ground(J2 + h)[0] - ground(J2 - h)[0]) / (2 * h)
+    return {"workload": f"J1-J2 L={L} J2={J2}, DMRG2 D={D}, {nsweeps} sweeps",
+            "E0": E0, "dmrg_s": round(dmrg_s, 2), "gradient_s": round(grad_s, 4),
+            "dE_dJ2_adjoint": g, "hellmann_feynman": hf, "central_fd_h1e-5": fd,
+            "adjoint_minus_hf": g - hf, "adjoint_minus_fd": g - fd,
+            "dE_dA_norm": float(sum(torch.sum(a * a).item() for a in res.dA) ** 0.5)}
+
+
 def gradients_j1j2(L=40, D=256, J2=0.25):
     """dE/dJ2 and dE/dA of E = <psi|H|psi>/<psi|psi> for the J1-J2 chain on a
     random D = 256 MPS (the cost does not depend on convergence), with stored
@@ -330,6 +409,9 @@ def main():
     if not args.no_gradients:
         print("gradients ...", file=sys.stderr, flush=True)
         out["gradients_j1j2_L40_D256"] = gradients_j1j2()
+        out["gradients_j1j2_L40_D256_converged"] = gradients_converged_j1j2()
+    print("xxz L=20 D=64 ...", file=sys.stderr, flush=True)
+    out["dmrg_xxz_L20_D64"] = xxz_dmrg_L20_D64(cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(2, D2)
     print(json.dumps(out))
