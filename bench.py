@@ -154,14 +154,21 @@ def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
     site = L // 2 - 1
     x = bm.two_site(psi[site], psi[site + 1])
     Le, Re = env.hstorage[site], env.hstorage[site + 2]
-    tab = bs.plan_heff2(x.layout, Le.layout, Re.layout, mpo[site], mpo[site + 1])
+    args = (x.layout, Le.layout, Re.layout, mpo[site], mpo[site + 1])
+    ts = time.perf_counter()
+    tab = bs.plan_heff2(*args)
+    py_plan_ms = 1e3 * (time.perf_counter() - ts)
+    ts = time.perf_counter()
+    bs.host_native_tables("heff2", *args, "f64")
+    native_plan_ms = 1e3 * (time.perf_counter() - ts)
     mid = psi.bond_space(site + 1)
     mv = {"sectors": len(mid.sectors), "max_sector": max(d for _, d in mid.sectors),
           "gflop_per_matvec": round(tab.flops() / 1e9, 3),
+          "plan_ms": {"python": round(py_plan_ms, 2), "native": round(native_plan_ms, 2)},
           "dense_equivalent_gflop": round((4.0 * 2048 ** 3 * 6 * 16 + 4.0 * 2048 ** 2 * 36 * 64) / 1e9, 1)}
     yref = None
     for prec in ("f64", "f32", "bf16x3"):
-        plan = bs.Plan(tab, prec)
+        plan = bs.make_plan("heff2", *args, prec)
         y = torch.empty_like(x.data)
         plan.apply(x.data, Le.data, Re.data, y)
         torch.cuda.synchronize()

@@ -14,6 +14,7 @@ SOURCES = [
     "mh_krylov.hip",
     "mh_blocksparse.hip",
     "mh_adjoint.hip",
+    "mh_bs_planner.cpp",
 ]
 ARCH = "gfx950"
 
@@ -51,7 +52,7 @@ def build(verbose=False, jobs=None):
     procs = []
     objs = []
     for src in SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmd = [hipcc] + [f for f in HIPCC_FLAGS if f != "-shared"] + EXTRA_FLAGS.get(src, []) + [
             "-c", os.path.join(CSRC, src), "-o", obj]

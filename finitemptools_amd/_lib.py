@@ -11,6 +11,7 @@ c_int = ctypes.c_int
 c_double = ctypes.c_double
 c_size_t = ctypes.c_size_t
 c_int64 = ctypes.c_int64
+c_int32 = ctypes.c_int32
 c_void_p = ctypes.c_void_p
 P = ctypes.POINTER
 
@@ -34,6 +35,11 @@ class HeffOp(ctypes.Structure):
         ("wl", c_int), ("wm", c_int), ("wr", c_int),
         ("W1", c_void_p), ("W2", c_void_p), ("L", c_void_p), ("R", c_void_p),
     ]
+
+
+class BsSectors(ctypes.Structure):
+    """mirror of ``mh_bs_sectors``"""
+    _fields_ = [("n", c_int), ("nq", c_int), ("charges", P(c_int32)), ("dims", P(c_int32))]
 
 
 # name -> (restype, argtypes)
@@ -84,6 +90,23 @@ SIGNATURES = {
     "mh_bs_plan_variants": (c_int, [c_void_p, P(c_int)]),
     "mh_bs_plan_force_variant": (c_int, [c_void_p, c_int]),
     "mh_bs_plan_destroy": (c_int, [c_void_p, c_void_p]),
+    "mh_bs_plan_sizes": (c_int, [c_void_p, P(c_int64)]),
+    "mh_bs_plan_stage_count": (c_int, [c_void_p, P(c_int)]),
+    "mh_bs_plan_stage_info": (c_int, [c_void_p, c_int, P(c_int32)]),
+    "mh_bs_plan_stage_gemm": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mh_bs_plan_stage_mode": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p]),
+    "mh_bs_plan_heff2": (c_int, [c_void_p, c_int, P(BsSectors), P(BsSectors), c_int, c_void_p,
+                                 c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                 P(c_void_p)]),
+    "mh_bs_plan_updateleft": (c_int, [c_void_p, c_int, P(BsSectors), P(BsSectors), c_int,
+                                      c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                      P(c_void_p)]),
+    "mh_bs_plan_updateright": (c_int, [c_void_p, c_int, P(BsSectors), P(BsSectors), c_int,
+                                       c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                       P(c_void_p)]),
+    "mh_bs_plan_two_site": (c_int, [c_void_p, P(BsSectors), P(BsSectors), P(BsSectors), c_int,
+                                    c_void_p, P(c_void_p)]),
     "mh_bs_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mh_bs_lanczos_eig": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                   c_double, c_int, P(c_double), c_void_p, P(c_int), P(c_int)]),

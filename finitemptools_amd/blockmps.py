@@ -300,9 +300,9 @@ class BlockExpectationCache:
         A = self.mps[site]
         old = self.hstorage[site]
         new_lay = EnvLayout(A.rspace, self.mpo.qw[site + 1])
-        tab = bs.plan_updateleft(A.layout, old.layout, self.mpo[site], new_lay)
+        plan = bs.make_plan("updateleft", A.layout, old.layout, self.mpo[site], new_lay, self.prec)
         out = torch.zeros(new_lay.size, dtype=torch.float64, device=A.data.device)
-        bs.Plan(tab, self.prec).apply(A.data, old.data, A.data, out)
+        plan.apply(A.data, old.data, A.data, out)
         self.hstorage[site + 1] = BlockEnv(new_lay, out)
         self.env_time += time.perf_counter() - t0
 
@@ -312,9 +312,9 @@ class BlockExpectationCache:
         A = self.mps[site]
         old = self.hstorage[site + 1]
         new_lay = EnvLayout(A.lspace, self.mpo.qw[site])
-        tab = bs.plan_updateright(A.layout, old.layout, self.mpo[site], new_lay)
+        plan = bs.make_plan("updateright", A.layout, old.layout, self.mpo[site], new_lay, self.prec)
         out = torch.zeros(new_lay.size, dtype=torch.float64, device=A.data.device)
-        bs.Plan(tab, self.prec).apply(A.data, old.data, A.data, out)
+        plan.apply(A.data, old.data, A.data, out)
         self.hstorage[site] = BlockEnv(new_lay, out)
         self.env_time += time.perf_counter() - t0
 
@@ -438,9 +438,9 @@ def two_site_layout(A1, A2):
 def two_site(A1, A2, xl=None):
     """x = A1 * A2 over the middle bond (dmrg.jl:98) by one grouped launch."""
     xl = xl or two_site_layout(A1, A2)
-    tab = bs.plan_two_site(A1.layout, A2.layout, xl)
+    plan = bs.make_plan("two_site", A1.layout, A2.layout, xl)
     out = torch.zeros(xl.size, dtype=torch.float64, device=A1.data.device)
-    bs.Plan(tab, "f64").apply(A1.data, A2.data, None, out)
+    plan.apply(A1.data, A2.data, None, out)
     return BlockTensor(xl, out)
 
 
@@ -617,8 +617,8 @@ def _bond_step(m, alg, site, absorb):
     t0 = time.perf_counter()
     x = two_site(A1, A2)
     L, R = hs[site], hs[site + 2]
-    tab = bs.plan_heff2(x.layout, L.layout, R.layout, m.mpo[site], m.mpo[site + 1])
-    plan = bs.Plan(tab, alg.prec)
+    plan = bs.make_plan("heff2", x.layout, L.layout, R.layout, m.mpo[site], m.mpo[site + 1],
+                        alg.prec)
     t1 = time.perf_counter()
     ev, vec, info, nit = plan.lanczos(L.data, R.data, x.data, alg.eig_maxiter, alg.eig_tol,
                                       reorth=alg.reorth)
@@ -640,7 +640,7 @@ def _bond_step(m, alg, site, absorb):
     st["plan_s"] = st.get("plan_s", 0.0) + (t1 - t0)
     st["lanczos_s"] += t2 - t1
     st["svd_s"] += t3 - t2
-    st["matvec_flops"] += tab.flops() * nit
+    st["matvec_flops"] += plan.flops() * nit
     st["bonds"] += 1
     return ev, n1, n2, svals, abs(1.0 - abs(e))
 

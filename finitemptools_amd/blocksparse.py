@@ -348,15 +348,22 @@ def _p(a):
 
 
 class Plan:
-    """A plan uploaded to one engine handle (freed with the object)."""
+    """A plan on one engine handle (freed with the object); its tables go to
+    the device on first use."""
 
-    def __init__(self, tables, prec="f64", device=None):
+    def __init__(self, tables, prec="f64", device=None, _native=None):
         from .engine import engine
         self.tables = tables
         self.eng = engine(device)
         self.lib = self.eng.lib
         self.h = self.eng.bind_stream()
         self.prec = prec
+        if _native is not None:
+            # built by a native planner: _native(h, &ptr) -> status
+            ptr = _lib.c_void_p()
+            self.eng.check(_native(self.h, _lib.ctypes.byref(ptr)), "native planner")
+            self.ptr = ptr
+            return
         sizes = (_lib.c_int64 * 6)(*tables.sizes)
         ptr = _lib.c_void_p()
         self.eng.check(self.lib.mh_bs_plan_create(self.h, _lib.PREC[prec], sizes,
@@ -414,6 +421,153 @@ class Plan:
                 self.ptr = None
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ native planner
+
+class _Sectors:
+    """An ``mh_bs_sectors`` for a Space (sectors in the Space's order); keeps
+    its arrays alive."""
+
+    def __init__(self, space):
+        q = [c for c, _ in space.sectors]
+        nq = len(q[0]) if q else 1
+        self.charges = np.ascontiguousarray(np.array(q, dtype=np.int32).reshape(-1, nq))
+        self.dims = np.ascontiguousarray(np.array([d for _, d in space.sectors], dtype=np.int32))
+        self.s = _lib.BsSectors(len(q), nq,
+                                self.charges.ctypes.data_as(_lib.P(_lib.c_int32)),
+                                self.dims.ctypes.data_as(_lib.P(_lib.c_int32)))
+
+    def ref(self):
+        return _lib.ctypes.byref(self.s)
+
+
+def _charges(qs, nq):
+    return np.ascontiguousarray(np.array(qs, dtype=np.int32).reshape(-1, nq))
+
+
+def _wcm(W):
+    return np.asfortranarray(np.asarray(W, dtype=np.float64))
+
+
+def _hp(a):
+    return a.ctypes.data_as(_lib.c_void_p)
+
+
+def native_builder(kind, *args):
+    """(call(h, out_ptr) -> status, keep-alive list) of a native planner:
+    kind "heff2": (xl, Lay, Ray, W1, W2, prec); "updateleft": (al, Lay, W,
+    Lay_new, prec); "updateright": (al, Ray, W, Ray_new, prec); "two_site":
+    (l1, l2, xl).  The layouts are those of the Python planner above."""
+    lib = _lib.load()
+    if kind == "heff2":
+        xl, Lay, Ray, W1, W2, prec = args
+        if xl.nphys != 2:
+            raise _lib.DimensionMismatch(2, "plan_heff2 needs a two-site layout")
+        nq = len(xl.phys[0])
+        L, R = _Sectors(xl.lspace), _Sectors(xl.rspace)
+        ph, ql, qr = _charges(xl.phys, nq), _charges(Lay.qw, nq), _charges(Ray.qw, nq)
+        w1, w2 = _wcm(W1), _wcm(W2)
+        if w1.shape[0] != Lay.w or w2.shape[2] != Ray.w or w1.shape[1] != xl.d:
+            raise _lib.DimensionMismatch(2, "MPO dimensions do not match the environments")
+        keep = [L, R, ph, ql, qr, w1, w2]
+        return (lambda h, out: lib.mh_bs_plan_heff2(
+            h, _lib.PREC[prec], L.ref(), R.ref(), xl.d, _hp(ph), Lay.w, _hp(ql), w1.shape[2],
+            Ray.w, _hp(qr), _hp(w1), _hp(w2), out)), keep
+    if kind in ("updateleft", "updateright"):
+        al, E, W, E_new, prec = args
+        if al.nphys != 1:
+            raise _lib.DimensionMismatch(2, f"{kind} needs a one-site layout")
+        nq = len(al.phys[0])
+        L, R = _Sectors(al.lspace), _Sectors(al.rspace)
+        ql, qr = (E.qw, E_new.qw) if kind == "updateleft" else (E_new.qw, E.qw)
+        ph, cl, cr, w = _charges(al.phys, nq), _charges(ql, nq), _charges(qr, nq), _wcm(W)
+        if w.shape[0] != len(ql) or w.shape[2] != len(qr) or w.shape[1] != al.d:
+            raise _lib.DimensionMismatch(2, f"{kind}: MPO does not match the channels")
+        fn = getattr(lib, "mh_bs_plan_" + kind)
+        keep = [L, R, ph, cl, cr, w]
+        return (lambda h, out: fn(h, _lib.PREC[prec], L.ref(), R.ref(), al.d, _hp(ph),
+                                  len(ql), _hp(cl), len(qr), _hp(cr), _hp(w), out)), keep
+    if kind == "two_site":
+        l1, l2, xl = args
+        nq = len(xl.phys[0])
+        L, M, R = _Sectors(l1.lspace), _Sectors(l1.rspace), _Sectors(l2.rspace)
+        ph = _charges(xl.phys, nq)
+        keep = [L, M, R, ph]
+        return (lambda h, out: lib.mh_bs_plan_two_site(h, L.ref(), M.ref(), R.ref(), xl.d,
+                                                       _hp(ph), out)), keep
+    raise ValueError(kind)
+
+
+def native_plan(kind, *args, device=None):
+    """A Plan built by the native planner (same tables as the Python one)."""
+    call, keep = native_builder(kind, *args)
+    prec = args[-1] if kind != "two_site" else "f64"
+    return Plan(None, prec, device, _native=call)
+
+
+_PY_PLANNERS = {"heff2": lambda xl, La, Ra, W1, W2, prec: plan_heff2(xl, La, Ra, W1, W2),
+                "updateleft": lambda al, E, W, En, prec: plan_updateleft(al, E, W, En),
+                "updateright": lambda al, E, W, En, prec: plan_updateright(al, E, W, En),
+                "two_site": lambda l1, l2, xl: plan_two_site(l1, l2, xl)}
+
+# "native": plans from the C planner (mh_bs_plan_heff2 ...); "python": the
+# table builders above, uploaded stage by stage.  Both give the same tables.
+PLANNER = "native"
+
+
+def make_plan(kind, *args, planner=None):
+    """Plan of one contraction (kind and arguments as native_builder)."""
+    if (planner or PLANNER) == "native":
+        return native_plan(kind, *args)
+    prec = args[-1] if kind != "two_site" else "f64"
+    return Plan(_PY_PLANNERS[kind](*args), prec)
+
+
+def read_tables(ptr, lib=None):
+    """PlanTables of a plan pointer (host tables, no device needed)."""
+    lib = lib or _lib.load()
+    sizes = (_lib.c_int64 * 6)()
+    _lib.raise_for(lib.mh_bs_plan_sizes(ptr, sizes), None, "sizes")
+    tab = PlanTables(list(sizes))
+    n = _lib.c_int()
+    _lib.raise_for(lib.mh_bs_plan_stage_count(ptr, _lib.ctypes.byref(n)), None, "stages")
+    for i in range(n.value):
+        info = (_lib.c_int32 * 8)()
+        _lib.raise_for(lib.mh_bs_plan_stage_info(ptr, i, info), None, "stage info")
+        kind, a, b, c, amc, bkc, n1, n2 = list(info)
+        if kind == 0:
+            ti = np.zeros((n1, 5), np.int32)
+            tc = np.zeros(n1, np.int64)
+            so = np.zeros((n2, 2), np.int64)
+            si = np.zeros((n2, 3), np.int32)
+            _lib.raise_for(lib.mh_bs_plan_stage_gemm(ptr, i, _hp(ti), _hp(tc), _hp(so), _hp(si)),
+                           None, "stage tables")
+            tab.stages.append(("gemm", dict(a_src=a, b_src=b, c_src=c, amc=amc, bkc=bkc,
+                                            task_i=ti, task_coff=tc, seg_off=so, seg_i=si)))
+        else:
+            do = np.zeros(n1, np.int64)
+            di = np.zeros((n1, 5), np.int32)
+            so = np.zeros(n2, np.int64)
+            sl = np.zeros(n2, np.int32)
+            cf = np.zeros(n2, np.float64)
+            _lib.raise_for(lib.mh_bs_plan_stage_mode(ptr, i, _hp(do), _hp(di), _hp(so), _hp(sl),
+                                                     _hp(cf)), None, "stage tables")
+            tab.stages.append(("mode", dict(in_src=a, out_src=b, dst_off=do, dst_i=di,
+                                            src_off=so, src_ld=sl, coef=cf)))
+    return tab
+
+
+def host_native_tables(kind, *args):
+    """Tables the native planner builds, without a device (h = NULL)."""
+    lib = _lib.load()
+    call, keep = native_builder(kind, *args)
+    ptr = _lib.c_void_p()
+    _lib.raise_for(call(None, _lib.ctypes.byref(ptr)), None, "native planner")
+    try:
+        return read_tables(ptr, lib)
+    finally:
+        lib.mh_bs_plan_destroy(None, ptr)
 
 
 def _dp(t):

@@ -24,6 +24,7 @@
 // for every precision; only the matrix-core arithmetic changes.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "mh_gemm_common.h"
@@ -77,14 +78,23 @@ struct BsStage {
   int amc, bkc;
   int nchunk;
   int ntile_v[NTS];          // tiles per shape (0: nothing to do)
-  GTile *tiles_v[NTS];
   mutable int choice;        // tuned variant (-1: not yet chosen)
   uint64_t key;              // shape class for the tuner
+  double flops;
+  // Device copies of the tables (set when the plan is uploaded).
+  GTile *tiles_v[NTS];
   GTask *tasks;
   GSeg *segs;
   MChunk *chunks;
   MContrib *contribs;
-  double flops;
+  // Host tables: the plan is built on the host and uploaded on first use.
+  std::vector<GTile> h_tiles[NTS];
+  std::vector<GTask> h_tasks;
+  std::vector<GSeg> h_segs;
+  std::vector<MChunk> h_chunks;
+  std::vector<MContrib> h_contribs;
+  std::vector<int64_t> h_doff;   // mode destinations as added (for inspection)
+  std::vector<int32_t> h_di;
 };
 
 }  // namespace mh
@@ -93,8 +103,11 @@ struct mh_bs_plan_s {
   int prec = MH_PREC_F64;
   int64_t size[mh::NSRC] = {0, 0, 0, 0, 0, 0};  // elements of each buffer
   std::vector<mh::BsStage> stages;
-  std::vector<void *> allocs;
   double flops = 0;
+  // One device allocation holds every table; made by the first apply on a
+  // handle, which fixes the plan's device.
+  void *dev = nullptr;
+  int device = -1;
 };
 
 namespace mh {
@@ -677,18 +690,77 @@ int bs_check_square(mh_handle h, const mh_bs_plan_s &p, const void *p1, const vo
   return MH_OK;
 }
 
+// Byte offsets of the device tables inside the plan's one allocation.
+static size_t al_bytes(size_t b) { return (b + 255) & ~size_t(255); }
+
 template <typename T>
-static int upload(mh_handle h, mh_bs_plan_s *p, const T *src, size_t n, T **dst) {
-  *dst = nullptr;
-  if (n == 0) return MH_OK;
-  void *d = nullptr;
-  if (hipMalloc(&d, n * sizeof(T)) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(h, MH_ERR_OOM, "plan allocation of %zu bytes", n * sizeof(T));
+static void lay_table(const std::vector<T> &v, size_t &at, std::vector<size_t> &offs) {
+  offs.push_back(at);
+  at += al_bytes(v.size() * sizeof(T));
+}
+
+// Upload the tables of a host-built plan to the handle's device (once); a
+// plan stays on the device of its first use.
+int bs_prepare(mh_handle h, mh_bs_plan_s &p) {
+  if (p.dev) {
+    if (p.device != h->device)
+      return fail(h, MH_ERR_ARG, "plan lives on device %d, handle on %d", p.device, h->device);
+    return MH_OK;
   }
-  p->allocs.push_back(d);
-  MH_HIP(h, hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
-  *dst = (T *)d;
+  size_t total = 0;
+  std::vector<size_t> offs;
+  for (const auto &st : p.stages) {
+    for (int sh = 0; sh < NTS; ++sh) lay_table(st.h_tiles[sh], total, offs);
+    lay_table(st.h_tasks, total, offs);
+    lay_table(st.h_segs, total, offs);
+    lay_table(st.h_chunks, total, offs);
+    lay_table(st.h_contribs, total, offs);
+  }
+  if (total == 0) total = 256;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(h->stream, &cs);
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(h, MH_ERR_ARG, "a plan's first use cannot be captured (it uploads its tables)");
+  std::vector<unsigned char> host(total, 0);
+  size_t k = 0;
+  auto put = [&](const auto &v) {
+    if (!v.empty()) std::memcpy(host.data() + offs[k], v.data(), v.size() * sizeof(v[0]));
+    ++k;
+  };
+  for (const auto &st : p.stages) {
+    for (int sh = 0; sh < NTS; ++sh) put(st.h_tiles[sh]);
+    put(st.h_tasks);
+    put(st.h_segs);
+    put(st.h_chunks);
+    put(st.h_contribs);
+  }
+  void *d = nullptr;
+  if (hipMalloc(&d, total) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, MH_ERR_OOM, "plan allocation of %zu bytes", total);
+  }
+  const hipError_t e1 = hipMemcpyAsync(d, host.data(), total, hipMemcpyHostToDevice, h->stream);
+  const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(h->stream) : e1;
+  if (e2 != hipSuccess) {
+    (void)hipFree(d);
+    return fail(h, MH_ERR_HIP, "plan upload: %s", hipGetErrorString(e2));
+  }
+  p.dev = d;
+  p.device = h->device;
+  unsigned char *base = (unsigned char *)d;
+  k = 0;
+  auto at = [&](const auto &v) -> void * {
+    void *r = v.empty() ? nullptr : (void *)(base + offs[k]);
+    ++k;
+    return r;
+  };
+  for (auto &st : p.stages) {
+    for (int sh = 0; sh < NTS; ++sh) st.tiles_v[sh] = (GTile *)at(st.h_tiles[sh]);
+    st.tasks = (GTask *)at(st.h_tasks);
+    st.segs = (GSeg *)at(st.h_segs);
+    st.chunks = (MChunk *)at(st.h_chunks);
+    st.contribs = (MContrib *)at(st.h_contribs);
+  }
   return MH_OK;
 }
 
@@ -706,7 +778,7 @@ using namespace mh;
 extern "C" {
 
 int mh_bs_plan_create(mh_handle h, int prec, const int64_t *sizes, mh_bs_plan *out) {
-  if (!h || !out || !sizes) return MH_ERR_ARG;
+  if (!out || !sizes) return MH_ERR_ARG;
   *out = nullptr;
   if (prec != MH_PREC_F64 && prec != MH_PREC_F32 && prec != MH_PREC_BF16X3)
     return fail(h, MH_ERR_ARG, "unknown precision %d", prec);
@@ -723,7 +795,7 @@ int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int 
                         int amc, int bkc, int ntask, const int32_t *task_i,
                         const int64_t *task_coff, int nseg, const int64_t *seg_off,
                         const int32_t *seg_i) {
-  if (!h || !plan) return MH_ERR_ARG;
+  if (!plan) return MH_ERR_ARG;
   if (ntask < 0 || nseg < 0) return fail(h, MH_ERR_SHAPE, "negative task count");
   if ((ntask > 0 && (!task_i || !task_coff)) || (nseg > 0 && (!seg_off || !seg_i)))
     return fail(h, MH_ERR_ARG, "null task or segment table");
@@ -802,12 +874,11 @@ int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int 
     st.key = (uint64_t)(st.amc | (st.bkc << 1)) | (lb(sm / ntask) << 2) | (lb(sn / ntask) << 12) |
              (lb(sk / ntask) << 22) | (lb((double)ntask) << 32);
   }
-  MH_TRY(upload(h, plan, tasks.data(), tasks.size(), &st.tasks));
-  MH_TRY(upload(h, plan, segs.data(), segs.size(), &st.segs));
+  st.h_tasks = std::move(tasks);
+  st.h_segs = std::move(segs);
   for (int sh = 0; sh < NTS; ++sh)
-    if (st.ntile_v[sh] > 0) MH_TRY(upload(h, plan, lists[sh].data(), lists[sh].size(), &st.tiles_v[sh]));
-  MH_HIP(h, hipStreamSynchronize(h->stream));
-  plan->stages.push_back(st);
+    if (st.ntile_v[sh] > 0) st.h_tiles[sh] = std::move(lists[sh]);
+  plan->stages.push_back(std::move(st));
   plan->flops += flops;
   return MH_OK;
 }
@@ -815,7 +886,7 @@ int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int 
 int mh_bs_plan_add_mode(mh_handle h, mh_bs_plan plan, int in_src, int out_src, int ndst,
                         const int64_t *dst_off, const int32_t *dst_i, int ncontrib,
                         const int64_t *src_off, const int32_t *src_ld, const double *coef) {
-  if (!h || !plan) return MH_ERR_ARG;
+  if (!plan) return MH_ERR_ARG;
   if (ndst < 0 || ncontrib < 0) return fail(h, MH_ERR_SHAPE, "negative count");
   if ((ndst > 0 && (!dst_off || !dst_i)) || (ncontrib > 0 && (!src_off || !src_ld || !coef)))
     return fail(h, MH_ERR_ARG, "null block or contribution table");
@@ -848,18 +919,22 @@ int mh_bs_plan_add_mode(mh_handle h, mh_bs_plan plan, int in_src, int out_src, i
   st.c_src = out_src;
   st.nchunk = (int)chunks.size();
   st.flops = flops;
-  MH_TRY(upload(h, plan, chunks.data(), chunks.size(), &st.chunks));
-  MH_TRY(upload(h, plan, cbs.data(), cbs.size(), &st.contribs));
-  MH_HIP(h, hipStreamSynchronize(h->stream));
-  plan->stages.push_back(st);
+  st.h_chunks = std::move(chunks);
+  st.h_contribs = std::move(cbs);
+  st.h_doff.assign(dst_off, dst_off + ndst);
+  st.h_di.assign(dst_i, dst_i + 5 * (size_t)ndst);
+  plan->stages.push_back(std::move(st));
   plan->flops += flops;
   return MH_OK;
 }
 
 int mh_bs_plan_destroy(mh_handle h, mh_bs_plan plan) {
-  if (!h || !plan) return MH_ERR_ARG;
-  (void)hipStreamSynchronize(h->stream);
-  for (void *a : plan->allocs) (void)hipFree(a);
+  if (!plan) return MH_ERR_ARG;
+  if (plan->dev) {
+    // work queued on the handle may still read the tables
+    if (h) (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(plan->dev);
+  }
   delete plan;
   return MH_OK;
 }
@@ -885,10 +960,80 @@ int mh_bs_plan_flops(mh_bs_plan plan, double *flops) {
   return MH_OK;
 }
 
+int mh_bs_plan_sizes(mh_bs_plan plan, int64_t *sizes) {
+  if (!plan || !sizes) return MH_ERR_ARG;
+  for (int i = 0; i < NSRC; ++i) sizes[i] = plan->size[i];
+  return MH_OK;
+}
+
+int mh_bs_plan_stage_count(mh_bs_plan plan, int *n) {
+  if (!plan || !n) return MH_ERR_ARG;
+  *n = (int)plan->stages.size();
+  return MH_OK;
+}
+
+int mh_bs_plan_stage_info(mh_bs_plan plan, int stage, int32_t *info) {
+  if (!plan || !info || stage < 0 || stage >= (int)plan->stages.size()) return MH_ERR_ARG;
+  const BsStage &st = plan->stages[stage];
+  if (st.kind == 0) {
+    const int32_t v[8] = {0, st.a_src, st.b_src, st.c_src, st.amc, st.bkc,
+                          (int32_t)st.h_tasks.size(), (int32_t)st.h_segs.size()};
+    std::memcpy(info, v, sizeof(v));
+  } else {
+    const int32_t v[8] = {1, st.a_src, st.c_src, 0, 0, 0, (int32_t)st.h_doff.size(),
+                          (int32_t)st.h_contribs.size()};
+    std::memcpy(info, v, sizeof(v));
+  }
+  return MH_OK;
+}
+
+int mh_bs_plan_stage_gemm(mh_bs_plan plan, int stage, int32_t *task_i, int64_t *task_coff,
+                          int64_t *seg_off, int32_t *seg_i) {
+  if (!plan || stage < 0 || stage >= (int)plan->stages.size()) return MH_ERR_ARG;
+  const BsStage &st = plan->stages[stage];
+  if (st.kind != 0) return MH_ERR_ARG;
+  for (size_t i = 0; i < st.h_tasks.size(); ++i) {
+    const GTask &t = st.h_tasks[i];
+    if (task_i) {
+      const int32_t v[5] = {t.M, t.N, t.seg0, t.nseg, t.ldc};
+      std::memcpy(task_i + 5 * i, v, sizeof(v));
+    }
+    if (task_coff) task_coff[i] = t.c_off;
+  }
+  for (size_t i = 0; i < st.h_segs.size(); ++i) {
+    const GSeg &g = st.h_segs[i];
+    if (seg_off) {
+      seg_off[2 * i] = g.a_off;
+      seg_off[2 * i + 1] = g.b_off;
+    }
+    if (seg_i) {
+      const int32_t v[3] = {g.K, g.lda, g.ldb};
+      std::memcpy(seg_i + 3 * i, v, sizeof(v));
+    }
+  }
+  return MH_OK;
+}
+
+int mh_bs_plan_stage_mode(mh_bs_plan plan, int stage, int64_t *dst_off, int32_t *dst_i,
+                          int64_t *src_off, int32_t *src_ld, double *coef) {
+  if (!plan || stage < 0 || stage >= (int)plan->stages.size()) return MH_ERR_ARG;
+  const BsStage &st = plan->stages[stage];
+  if (st.kind != 1) return MH_ERR_ARG;
+  if (dst_off) std::copy(st.h_doff.begin(), st.h_doff.end(), dst_off);
+  if (dst_i) std::copy(st.h_di.begin(), st.h_di.end(), dst_i);
+  for (size_t c = 0; c < st.h_contribs.size(); ++c) {
+    if (src_off) src_off[c] = st.h_contribs[c].src_off;
+    if (src_ld) src_ld[c] = st.h_contribs[c].ld_src;
+    if (coef) coef[c] = st.h_contribs[c].coef;
+  }
+  return MH_OK;
+}
+
 int mh_bs_apply(mh_handle h, mh_bs_plan plan, const void *x, const void *p1, const void *p2,
                 void *y) {
   if (!h || !plan) return MH_ERR_ARG;
   MH_TRY(bs_check_buffers(h, *plan, x, p1, p2, y));
+  MH_TRY(bs_prepare(h, *plan));
   MH_TRY(ensure_ws(h, bs_ws_bytes(*plan)));
   return bs_apply_ws(h, *plan, x, p1, p2, y, h->ws);
 }

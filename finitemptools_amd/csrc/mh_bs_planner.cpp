@@ -1,0 +1,559 @@
+// Native planner for the block-sparse contractions: builds the stage tables of
+// a plan directly from the charge sectors of the bonds, the physical charges
+// and the MPO tensors, with the storage conventions of the block buffers:
+//
+//   MPS-type tensor A[a, s1..sn, b], q(a) + sum q(s) = q(b): blocks (qa, s..)
+//   in the order of the left sectors, then the physical indices (first index
+//   slowest); one left sector's blocks are adjacent (its group, a D_qa x ncols
+//   column-major matrix).
+//   Environment E[a_bra, w, a_ket], q(a_bra) = q(w) + q(a_ket): blocks (q, w)
+//   in the order of the sectors, then the channels; one sector's blocks form
+//   a D_q x K_q column-major matrix.
+//
+// The loops and their orders are those of the Python planner
+// (finitemptools_amd/blocksparse.py), so both produce the same tables.
+#include <cstdint>
+#include <array>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+namespace {
+
+constexpr int MAXQ = 4;
+
+// A charge packed into 64 bits (16 bits per component, offset binary).
+inline int64_t qkey(const int32_t *q, int nq) {
+  int64_t k = 0;
+  for (int i = 0; i < nq; ++i) k |= (int64_t)((uint32_t)(q[i] + 32768) & 0xffff) << (16 * i);
+  return k;
+}
+inline int64_t qadd(int64_t a, int64_t b, int nq) {
+  int64_t k = 0;
+  for (int i = 0; i < nq; ++i) {
+    const int x = (int)((a >> (16 * i)) & 0xffff) - 32768 + (int)((b >> (16 * i)) & 0xffff) - 32768;
+    k |= (int64_t)((uint32_t)(x + 32768) & 0xffff) << (16 * i);
+  }
+  return k;
+}
+inline int64_t qneg(int64_t a, int nq) {
+  int64_t k = 0;
+  for (int i = 0; i < nq; ++i) {
+    const int x = -((int)((a >> (16 * i)) & 0xffff) - 32768);
+    k |= (int64_t)((uint32_t)(x + 32768) & 0xffff) << (16 * i);
+  }
+  return k;
+}
+inline int64_t qsub(int64_t a, int64_t b, int nq) { return qadd(a, qneg(b, nq), nq); }
+
+struct Sp {  // sectors in storage order
+  std::vector<int64_t> q;
+  std::vector<int> dim;
+  std::unordered_map<int64_t, int> idx;
+  int find(int64_t k) const {
+    auto it = idx.find(k);
+    return it == idx.end() ? -1 : it->second;
+  }
+};
+
+struct Blk {
+  int qa, qb;     // sector indices (left, right)
+  int p0, p1;     // physical indices (p1 = -1 for one leg)
+  int64_t off;
+  int rows, cols, colstart;
+};
+
+struct Lay {  // MPS-type layout
+  const Sp *L, *R;
+  int d, nphys;
+  std::vector<Blk> blocks;
+  std::unordered_map<int64_t, int> index;  // (qa, p0, p1) -> block
+  std::vector<int64_t> g0;                 // per left sector: group offset
+  std::vector<int> gcols;                  // per left sector: group columns (0: none)
+  std::vector<std::vector<int>> by_right, by_left;
+  int64_t size = 0;
+  static int64_t key(int qa, int p0, int p1) { return ((int64_t)qa << 20) | ((p0 + 1) << 10) | (p1 + 1); }
+  int find(int qa, int p0, int p1) const {
+    auto it = index.find(key(qa, p0, p1));
+    return it == index.end() ? -1 : it->second;
+  }
+};
+
+struct Env {  // environment layout
+  const Sp *S;
+  std::vector<int64_t> qw;
+  std::vector<int64_t> off;       // (q * w + ch) -> offset, -1 if absent
+  std::vector<int> colstart;      // (q * w + ch)
+  std::vector<int64_t> g0;        // per sector
+  std::vector<int> K;             // per sector (0: none)
+  int64_t size = 0;
+  int w = 0;
+};
+
+struct Ctx {
+  mh_handle h;
+  int nq;
+};
+
+int mk_space(mh_handle h, const mh_bs_sectors *s, int nq, Sp &out) {
+  if (!s || s->n < 0 || (s->n > 0 && (!s->charges || !s->dims)))
+    return fail(h, MH_ERR_ARG, "bad sector list");
+  if (s->nq != nq) return fail(h, MH_ERR_ARG, "charge components differ");
+  out.q.resize(s->n);
+  out.dim.resize(s->n);
+  for (int i = 0; i < s->n; ++i) {
+    out.q[i] = qkey(s->charges + (size_t)i * nq, nq);
+    out.dim[i] = s->dims[i];
+    if (out.dim[i] <= 0) return fail(h, MH_ERR_SHAPE, "sector %d has dimension %d", i, out.dim[i]);
+    if (!out.idx.emplace(out.q[i], i).second) return fail(h, MH_ERR_ARG, "repeated sector %d", i);
+  }
+  return MH_OK;
+}
+
+void mk_layout(const Sp &L, const Sp &R, const std::vector<int64_t> &phys, int nphys, Lay &lay,
+               int nq) {
+  lay.L = &L;
+  lay.R = &R;
+  lay.d = (int)phys.size();
+  lay.nphys = nphys;
+  lay.g0.assign(L.q.size(), 0);
+  lay.gcols.assign(L.q.size(), 0);
+  lay.by_right.assign(R.q.size(), {});
+  lay.by_left.assign(L.q.size(), {});
+  int64_t off = 0;
+  const int np0 = lay.d, np1 = nphys == 2 ? lay.d : 1;
+  for (int a = 0; a < (int)L.q.size(); ++a) {
+    const int64_t g = off;
+    int ncols = 0;
+    for (int p0 = 0; p0 < np0; ++p0)
+      for (int p1 = 0; p1 < np1; ++p1) {
+        int64_t qb = qadd(L.q[a], phys[p0], nq);
+        if (nphys == 2) qb = qadd(qb, phys[p1], nq);
+        const int b = R.find(qb);
+        if (b < 0) continue;
+        Blk blk{a, b, p0, nphys == 2 ? p1 : -1, off, L.dim[a], R.dim[b], ncols};
+        const int id = (int)lay.blocks.size();
+        lay.blocks.push_back(blk);
+        lay.index.emplace(Lay::key(a, blk.p0, blk.p1), id);
+        lay.by_right[b].push_back(id);
+        lay.by_left[a].push_back(id);
+        off += (int64_t)blk.rows * blk.cols;
+        ncols += blk.cols;
+      }
+    if (ncols) {
+      lay.g0[a] = g;
+      lay.gcols[a] = ncols;
+    }
+  }
+  lay.size = off;
+}
+
+void mk_env(const Sp &S, const std::vector<int64_t> &qw, Env &e, int nq) {
+  e.S = &S;
+  e.qw = qw;
+  e.w = (int)qw.size();
+  const int ns = (int)S.q.size();
+  e.off.assign((size_t)ns * e.w, -1);
+  e.colstart.assign((size_t)ns * e.w, 0);
+  e.g0.assign(ns, 0);
+  e.K.assign(ns, 0);
+  int64_t off = 0;
+  for (int q = 0; q < ns; ++q) {
+    const int64_t g = off;
+    int K = 0;
+    for (int c = 0; c < e.w; ++c) {
+      const int k = S.find(qsub(S.q[q], qw[c], nq));
+      if (k < 0) continue;
+      e.off[(size_t)q * e.w + c] = off;
+      e.colstart[(size_t)q * e.w + c] = K;
+      off += (int64_t)S.dim[q] * S.dim[k];
+      K += S.dim[k];
+    }
+    if (K) {
+      e.g0[q] = g;
+      e.K[q] = K;
+    }
+  }
+  e.size = off;
+}
+
+// Stage tables in the form mh_bs_plan_add_gemm / _add_mode take.
+struct GemmTab {
+  std::vector<int32_t> ti;   // M, N, seg0, nseg, ldc
+  std::vector<int64_t> tc;   // c_off
+  std::vector<int64_t> so;   // a_off, b_off
+  std::vector<int32_t> si;   // K, lda, ldb
+  void task(int M, int N, int64_t coff, int ldc) {
+    ti.insert(ti.end(), {M, N, (int32_t)(so.size() / 2), 0, ldc});
+    tc.push_back(coff);
+  }
+  void seg(int64_t a, int64_t b, int K, int lda, int ldb) {
+    so.insert(so.end(), {a, b});
+    si.insert(si.end(), {K, lda, ldb});
+    ti[ti.size() - 2] += 1;  // nseg of the last task
+  }
+  int add(mh_handle h, mh_bs_plan p, int a, int b, int c, int amc, int bkc) {
+    return mh_bs_plan_add_gemm(h, p, a, b, c, amc, bkc, (int)tc.size(), ti.data(), tc.data(),
+                               (int)(si.size() / 3), so.data(), si.data());
+  }
+};
+struct ModeTab {
+  std::vector<int64_t> doff;
+  std::vector<int32_t> di;   // rows, cols, ld, c0, nc
+  std::vector<int64_t> soff;
+  std::vector<int32_t> sld;
+  std::vector<double> coef;
+  void dest(int64_t off, int rows, int cols, int ld) {
+    doff.push_back(off);
+    di.insert(di.end(), {rows, cols, ld, (int32_t)coef.size(), 0});
+  }
+  void contrib(int64_t off, int ld, double c) {
+    soff.push_back(off);
+    sld.push_back(ld);
+    coef.push_back(c);
+    di[di.size() - 1] += 1;
+  }
+  int add(mh_handle h, mh_bs_plan p, int in, int out) {
+    return mh_bs_plan_add_mode(h, p, in, out, (int)doff.size(), doff.data(), di.data(),
+                               (int)coef.size(), soff.data(), sld.data(), coef.data());
+  }
+};
+
+// W[wl, so, wr, si] column-major
+inline double wat(const double *W, int wl, int d, int wr, int a, int so, int b, int si) {
+  return W[a + (int64_t)wl * (so + (int64_t)d * (b + (int64_t)wr * si))];
+}
+
+// T1[(qa,w)] = L[(qa,w)] * x_group[qa - q(w)]: tasks, T1 offsets (-1: none), T1 size.
+int64_t gemm1_env_vec(const Env &E, const Lay &x, GemmTab &g, std::vector<int64_t> &t1off,
+                      int nq) {
+  const Sp &S = *E.S;
+  t1off.assign((size_t)S.q.size() * E.w, -1);
+  int64_t t1 = 0;
+  for (int qa = 0; qa < (int)S.q.size(); ++qa) {
+    const int Da = S.dim[qa];
+    for (int w = 0; w < E.w; ++w) {
+      const int64_t eo = E.off[(size_t)qa * E.w + w];
+      if (eo < 0) continue;
+      const int qk = S.find(qsub(S.q[qa], E.qw[w], nq));
+      if (qk < 0 || x.gcols[qk] == 0) continue;
+      const int ncols = x.gcols[qk], Dk = S.dim[qk];
+      t1off[(size_t)qa * E.w + w] = t1;
+      g.task(Da, ncols, t1, Da);
+      g.seg(eo, x.g0[qk], Dk, Da, Dk);
+      t1 += (int64_t)Da * ncols;
+    }
+  }
+  return t1;
+}
+
+int finish(mh_handle h, int prec, const int64_t sizes[6], const std::vector<int> &kinds,
+           std::vector<GemmTab> &g, std::vector<ModeTab> &m,
+           const std::vector<std::array<int, 5>> &args, mh_bs_plan *out) {
+  mh_bs_plan p = nullptr;
+  MH_TRY(mh_bs_plan_create(h, prec, sizes, &p));
+  size_t gi = 0, mi = 0;
+  for (size_t s = 0; s < kinds.size(); ++s) {
+    const auto &a = args[s];
+    const int st = kinds[s] == 0 ? g[gi++].add(h, p, a[0], a[1], a[2], a[3], a[4])
+                                 : m[mi++].add(h, p, a[0], a[1]);
+    if (st != MH_OK) {
+      mh_bs_plan_destroy(h, p);
+      return st;
+    }
+  }
+  *out = p;
+  return MH_OK;
+}
+
+int read_common(mh_handle h, int d, const int32_t *phys, int nq, std::vector<int64_t> &ph) {
+  if (d <= 0 || !phys) return fail(h, MH_ERR_ARG, "no physical charges");
+  ph.resize(d);
+  for (int s = 0; s < d; ++s) ph[s] = qkey(phys + (size_t)s * nq, nq);
+  return MH_OK;
+}
+
+std::vector<int64_t> read_qw(const int32_t *q, int w, int nq) {
+  std::vector<int64_t> v(w);
+  for (int i = 0; i < w; ++i) v[i] = qkey(q + (size_t)i * nq, nq);
+  return v;
+}
+
+}  // namespace
+}  // namespace mh
+
+using namespace mh;
+
+extern "C" {
+
+int mh_bs_plan_heff2(mh_handle h, int prec, const mh_bs_sectors *left,
+                     const mh_bs_sectors *right, int d, const int32_t *phys, int wl,
+                     const int32_t *qwl, int wm, int wr, const int32_t *qwr, const double *W1,
+                     const double *W2, mh_bs_plan *plan) {
+  if (!left || !right || !plan || !W1 || !W2 || !qwl || !qwr) return MH_ERR_ARG;
+  const int nq = left->nq;
+  if (nq < 1 || nq > MAXQ) return fail(h, MH_ERR_ARG, "1 to %d charge components", MAXQ);
+  if (wl <= 0 || wm <= 0 || wr <= 0) return fail(h, MH_ERR_SHAPE, "non-positive MPO dimension");
+  Sp Sl, Sr;
+  std::vector<int64_t> ph;
+  MH_TRY(mk_space(h, left, nq, Sl));
+  MH_TRY(mk_space(h, right, nq, Sr));
+  MH_TRY(read_common(h, d, phys, nq, ph));
+  Lay x;
+  mk_layout(Sl, Sr, ph, 2, x, nq);
+  Env Le, Re;
+  mk_env(Sl, read_qw(qwl, wl, nq), Le, nq);
+  mk_env(Sr, read_qw(qwr, wr, nq), Re, nq);
+  // Mt[w, s1', s2', w', s1, s2] = sum_m W1[w,s1',m,s1] W2[m,s2',w',s2], keyed by
+  // (s1', s2', w') and listed in (w, s1, s2) order.
+  struct Ent { int w, s1, s2; double c; };
+  std::vector<std::vector<Ent>> ent((size_t)d * d * wr);
+  for (int w = 0; w < wl; ++w)
+    for (int s1o = 0; s1o < d; ++s1o)
+      for (int s2o = 0; s2o < d; ++s2o)
+        for (int wo = 0; wo < wr; ++wo)
+          for (int s1 = 0; s1 < d; ++s1)
+            for (int s2 = 0; s2 < d; ++s2) {
+              double c = 0.0;
+              for (int m = 0; m < wm; ++m)
+                c += wat(W1, wl, d, wm, w, s1o, m, s1) * wat(W2, wm, d, wr, m, s2o, wo, s2);
+              if (c != 0.0) ent[(size_t)s1o + d * (s2o + (size_t)d * wo)].push_back({w, s1, s2, c});
+            }
+  std::vector<GemmTab> g(2);
+  std::vector<ModeTab> md(1);
+  std::vector<int64_t> t1off;
+  const int64_t t1 = gemm1_env_vec(Le, x, g[0], t1off, nq);
+  // T2 per right sector: rows = the blocks ending there, columns = R's sector matrix.
+  const int ns = (int)Sr.q.size();
+  std::vector<int64_t> t2off(ns, -1);
+  std::vector<int> Rrows(ns, 0);
+  std::vector<int> rowoff(x.blocks.size(), 0);
+  int64_t t2 = 0;
+  for (int qb = 0; qb < ns; ++qb) {
+    if (x.by_right[qb].empty()) continue;
+    if (Re.K[qb] == 0) return fail(h, MH_ERR_ARG, "right environment has no block for sector %d", qb);
+    int r = 0;
+    for (int id : x.by_right[qb]) {
+      rowoff[id] = r;
+      r += x.blocks[id].rows;
+    }
+    Rrows[qb] = r;
+    t2off[qb] = t2;
+    t2 += (int64_t)r * Re.K[qb];
+  }
+  for (int qb = 0; qb < ns; ++qb) {
+    if (t2off[qb] < 0) continue;
+    const int Rq = Rrows[qb];
+    for (int id : x.by_right[qb]) {
+      const Blk &b = x.blocks[id];
+      const int qa = b.qa, Da = b.rows;
+      for (int w2 = 0; w2 < wr; ++w2) {
+        const int64_t ro = Re.off[(size_t)qb * wr + w2];
+        if (ro < 0) continue;
+        const int qbk = Sr.find(qsub(Sr.q[qb], Re.qw[w2], nq));
+        const int cols = Sr.dim[qbk];
+        md[0].dest(t2off[qb] + rowoff[id] + (int64_t)Rq * Re.colstart[(size_t)qb * wr + w2], Da,
+                   cols, Rq);
+        for (const Ent &e : ent[(size_t)b.p0 + d * (b.p1 + (size_t)d * w2)]) {
+          const int64_t to = t1off[(size_t)qa * wl + e.w];
+          if (to < 0) continue;
+          const int qk = Sl.find(qsub(Sl.q[qa], Le.qw[e.w], nq));
+          const int src = x.find(qk, e.s1, e.s2);
+          if (src < 0) continue;
+          if (x.blocks[src].qb != qbk) return fail(h, MH_ERR_ARG, "MPO entry does not conserve the charges");
+          md[0].contrib(to + (int64_t)Da * x.blocks[src].colstart, Da, e.c);
+        }
+      }
+    }
+  }
+  for (size_t id = 0; id < x.blocks.size(); ++id) {
+    const Blk &b = x.blocks[id];
+    g[1].task(b.rows, b.cols, b.off, b.rows);
+    g[1].seg(t2off[b.qb] + rowoff[id], Re.g0[b.qb], Re.K[b.qb], Rrows[b.qb], b.cols);
+  }
+  const int64_t sizes[6] = {x.size, x.size, Le.size, Re.size, t1, t2};
+  return finish(h, prec, sizes, {0, 1, 0}, g, md,
+                {{MH_BS_P1, MH_BS_X, MH_BS_T1, 1, 1}, {MH_BS_T1, MH_BS_T2, 0, 0, 0},
+                 {MH_BS_T2, MH_BS_P2, MH_BS_Y, 1, 0}},
+                plan);
+}
+
+int mh_bs_plan_updateleft(mh_handle h, int prec, const mh_bs_sectors *left,
+                          const mh_bs_sectors *right, int d, const int32_t *phys, int wl,
+                          const int32_t *qwl, int wr, const int32_t *qwr, const double *W,
+                          mh_bs_plan *plan) {
+  if (!left || !right || !plan || !W || !qwl || !qwr) return MH_ERR_ARG;
+  const int nq = left->nq;
+  if (nq < 1 || nq > MAXQ) return fail(h, MH_ERR_ARG, "1 to %d charge components", MAXQ);
+  Sp Sl, Sr;
+  std::vector<int64_t> ph;
+  MH_TRY(mk_space(h, left, nq, Sl));
+  MH_TRY(mk_space(h, right, nq, Sr));
+  MH_TRY(read_common(h, d, phys, nq, ph));
+  Lay al;
+  mk_layout(Sl, Sr, ph, 1, al, nq);
+  Env Le, Ln;
+  mk_env(Sl, read_qw(qwl, wl, nq), Le, nq);
+  mk_env(Sr, read_qw(qwr, wr, nq), Ln, nq);
+  struct Ent { int w, si; double c; };
+  std::vector<std::vector<Ent>> ent((size_t)d * wr);  // keyed (so, w2), listed in (w, si) order
+  for (int w = 0; w < wl; ++w)
+    for (int so = 0; so < d; ++so)
+      for (int w2 = 0; w2 < wr; ++w2)
+        for (int si = 0; si < d; ++si) {
+          const double c = wat(W, wl, d, wr, w, so, w2, si);
+          if (c != 0.0) ent[(size_t)so + d * w2].push_back({w, si, c});
+        }
+  std::vector<GemmTab> g(2);
+  std::vector<ModeTab> md(1);
+  std::vector<int64_t> t1off;
+  const int64_t t1 = gemm1_env_vec(Le, al, g[0], t1off, nq);
+  const int ns = (int)Sr.q.size();
+  std::vector<int> RA(ns, 0), rowoff(al.blocks.size(), 0);
+  for (int qb = 0; qb < ns; ++qb) {
+    int r = 0;
+    for (int id : al.by_right[qb]) {
+      rowoff[id] = r;
+      r += al.blocks[id].rows;
+    }
+    RA[qb] = r;
+  }
+  int64_t t2 = 0;
+  for (int qb = 0; qb < ns; ++qb)
+    for (int w2 = 0; w2 < wr; ++w2) {
+      const int64_t loff = Ln.off[(size_t)qb * wr + w2];
+      if (loff < 0) continue;
+      const int qbk = Sr.find(qsub(Sr.q[qb], Ln.qw[w2], nq));
+      const int cols = Sr.dim[qbk];
+      if (RA[qb] == 0) return fail(h, MH_ERR_ARG, "sector %d of the right bond is unreachable", qb);
+      g[1].task(Sr.dim[qb], cols, loff, Sr.dim[qb]);
+      for (int id : al.by_right[qb]) {
+        const Blk &b = al.blocks[id];
+        const int qa = b.qa, Da = b.rows;
+        md[0].dest(t2 + rowoff[id], Da, cols, RA[qb]);
+        for (const Ent &e : ent[(size_t)b.p0 + d * w2]) {
+          const int64_t to = t1off[(size_t)qa * wl + e.w];
+          if (to < 0) continue;
+          const int qk = Sl.find(qsub(Sl.q[qa], Le.qw[e.w], nq));
+          const int src = al.find(qk, e.si, -1);
+          if (src < 0) continue;
+          if (al.blocks[src].qb != qbk) return fail(h, MH_ERR_ARG, "MPO entry does not conserve the charges");
+          md[0].contrib(to + (int64_t)Da * al.blocks[src].colstart, Da, e.c);
+        }
+        g[1].seg(b.off, t2 + rowoff[id], Da, Da, RA[qb]);
+      }
+      t2 += (int64_t)RA[qb] * cols;
+    }
+  const int64_t sizes[6] = {al.size, Ln.size, Le.size, al.size, t1, t2};
+  return finish(h, prec, sizes, {0, 1, 0}, g, md,
+                {{MH_BS_P1, MH_BS_X, MH_BS_T1, 1, 1}, {MH_BS_T1, MH_BS_T2, 0, 0, 0},
+                 {MH_BS_P2, MH_BS_T2, MH_BS_Y, 0, 1}},
+                plan);
+}
+
+int mh_bs_plan_updateright(mh_handle h, int prec, const mh_bs_sectors *left,
+                           const mh_bs_sectors *right, int d, const int32_t *phys, int wl,
+                           const int32_t *qwl, int wr, const int32_t *qwr, const double *W,
+                           mh_bs_plan *plan) {
+  if (!left || !right || !plan || !W || !qwl || !qwr) return MH_ERR_ARG;
+  const int nq = left->nq;
+  if (nq < 1 || nq > MAXQ) return fail(h, MH_ERR_ARG, "1 to %d charge components", MAXQ);
+  Sp Sl, Sr;
+  std::vector<int64_t> ph;
+  MH_TRY(mk_space(h, left, nq, Sl));
+  MH_TRY(mk_space(h, right, nq, Sr));
+  MH_TRY(read_common(h, d, phys, nq, ph));
+  Lay al;
+  mk_layout(Sl, Sr, ph, 1, al, nq);
+  Env Re, Rn;
+  mk_env(Sr, read_qw(qwr, wr, nq), Re, nq);
+  mk_env(Sl, read_qw(qwl, wl, nq), Rn, nq);
+  struct Ent { int so, w2; double c; };
+  std::vector<std::vector<Ent>> ent((size_t)wl * d);  // keyed (w, si), listed in (so, w2) order
+  for (int w = 0; w < wl; ++w)
+    for (int so = 0; so < d; ++so)
+      for (int w2 = 0; w2 < wr; ++w2)
+        for (int si = 0; si < d; ++si) {
+          const double c = wat(W, wl, d, wr, w, so, w2, si);
+          if (c != 0.0) ent[(size_t)w + wl * si].push_back({so, w2, c});
+        }
+  std::vector<GemmTab> g(2);
+  std::vector<ModeTab> md(1);
+  std::vector<int64_t> uoff(al.blocks.size(), -1);
+  int64_t t1 = 0;
+  for (size_t id = 0; id < al.blocks.size(); ++id) {
+    const Blk &b = al.blocks[id];
+    if (Re.K[b.qb] == 0) continue;
+    uoff[id] = t1;
+    g[0].task(b.rows, Re.K[b.qb], t1, b.rows);
+    g[0].seg(b.off, Re.g0[b.qb], b.cols, b.rows, b.cols);
+    t1 += (int64_t)b.rows * Re.K[b.qb];
+  }
+  int64_t t2 = 0;
+  const int nsl = (int)Sl.q.size();
+  for (int qa = 0; qa < nsl; ++qa)
+    for (int w = 0; w < wl; ++w) {
+      const int64_t roff = Rn.off[(size_t)qa * wl + w];
+      if (roff < 0) continue;
+      const int Da = Sl.dim[qa];
+      const int qk = Sl.find(qsub(Sl.q[qa], Rn.qw[w], nq));
+      if (al.gcols[qk] == 0) return fail(h, MH_ERR_ARG, "sector %d of the left bond is unreachable", qk);
+      for (int id : al.by_left[qk]) {
+        const Blk &b = al.blocks[id];
+        md[0].dest(t2 + (int64_t)Da * b.colstart, Da, b.cols, Da);
+        for (const Ent &e : ent[(size_t)w + wl * b.p0]) {
+          const int qb = Sr.find(qadd(Sl.q[qa], ph[e.so], nq));
+          const int ua = al.find(qa, e.so, -1);
+          if (ua < 0 || uoff[ua] < 0 || qb < 0 || Re.off[(size_t)qb * wr + e.w2] < 0) continue;
+          if (Sr.find(qsub(Sr.q[qb], Re.qw[e.w2], nq)) != b.qb)
+            return fail(h, MH_ERR_ARG, "MPO entry does not conserve the charges");
+          md[0].contrib(uoff[ua] + (int64_t)Da * Re.colstart[(size_t)qb * wr + e.w2], Da, e.c);
+        }
+      }
+      g[1].task(Da, Sl.dim[qk], roff, Da);
+      g[1].seg(t2, al.g0[qk], al.gcols[qk], Da, Sl.dim[qk]);
+      t2 += (int64_t)Da * al.gcols[qk];
+    }
+  const int64_t sizes[6] = {al.size, Rn.size, Re.size, al.size, t1, t2};
+  return finish(h, prec, sizes, {0, 1, 0}, g, md,
+                {{MH_BS_P2, MH_BS_P1, MH_BS_T1, 1, 1}, {MH_BS_T1, MH_BS_T2, 0, 0, 0},
+                 {MH_BS_T2, MH_BS_X, MH_BS_Y, 1, 0}},
+                plan);
+}
+
+int mh_bs_plan_two_site(mh_handle h, const mh_bs_sectors *left, const mh_bs_sectors *mid,
+                        const mh_bs_sectors *right, int d, const int32_t *phys, mh_bs_plan *plan) {
+  if (!left || !mid || !right || !plan) return MH_ERR_ARG;
+  const int nq = left->nq;
+  if (nq < 1 || nq > MAXQ) return fail(h, MH_ERR_ARG, "1 to %d charge components", MAXQ);
+  Sp Sl, Sm, Sr;
+  std::vector<int64_t> ph;
+  MH_TRY(mk_space(h, left, nq, Sl));
+  MH_TRY(mk_space(h, mid, nq, Sm));
+  MH_TRY(mk_space(h, right, nq, Sr));
+  MH_TRY(read_common(h, d, phys, nq, ph));
+  Lay l1, l2, xl;
+  mk_layout(Sl, Sm, ph, 1, l1, nq);
+  mk_layout(Sm, Sr, ph, 1, l2, nq);
+  mk_layout(Sl, Sr, ph, 2, xl, nq);
+  std::vector<GemmTab> g(1);
+  std::vector<ModeTab> md;
+  for (const Blk &b : xl.blocks) {
+    const int i1 = l1.find(b.qa, b.p0, -1);
+    if (i1 < 0) continue;
+    const Blk &b1 = l1.blocks[i1];
+    const int i2 = l2.find(b1.qb, b.p1, -1);
+    if (i2 < 0) continue;
+    const Blk &b2 = l2.blocks[i2];
+    g[0].task(b.rows, b.cols, b.off, b.rows);
+    g[0].seg(b1.off, b2.off, b1.cols, b1.rows, b2.rows);
+  }
+  const int64_t sizes[6] = {l1.size, xl.size, l2.size, 0, 0, 0};
+  return finish(h, MH_PREC_F64, sizes, {0}, g, md, {{MH_BS_X, MH_BS_P1, MH_BS_Y, 1, 1}}, plan);
+}
+
+}  // extern "C"

@@ -135,6 +135,8 @@ size_t bs_ws_bytes(const mh_bs_plan_s &p);
 size_t bs_vec_len(const mh_bs_plan_s &p);
 int bs_apply_ws(mh_handle h, const mh_bs_plan_s &p, const void *x, const void *p1,
                 const void *p2, void *y, void *ws);
+// Upload a host-built plan's tables to the handle's device (first use only).
+int bs_prepare(mh_handle h, mh_bs_plan_s &p);
 // An operator plan (x and y of one size) with the environment buffers it needs.
 int bs_check_square(mh_handle h, const mh_bs_plan_s &p, const void *p1, const void *p2);
 

@@ -703,6 +703,7 @@ int mh_bs_lanczos_eig(mh_handle h, mh_bs_plan plan, const void *p1, const void *
   if (!x0 || !xout) return fail(h, MH_ERR_ARG, "null vector");
   if (kmax < 1 || kmax > 128) return fail(h, MH_ERR_ARG, "kmax must be in [1, 128]");
   MH_TRY(bs_check_square(h, *plan, p1, p2));
+  MH_TRY(bs_prepare(h, *plan));
   Krylov K(h, *plan, p1, p2, kmax);
   return lanczos_eig_run(h, K, x0, kmax, btol, reorth, eigval, xout, info, niter);
 }

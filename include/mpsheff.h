@@ -209,7 +209,11 @@ enum mh_bs_buffer {
   MH_BS_T2 = 5
 };
 /* sizes[6]: elements of each buffer; every table entry added later is
- * checked against them, so a plan never addresses outside its buffers. */
+ * checked against them, so a plan never addresses outside its buffers.
+ * A plan is built on the host: h may be NULL while building (errors then
+ * carry no message).  Its first apply / Lanczos on a handle uploads the
+ * tables to that handle's device in one allocation (not under stream
+ * capture); the plan then stays on that device. */
 int mh_bs_plan_create(mh_handle h, int prec, const int64_t *sizes, mh_bs_plan *plan);
 /* task_i[5*t] = {M, N, seg0, nseg, ldc}, task_coff[t] = offset of C;
  * seg_off[2*s] = {offset of A, offset of B}, seg_i[3*s] = {K, lda, ldb}.
@@ -236,6 +240,66 @@ int mh_bs_plan_destroy(mh_handle h, mh_bs_plan plan);
 /* run the plan: y = contraction(x; p1, p2) */
 int mh_bs_apply(mh_handle h, mh_bs_plan plan, const void *x, const void *p1, const void *p2,
                 void *y);
+/* ---- native planner: plans straight from the charge sectors -------------
+ * A bond is a list of sectors (charge, dimension) in the caller's storage
+ * order; the planners lay the blocks out as
+ *   MPS-type A[a, s1..sn, b], q(a) + sum q(s) = q(b): blocks (qa, s1..sn) in
+ *     bond order of qa, then the physical indices (s1 slowest); the blocks
+ *     of one left sector are adjacent: one D_qa x ncols column-major matrix;
+ *   environment E[a_bra, w, a_ket], q(a_bra) = q(w) + q(a_ket): blocks
+ *     (q, w) in bond order of q, then channel; one sector's blocks form a
+ *     D_q x K_q column-major matrix.
+ * These are the layouts of the Python planner (blocksparse.py), which lists
+ * the sectors sorted by charge; both produce the same tables.
+ * charges: nq ints per sector (nq = 1..4 U(1) factors, |q| < 32768).
+ * phys: d charges of the physical index (nq ints each).  qwl / qwr: charge
+ * of every MPO channel on the left / right MPO bond (nq ints each); MPO
+ * tensors W[wl, s_out, wr, s_in] column-major FP64.  Errors: MH_ERR_ARG for
+ * an MPO entry that breaks the charge rule or an environment sector that no
+ * block reaches, MH_ERR_SHAPE for a non-positive dimension.               */
+typedef struct {
+  int n;                  /* number of sectors   */
+  int nq;                 /* charge components   */
+  const int32_t *charges; /* [n * nq]            */
+  const int32_t *dims;    /* [n]                 */
+} mh_bs_sectors;
+/* ac2_prime(x, W1, W2, L, R) (algorithms/derivatives.jl:20-22): x on
+ * left | d | d | right; buffers X = x, Y = y, P1 = L (over left, channels
+ * qwl), P2 = R (over right, channels qwr); W1: (wl,d,wm,d), W2: (wm,d,wr,d). */
+int mh_bs_plan_heff2(mh_handle h, int prec, const mh_bs_sectors *left,
+                     const mh_bs_sectors *right, int d, const int32_t *phys, int wl,
+                     const int32_t *qwl, int wm, int wr, const int32_t *qwr, const double *W1,
+                     const double *W2, mh_bs_plan *plan);
+/* updateleft(L, A, W, B) (mpo/transfer.jl:31-34): A, B on left | d | right;
+ * X = ket B, P1 = L (over left, qwl), P2 = bra A, Y = L' (over right, qwr). */
+int mh_bs_plan_updateleft(mh_handle h, int prec, const mh_bs_sectors *left,
+                          const mh_bs_sectors *right, int d, const int32_t *phys, int wl,
+                          const int32_t *qwl, int wr, const int32_t *qwr, const double *W,
+                          mh_bs_plan *plan);
+/* updateright(R, A, W, B) (mpo/transfer.jl:26-29): X = ket B, P1 = R (over
+ * right, qwr), P2 = bra A, Y = R' (over left, qwl). */
+int mh_bs_plan_updateright(mh_handle h, int prec, const mh_bs_sectors *left,
+                           const mh_bs_sectors *right, int d, const int32_t *phys, int wl,
+                           const int32_t *qwl, int wr, const int32_t *qwr, const double *W,
+                           mh_bs_plan *plan);
+/* two-site tensor x[a,s1,s2,b] = A1[a,s1,m] A2[m,s2,b] (dmrg.jl:98), FP64:
+ * X = A1 (left | d | mid), P1 = A2 (mid | d | right), Y = x. */
+int mh_bs_plan_two_site(mh_handle h, const mh_bs_sectors *left, const mh_bs_sectors *mid,
+                        const mh_bs_sectors *right, int d, const int32_t *phys,
+                        mh_bs_plan *plan);
+/* sizes[6] of a plan's buffers (elements), as mh_bs_plan_create took them. */
+int mh_bs_plan_sizes(mh_bs_plan plan, int64_t *sizes);
+/* Inspection of a plan's stages (host tables, no device needed):
+ * info[8] = {0 (gemm), a_src, b_src, c_src, amc, bkc, ntask, nseg} or
+ *           {1 (mode), in_src, out_src, 0, 0, 0, ndst, ncontrib};
+ * the table getters fill arrays in the form of _add_gemm / _add_mode (any
+ * pointer may be NULL). */
+int mh_bs_plan_stage_count(mh_bs_plan plan, int *n);
+int mh_bs_plan_stage_info(mh_bs_plan plan, int stage, int32_t *info);
+int mh_bs_plan_stage_gemm(mh_bs_plan plan, int stage, int32_t *task_i, int64_t *task_coff,
+                          int64_t *seg_off, int32_t *seg_i);
+int mh_bs_plan_stage_mode(mh_bs_plan plan, int stage, int64_t *dst_off, int32_t *dst_i,
+                          int64_t *src_off, int32_t *src_ld, double *coef);
 /* mh_lanczos_eig with a block-sparse H_eff plan (x and y of equal size);
  * the Lanczos vectors are the flat block buffers. */
 int mh_bs_lanczos_eig(mh_handle h, mh_bs_plan plan, const void *p1, const void *p2,

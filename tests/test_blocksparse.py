@@ -264,3 +264,91 @@ def test_prodmps_is_the_basis_state():
     v = psi.to_dense().reshape((4,) * 4)
     assert v[tuple(states)] == 1.0 and np.count_nonzero(v) == 1
     assert psi.total_charge() == tuple(np.array(phys)[states].sum(0))
+
+
+# ------------------------------------------------ native planner (C, host only)
+
+def assert_same_tables(ta, tb):
+    """Identical plans: sizes, stage kinds and buffers, every table entry;
+    the MPO coefficients agree to the last bit."""
+    assert ta.sizes == tb.sizes
+    assert [k for k, _ in ta.stages] == [k for k, _ in tb.stages]
+    for (_, a), (_, b) in zip(ta.stages, tb.stages):
+        assert sorted(a) == sorted(b)
+        for key in a:
+            va, vb = np.asarray(a[key]), np.asarray(b[key])
+            assert va.shape == vb.shape, key
+            assert np.array_equal(va, vb), key
+
+
+def native_cases():
+    """(name, kind, args) over both models, inner and boundary bonds."""
+    rng = np.random.default_rng(21)
+    out = []
+    for model in ("hubbard", "xxz"):
+        L = 6
+        if model == "hubbard":
+            phys = bm.hubbard_charges()
+            mpo = bm.BlockMPO(models.hubbard_mpo(L, 1.0, 4.0), phys)
+            inner = [(a, b) for a in range(1, 4) for b in range(1, 4)]
+            first, last = (0, 0), (3, 3)
+        else:
+            phys, mpo = xxz_case(rng, L)
+            inner = [(q,) for q in range(-3, 4)]
+            first, last = (0,), (0,)
+        for site in (0, 2, L - 2):
+            Sl = bs.Space([(first, 1)]) if site == 0 else rand_space(rng, inner)
+            Sr = bs.Space([(last, 1)]) if site + 2 == L else rand_space(rng, reachable(Sl, phys, 2))
+            xl = bs.BlockLayout(Sl, phys, 2, Sr)
+            Lay, Ray = bs.EnvLayout(Sl, mpo.qw[site]), bs.EnvLayout(Sr, mpo.qw[site + 2])
+            out.append((f"{model}-heff2-{site}", "heff2",
+                        (xl, Lay, Ray, mpo[site], mpo[site + 1], "f64")))
+        Sl = rand_space(rng, inner)
+        Sm = rand_space(rng, reachable(Sl, phys, 1))
+        Sr = rand_space(rng, reachable(Sm, phys, 1))
+        al = bs.BlockLayout(Sl, phys, 1, Sm)
+        out.append((f"{model}-updateleft", "updateleft",
+                    (al, bs.EnvLayout(Sl, mpo.qw[2]), mpo[2], bs.EnvLayout(Sm, mpo.qw[3]), "f32")))
+        out.append((f"{model}-updateright", "updateright",
+                    (al, bs.EnvLayout(Sm, mpo.qw[3]), mpo[2], bs.EnvLayout(Sl, mpo.qw[2]), "bf16x3")))
+        l2 = bs.BlockLayout(Sm, phys, 1, Sr)
+        out.append((f"{model}-two_site", "two_site", (al, l2, bs.BlockLayout(Sl, phys, 2, Sr))))
+    return out
+
+
+NATIVE_CASES = native_cases()
+
+
+@pytest.mark.parametrize("case", NATIVE_CASES, ids=[c[0] for c in NATIVE_CASES])
+def test_native_planner_builds_the_python_tables(case):
+    """The C planner (mh_bs_plan_heff2 / updateleft / updateright / two_site),
+    run without a device, lays out the same buffers and emits the same stage
+    tables as the Python planner, whose tables the dense oracle pins above."""
+    _, kind, args = case
+    assert_same_tables(bs.host_native_tables(kind, *args), bs._PY_PLANNERS[kind](*args))
+
+
+def test_native_planner_result_matches_dense_oracle():
+    """Native tables run through the stage executor give ac2_prime."""
+    rng = np.random.default_rng(4)
+    _, kind, args = NATIVE_CASES[1]
+    xl, Lay, Ray, W1, W2, _ = args
+    x, Le, Re = rand_block(rng, xl), rand_env(rng, Lay), rand_env(rng, Ray)
+    tab = bs.host_native_tables(kind, *args)
+    out = execute(tab, [x.data.numpy(), np.full(xl.size, np.nan), Le.data.numpy(),
+                        Re.data.numpy(), None, None])
+    check_block_result(xl, out[1], OC.ac2_prime(x.to_dense(), W1, W2, Le.to_dense(),
+                                                Re.to_dense()))
+
+
+def test_native_planner_rejects_charge_breaking_mpo():
+    """An MPO entry that changes the total charge: both planners refuse it."""
+    from finitemptools_amd import _lib
+    _, kind, args = NATIVE_CASES[1]
+    xl, Lay, Ray, W1, W2, prec = args
+    bad = W1.copy()
+    bad[0, 0, 0, 3] = 0.5    # |up,dn> -> |empty> with the identity channel
+    with pytest.raises(_lib.MHError):
+        bs.host_native_tables(kind, xl, Lay, Ray, bad, W2, prec)
+    with pytest.raises(ValueError):
+        bs.plan_heff2(xl, Lay, Ray, bad, W2)

@@ -212,3 +212,66 @@ def test_block_dmrg2_xxz_u1(gpu):
     psi = bm.prodmps(phys, [up, dn] * 6)
     energies, delta, _, _ = run_block_dmrg(models.heisenberg_xxz_mpo(12), phys, psi, 64, 6)
     assert abs(energies[-1] - ED["xxz_L12_J1_D1"]) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16x3"])
+def test_native_plan_equals_python_plan_on_device(gpu, prec):
+    """A plan from the C planner and one uploaded from the Python tables give
+    bitwise the same H_eff x (same tables, same tile variant), and match the
+    numpy execution of the tables."""
+    rng = np.random.default_rng(8)
+    phys = bm.hubbard_charges()
+    mpo = bm.BlockMPO(models.hubbard_mpo(8, 1.0, 4.0), phys)
+    Sl = rand_space(rng, [(a, b) for a in range(1, 5) for b in range(1, 5)], 3, 70)
+    Sr = rand_space(rng, reachable(Sl, phys, 2), 3, 70)
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    Lay, Ray = bs.EnvLayout(Sl, mpo.qw[3]), bs.EnvLayout(Sr, mpo.qw[5])
+    args = (xl, Lay, Ray, mpo[3], mpo[4], prec)
+    x = rng.standard_normal(xl.size)
+    Le = rng.standard_normal(Lay.size) / 20
+    Re = rng.standard_normal(Ray.size) / 20
+    ys = []
+    for planner in ("native", "python"):
+        plan = bs.make_plan("heff2", *args, planner=planner)
+        plan.force_variant(0)
+        y = torch.full((xl.size,), float("nan"), dtype=torch.float64, device="cuda")
+        plan.apply(dev(x), dev(Le), dev(Re), y)
+        ys.append(y.cpu().numpy())
+    assert np.array_equal(ys[0], ys[1])
+    ref = execute(bs.plan_heff2(*args[:5]), [x, np.zeros(xl.size), Le, Re, None, None])[1]
+    assert rel(ys[0], ref) < {"f64": 1e-12, "f32": 1e-5, "bf16x3": 1e-4}[prec]
+
+
+def test_native_env_and_two_site_plans_on_device(gpu):
+    """updateleft / updateright / two-site plans from the C planner against
+    the dense oracle."""
+    rng = np.random.default_rng(9)
+    phys = bm.hubbard_charges()
+    mpo = bm.BlockMPO(models.hubbard_mpo(8, 1.0, 4.0), phys)
+    Sl = rand_space(rng, [(a, b) for a in range(1, 5) for b in range(1, 5)], 2, 30)
+    Sm = rand_space(rng, reachable(Sl, phys, 1), 2, 30)
+    Sr = rand_space(rng, reachable(Sm, phys, 1), 2, 30)
+    al, l2 = bs.BlockLayout(Sl, phys, 1, Sm), bs.BlockLayout(Sm, phys, 1, Sr)
+    A = bs.BlockTensor(al, dev(rng.standard_normal(al.size)))
+    B = bs.BlockTensor(al, dev(rng.standard_normal(al.size)))
+    A2 = bs.BlockTensor(l2, dev(rng.standard_normal(l2.size)))
+    W = mpo[3]
+    Lay, Lnew = bs.EnvLayout(Sl, mpo.qw[3]), bs.EnvLayout(Sm, mpo.qw[4])
+    Le = bs.BlockEnv(Lay, dev(rng.standard_normal(Lay.size)))
+    y = torch.zeros(Lnew.size, dtype=torch.float64, device="cuda")
+    bs.make_plan("updateleft", al, Lay, W, Lnew, "f64", planner="native").apply(
+        B.data, Le.data, A.data, y)
+    ref = OC.updateleft(Le.to_dense(), A.to_dense(), W, B.to_dense())
+    assert rel(bs.BlockEnv(Lnew, y.cpu()).to_dense(), ref) < 1e-12
+    Ray, Rnew = bs.EnvLayout(Sm, mpo.qw[4]), bs.EnvLayout(Sl, mpo.qw[3])
+    Re = bs.BlockEnv(Ray, dev(rng.standard_normal(Ray.size)))
+    y = torch.zeros(Rnew.size, dtype=torch.float64, device="cuda")
+    bs.make_plan("updateright", al, Ray, W, Rnew, "f64", planner="native").apply(
+        B.data, Re.data, A.data, y)
+    ref = OC.updateright(Re.to_dense(), A.to_dense(), W, B.to_dense())
+    assert rel(bs.BlockEnv(Rnew, y.cpu()).to_dense(), ref) < 1e-12
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    y = torch.zeros(xl.size, dtype=torch.float64, device="cuda")
+    bs.make_plan("two_site", al, l2, xl, planner="native").apply(A.data, A2.data, None, y)
+    ref = np.einsum("asm,mtb->astb", A.to_dense(), A2.to_dense())
+    assert rel(bs.BlockTensor(xl, y.cpu()).to_dense(), ref) < 1e-12
