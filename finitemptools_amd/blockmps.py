@@ -506,6 +506,70 @@ def split_two_site(tmap, vec_host, trunc, absorb, device):
     return A1, A2, svals
 
 
+_EIG_POOL = {}
+
+
+def _eig_workers(dev, n):
+    """n side streams (default priority) and a thread pool for the batched
+    eigensolves of one device, created once."""
+    import concurrent.futures
+    key = (str(dev), n)
+    if key not in _EIG_POOL:
+        _EIG_POOL[key] = (concurrent.futures.ThreadPoolExecutor(max_workers=n),
+                          [torch.cuda.Stream(device=dev) for _ in range(n)])
+    return _EIG_POOL[key]
+
+
+EIG_THREADS = 4
+
+
+def _batched_eigh(grams, dev):
+    """Eigen-decompositions of the sector Gram matrices: one batched eigh per
+    size class (zero padding).  A batched call costs a latency that grows
+    with n plus a little per matrix, so the classes are geometric (a new
+    class starts below 0.5 x the largest of the current one) and, on a GPU,
+    run concurrently on side streams from a small thread pool (the solver is
+    launch-latency bound).  Returns [(nb, charges, lam, V)]."""
+    classes = {}
+    cur = None
+    for c in sorted(grams, key=lambda c: -grams[c].shape[0]):
+        n = grams[c].shape[0]
+        if cur is None or n < 0.5 * cur:
+            cur = -(-n // 32) * 32
+        classes.setdefault(cur, []).append(c)
+
+    def solve(nb, cs):
+        Gb = torch.zeros((len(cs), nb, nb), dtype=torch.float64, device=dev)
+        for j, c in enumerate(cs):
+            n = grams[c].shape[0]
+            Gb[j, :n, :n] = grams[c]
+        lam, V = torch.linalg.eigh(Gb)
+        return (nb, cs, lam, V)
+
+    items = list(classes.items())
+    if dev.type != "cuda" or len(items) < 2 or EIG_THREADS < 2:
+        return [solve(nb, cs) for nb, cs in items]
+    pool, streams = _eig_workers(dev, EIG_THREADS)
+    main = torch.cuda.current_stream(dev)
+    ready = torch.cuda.Event()
+    ready.record(main)
+
+    def task(k):
+        s = streams[k % len(streams)]
+        with torch.cuda.stream(s):
+            s.wait_event(ready)
+            out = solve(*items[k])
+            done = torch.cuda.Event()
+            done.record(s)
+        return out, done
+
+    # classes on one stream run in order; the largest go first
+    results = list(pool.map(task, range(len(items))))
+    for _, done in results:
+        main.wait_event(done)
+    return [r for r, _ in results]
+
+
 def split_two_site_rho(tmap, vec, trunc, absorb):
     """Device split by the sector reduced density matrices (White's original
     DMRG step): for the left sweep rho_c = M_c M_c^T gives u and s^2, and
@@ -524,27 +588,9 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
         M = cat[tmap.base[c]:tmap.base[c] + R * C].view(R, C)
         mats[c] = M
         grams[c] = M @ M.t() if left else M.t() @ M
-    # Batched eigh per size class with zero padding.  A batched call costs a
-    # fixed latency plus roughly n^3 per matrix, so the classes are
-    # geometric: a new class starts when a matrix is below 0.55 x the
-    # largest of the current one.
-    classes = {}
-    cur = None
-    for c in sorted(grams, key=lambda c: -grams[c].shape[0]):
-        n = grams[c].shape[0]
-        if cur is None or n < 0.55 * cur:
-            cur = -(-n // 32) * 32
-        classes.setdefault(cur, []).append(c)
+    pending = _batched_eigh(grams, dev)
     eig = {}
     lam_host = {}
-    pending = []
-    for nb, cs in classes.items():
-        Gb = torch.zeros((len(cs), nb, nb), dtype=torch.float64, device=dev)
-        for j, c in enumerate(cs):
-            n = grams[c].shape[0]
-            Gb[j, :n, :n] = grams[c]
-        lam, V = torch.linalg.eigh(Gb)
-        pending.append((nb, cs, lam, V))
     lam_all = torch.cat([p[2].reshape(-1) for p in pending]).cpu().numpy() if pending else None
     pos = 0
     for nb, cs, lam, V in pending:
