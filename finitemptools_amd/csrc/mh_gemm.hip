@@ -425,6 +425,243 @@ static bool cand_ok_f64(const Cand &c, const GemmP &p, bool cplx) {
 
 static const GemmFamily kF64Family = {"f64", kCands, kNCand, 16, cand_ok_f64, launch_f64};
 
+// ------------------------------------------------ fused first H_eff step
+//
+// One- and two-site H_eff, FP64 real:
+//   T1[a, w, s, b'] = sum_a' L[a, w, a'] x[a', s, b']           (MFMA)
+//   T2[a, j, b']    = sum_i T1[a, i, b'] Mt[i, j],  i = w + WL*s (epilogue)
+// (s: the physical index of x, DS values; j = s' + DS*w', NJ = DS*WR.)
+// A block owns a = a0..a0+31 for EVERY channel w (BM = 32*WL logical rows:
+// row r is a = a0 + r%32 of channel r/32) and 64/DS values of b' for every s
+// (BN = 64 columns c = s + DS*b', contiguous in x), so the whole T1[a, :, b']
+// of each of its (a, b') pairs is inside the block and T1 never goes to HBM.
+// The K loop (a') is k_gemm's (double-buffered LDS, one barrier per step,
+// v_mfma_f64_16x16x4f64).  Epilogue in two halves of 32 columns: the waves
+// owning a half write it to LDS ([column][row], pitch BM + 1), then every
+// thread forms T2 for one or more (a, b') pairs from NI values; Mt entries
+// are wave-uniform (scalar loads).  Needs Dl even and 16-byte aligned L, x.
+template <int WL, int DS, int WR>
+__global__ void __launch_bounds__(256)
+    k_heff_step1(const double *__restrict__ L, const double *__restrict__ X,
+                 const double *__restrict__ Mt, double *__restrict__ T2, int Dl, int Dr,
+                 int tiles_a, int tiles_b, int xcd_chunk) {
+  constexpr int BA = 32, BM = BA * WL, BN = 64, BK = 16;
+  constexpr int NI = WL * DS, NJ = DS * WR;
+  constexpr int WTM = BM / 2, WTN = BN / 2, TM = WTM / 16, TN = WTN / 16;
+  constexpr int HALF = BN / 2, BBH = HALF / DS;  // columns / b' values per half
+  constexpr int NPAIRS = BA * BBH;               // (a, b') pairs per half
+  static_assert(BN % DS == 0 && HALF % DS == 0 && NPAIRS % 256 == 0, "tile");
+  using SA = Stage<false, BM, BK, true>;   // L rows, staged [k][r]
+  using SB = Stage<false, BN, BK, false>;  // x columns, staged [c][k]
+  constexpr int CP = BM + 1;
+  constexpr int NSTAGE = 2 * SA::SIZE + 2 * SB::SIZE;
+  constexpr int NEPI = HALF * CP;
+  __shared__ __align__(16) double smem[NSTAGE > NEPI ? NSTAGE : NEPI];
+  double *const sA = smem;
+  double *const sB = smem + 2 * SA::SIZE;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  // XCD-contiguous tile order: block id -> XCD id % 8 -> a contiguous range
+  // of tiles (a fastest), so one XCD's blocks share x columns in its L2.
+  const int local = blockIdx.x >> 3;
+  const int t = (blockIdx.x & 7) * xcd_chunk + local;
+  if (local >= xcd_chunk || t >= tiles_a * tiles_b) return;
+  const int a0 = (t % tiles_a) * BA, c0 = (t / tiles_a) * BN;
+  const int K = Dl, NC = DS * Dr;
+  const int64_t ldL = (int64_t)Dl * WL;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  int64_t aoff[SA::NPAIR], boff[SB::NPAIR];
+  bool aok[SA::NPAIR][2], bok[SB::NPAIR];
+#pragma unroll
+  for (int i = 0; i < SA::NPAIR; ++i) {
+    int r, k;
+    SA::coord(tid, i, r, k);
+    const int a = a0 + r % BA, w = r / BA;
+    aok[i][0] = a < Dl;
+    aok[i][1] = a + 1 < Dl;
+    // a is even and Dl is even: the clamped pair stays inside channel w
+    aoff[i] = (int64_t)min(a, Dl - 2) + (int64_t)Dl * w + (int64_t)k * ldL;
+  }
+#pragma unroll
+  for (int i = 0; i < SB::NPAIR; ++i) {
+    int r, k;
+    SB::coord(tid, i, r, k);
+    const int c = c0 + r;
+    bok[i] = c < NC;
+    boff[i] = (int64_t)min(c, NC - 1) * Dl + k;
+  }
+
+  double ra[SA::NPAIR][2], rb[SB::NPAIR][2];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      const int kc = min(k0 + k, K - 1);
+      const double2 v = *reinterpret_cast<const double2 *>(L + aoff[i] + (int64_t)(kc - k) * ldL);
+      ra[i][0] = v.x;
+      ra[i][1] = v.y;
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      const int kc = min(k0 + k, K - 2);
+      const double2 v = *reinterpret_cast<const double2 *>(X + boff[i] + (kc - k));
+      rb[i][0] = v.x;
+      rb[i][1] = v.y;
+    }
+  };
+  auto store = [&](int buf, int k0) {
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      const bool kok = k0 + k < K;
+      double2 v;
+      v.x = (aok[i][0] && kok) ? ra[i][0] : 0.0;
+      v.y = (aok[i][1] && kok) ? ra[i][1] : 0.0;
+      *reinterpret_cast<double2 *>(&sA[buf * SA::SIZE + k * SA::PITCH + r]) = v;
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      double2 v;
+      v.x = (bok[i] && k0 + k < K) ? rb[i][0] : 0.0;
+      v.y = (bok[i] && k0 + k + 1 < K) ? rb[i][1] : 0.0;
+      *reinterpret_cast<double2 *>(&sB[buf * SB::SIZE + r * SB::PITCH + k]) = v;
+    }
+  };
+
+  d4_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  auto compute = [&](int buf) {
+    const double *A = sA + buf * SA::SIZE;
+    const double *B = sB + buf * SB::SIZE;
+#pragma unroll
+    for (int kb = 0; kb < BK; kb += 16) {
+      double a[TM][4], b[TN][4];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int r = wm * WTM + i * 16 + lr;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) a[i][s] = A[(kb + 4 * lk + s) * SA::PITCH + r];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int r = wn * WTN + j * 16 + lr;
+        const double2 *src = reinterpret_cast<const double2 *>(&B[r * SB::PITCH + kb + 4 * lk]);
+        const double2 u = src[0], v = src[1];
+        b[j][0] = u.x; b[j][1] = u.y; b[j][2] = v.x; b[j][3] = v.y;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  load(0);
+  store(0, 0);
+  __syncthreads();
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int k1 = (kt + 1) * BK;
+    load(k1);
+    compute(kt & 1);
+    store((kt + 1) & 1, k1);
+    __syncthreads();
+  }
+  compute((nk - 1) & 1);
+
+  // Epilogue: half h holds the columns of the waves with wn == h.
+  double *const Cs = smem;
+  const int bq0 = c0 / DS;  // first b' of the block
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();  // LDS free: the K loop (h = 0) or the previous half is done
+    if (wn == h) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int m = wm * WTM + i * 16 + lk + 4 * r;
+            const int n = j * 16 + lr;  // column inside the half
+            Cs[n * CP + m] = acc[i][j][r];
+          }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NPAIRS / 256; ++q) {
+      const int pr = tid + 256 * q;
+      const int al = pr % BA, bl = pr / BA;
+      const int a = a0 + al, bq = bq0 + h * BBH + bl;
+      double v[NI];
+#pragma unroll
+      for (int s = 0; s < DS; ++s)
+#pragma unroll
+        for (int w = 0; w < WL; ++w) v[w + WL * s] = Cs[(bl * DS + s) * CP + al + BA * w];
+      if (a < Dl && bq < Dr) {
+        double *dst = T2 + a + (int64_t)Dl * NJ * bq;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          double o = 0.0;
+#pragma unroll
+          for (int i = 0; i < NI; ++i) o = __builtin_fma(v[i], Mt[i + NI * j], o);
+          dst[(int64_t)Dl * j] = o;
+        }
+      }
+    }
+  }
+}
+
+template <int WL, int DS>
+static void launch_step1(int Dl, int Dr, const double *L, const double *X, const double *Mt,
+                         double *T2, hipStream_t s) {
+  const int tiles_a = (Dl + 31) / 32, tiles_b = (DS * Dr + 63) / 64;
+  const int chunk = (tiles_a * tiles_b + 7) / 8;
+  hipLaunchKernelGGL((k_heff_step1<WL, DS, WL>), dim3(8 * chunk), dim3(256), 0, s, L, X, Mt, T2,
+                     Dl, Dr, tiles_a, tiles_b, chunk);
+}
+
+// T2 = mode(L x) in one launch for FP64 real operators with wl = wr in
+// {3,4,5,6,8} and a physical extent of 2 or 4 (one-site d = 2, 4; two-site
+// d = 2); MH_ERR_UNSUPPORTED otherwise (the caller runs GEMM + mode product).
+int heff_step1_fused(mh_handle h, int Dl, int Dr, int wl, int ds, int wr, const void *L,
+                     const void *x, const void *Mt, void *T2) {
+  auto al16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+  if (wl != wr || Dl < 2 || Dl % 2 != 0 || !al16(L) || !al16(x)) return MH_ERR_UNSUPPORTED;
+  const double *Lp = (const double *)L, *Xp = (const double *)x, *Mp = (const double *)Mt;
+  double *Tp = (double *)T2;
+  hipStream_t s = h->stream;
+  // profiled as a GEMM launch with the GEMM's 2 M N K flops
+  const double flops = 2.0 * Dl * (double)wl * ds * Dr * Dl;
+  int slot;
+#define MH_S1(W, D)                                   \
+  if (wl == W && ds == D) {                           \
+    prof_begin(h, MH_K_GEMM, flops, &slot);           \
+    launch_step1<W, D>(Dl, Dr, Lp, Xp, Mp, Tp, s);    \
+    prof_end(h, slot);                                \
+    return check_launch(h, "fused H_eff step 1");     \
+  }
+  MH_S1(5, 4) MH_S1(5, 2) MH_S1(3, 4) MH_S1(3, 2) MH_S1(4, 4) MH_S1(4, 2)
+  MH_S1(6, 4) MH_S1(6, 2) MH_S1(8, 4) MH_S1(8, 2)
+#undef MH_S1
+  return MH_ERR_UNSUPPORTED;
+}
+
 // Register-only FP64 MFMA throughput probe: four independent accumulator
 // chains per wave, no memory traffic in the loop.  Measures the matrix-core
 // rate this device actually sustains.

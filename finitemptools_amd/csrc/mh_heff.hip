@@ -347,13 +347,21 @@ int heff_apply_ws(mh_handle h, const mh_heff_op &op, const void *x, void *y,
     MH_TRY(build_mode_matrix(h, op.dtype, op.kind == MH_HEFF2 ? MODE_TWO : MODE_SITE,
                              wl, op.wm, wr, d1, op.d2, op.W1, op.W2, Mt));
   }
-  // T1[(a,w), (s,b')] = L[(a,w), a'] x[a', (s,b')]
-  GemmDesc g1 = gd(Dl * wl, ds * Dr, Dl, op.L, 1, (int64_t)Dl * wl, 0, x, 1, Dl, T1, 1,
-              (int64_t)Dl * wl);
-  MH_TRY(gemm(h, op.dtype, op.prec, g1));
-  // T2[a, (s',w'), b'] = sum_{(w,s)} T1[a, (w,s), b'] Mt[(w,s), (s',w')]
-  MH_TRY(mode_product(h, op.dtype, Dl, ni, nj, Dr, T1, (int64_t)Dl * ni, Mt, T2,
-                      (int64_t)Dl * nj));
+  int fused = MH_ERR_UNSUPPORTED;
+  if (h->fusion && op.dtype == MH_F64) {
+    // T2 = mode(L x) without T1 in HBM
+    fused = heff_step1_fused(h, Dl, Dr, wl, ds, wr, op.L, x, Mt, T2);
+    if (fused != MH_OK && fused != MH_ERR_UNSUPPORTED) return fused;
+  }
+  if (fused != MH_OK) {
+    // T1[(a,w), (s,b')] = L[(a,w), a'] x[a', (s,b')]
+    GemmDesc g1 = gd(Dl * wl, ds * Dr, Dl, op.L, 1, (int64_t)Dl * wl, 0, x, 1, Dl, T1, 1,
+                (int64_t)Dl * wl);
+    MH_TRY(gemm(h, op.dtype, op.prec, g1));
+    // T2[a, (s',w'), b'] = sum_{(w,s)} T1[a, (w,s), b'] Mt[(w,s), (s',w')]
+    MH_TRY(mode_product(h, op.dtype, Dl, ni, nj, Dr, T1, (int64_t)Dl * ni, Mt, T2,
+                        (int64_t)Dl * nj));
+  }
   // y[(a,s'), b] = sum_{(w',b')} T2[(a,s'), (w',b')] R[b, (w',b')]
   GemmDesc g3 = gd(Dl * ds, Dr, wr * Dr, T2, 1, (int64_t)Dl * ds, 0, op.R, Dr, 1, y, 1,
               (int64_t)Dl * ds);

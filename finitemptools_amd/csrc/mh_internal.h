@@ -52,6 +52,8 @@ struct mh_context {
   // Krylov recurrences as hipGraphs: captured on a private stream (a
   // non-blocking stream at the default priority), launched on `stream`.
   int graphs = 1;
+  // Fused first H_eff step (GEMM with the mode product in its epilogue).
+  int fusion = 1;
   hipStream_t cap_stream = nullptr;
   int autotune = 1;
   int tune_log = 0;
@@ -114,6 +116,10 @@ enum ModeKind {
 };
 int build_mode_matrix(mh_handle h, int dtype, int kind, int wl, int wm, int wr,
                       int d1, int d2, const void *W1, const void *W2, void *Mt);
+// T2 = mode(L x) in one launch (FP64 real, supported (wl, ds) only);
+// MH_ERR_UNSUPPORTED when the shape has no fused kernel.
+int heff_step1_fused(mh_handle h, int Dl, int Dr, int wl, int ds, int wr, const void *L,
+                     const void *x, const void *Mt, void *T2);
 int mode_product(mh_handle h, int dtype, int P, int ni, int nj, int Q,
                  const void *in, int64_t sIq, const void *Mt, void *out,
                  int64_t sOq);

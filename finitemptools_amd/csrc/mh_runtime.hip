@@ -134,6 +134,8 @@ int mh_create(int device, mh_handle *out) {
   if (const char *s = getenv("MH_TUNE_LOG")) h->tune_log = atoi(s) != 0;
   // MH_GRAPHS=0: run the Krylov recurrences eagerly instead of as a hipGraph.
   if (const char *s = getenv("MH_GRAPHS")) h->graphs = atoi(s) != 0;
+  // MH_FUSION=0: run the first H_eff step as GEMM + separate mode product.
+  if (const char *s = getenv("MH_FUSION")) h->fusion = atoi(s) != 0;
   *out = h;
   return MH_OK;
 }
@@ -160,6 +162,12 @@ int mh_destroy(mh_handle h) {
 int mh_set_graphs(mh_handle h, int enable) {
   if (!h) return MH_ERR_ARG;
   h->graphs = enable ? 1 : 0;
+  return MH_OK;
+}
+
+int mh_set_fusion(mh_handle h, int enable) {
+  if (!h) return MH_ERR_ARG;
+  h->fusion = enable ? 1 : 0;
   return MH_OK;
 }
 

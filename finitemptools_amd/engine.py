@@ -114,6 +114,10 @@ class Engine:
         """Krylov recurrences as hipGraphs (default) or eager launches."""
         self.check(self.lib.mh_set_graphs(self.h, 1 if on else 0))
 
+    def set_fusion(self, on=True):
+        """Fused first H_eff step (default) or GEMM + separate mode product."""
+        self.check(self.lib.mh_set_fusion(self.h, 1 if on else 0))
+
     def profile_read(self, kclass):
         ms = ctypes.c_double()
         n = ctypes.c_int64()

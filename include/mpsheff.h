@@ -150,6 +150,10 @@ int mh_lanczos_expm(mh_handle h, const mh_heff_op *op, const void *x0,
  * (exponentiate re-launches it for every sub-step).  Results are identical
  * to the eager launches. */
 int mh_set_graphs(mh_handle h, int enable);
+/* enable = 0: run the first step of the FP64 one-/two-site H_eff as a GEMM
+ * followed by a separate mode-product launch instead of the fused kernel
+ * (default on; environment MH_FUSION=0). */
+int mh_set_fusion(mh_handle h, int enable);
 
 /* ---- Krylov vector primitives (simple_lanczos.jl:28-38,50-63) ------------ */
 /* out[0] + i out[1] = <x, y>  (conjugate-linear in x, as Julia dot) */

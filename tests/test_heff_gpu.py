@@ -285,3 +285,45 @@ def test_benchmark_size_parity_and_properties(gpu, which):
     lhs = np.vdot(z, host(hx))
     rhs = np.vdot(host(hz), x)
     assert abs(lhs - rhs) < 1e-11 * np.linalg.norm(host(hx)) * np.linalg.norm(z)
+
+
+FUSED_CASES = [  # (kind, Wk or (w, d), Dl, Dr): shapes the fused first step takes
+    (2, "xxz", 2, 2), (2, "xxz", 34, 50), (2, "xxz", 96, 130), (2, "tfim", 62, 18),
+    (2, (4, 2), 40, 66), (2, (8, 2), 36, 20), (1, "xxz", 130, 70), (1, "hubbard", 50, 64),
+    (1, (8, 4), 32, 33), (1, (3, 2), 2, 7),
+]
+
+
+@pytest.mark.parametrize("kind,Wk,Dl,Dr", FUSED_CASES)
+def test_fused_first_step_matches_unfused_and_oracle(gpu, kind, Wk, Dl, Dr):
+    """The fused GEMM + mode-product kernel (FP64 real, wl = wr in {3,4,5,6,8},
+    physical extent 2 or 4, even Dl) against the oracle and against the
+    two-launch path; ragged Dl, Dr that are not multiples of the 32 x 64/ds
+    block tile."""
+    from finitemptools_amd.derivatives import ac2_prime, ac_prime
+    from finitemptools_amd.engine import engine
+    rng = np.random.default_rng(Dl * 31 + Dr)
+    if isinstance(Wk, tuple):
+        w, d = Wk
+        W = rand(rng, (w, d, w, d))
+    else:
+        W = mpo_tensor(Wk)
+        w, d = W.shape[0], W.shape[1]
+    L, R = _envs(rng, Dl, w, Dr, w, False)
+    eng = engine()
+    ys = []
+    for fused in (True, False):
+        eng.set_fusion(fused)
+        try:
+            if kind == 2:
+                x = rand(np.random.default_rng(5), (Dl, d, d, Dr))
+                ys.append(host(ac2_prime(dev(x), dev(W), dev(W), dev(L), dev(R))))
+            else:
+                x = rand(np.random.default_rng(5), (Dl, d, Dr))
+                ys.append(host(ac_prime(dev(x), dev(W), dev(L), dev(R))))
+        finally:
+            eng.set_fusion(True)
+    ref = C.ac2_prime(x, W, W, L, R) if kind == 2 else C.ac_prime(x, W, L, R)
+    assert relerr(ys[0], ref) < TOL
+    assert relerr(ys[1], ref) < TOL
+    assert relerr(ys[0], ys[1]) < 1e-13
