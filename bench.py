@@ -9,6 +9,12 @@ value is FP64 TFLOP/s with the algorithmic flop count
 F2 = 4 D^3 w d^2 + 4 D^2 w^2 d^3 per matvec.  The one-site matvec at
 D = 1024 (F1 = 4 D^3 w d + 2 D^2 w^2 d^2) is reported beside it.
 
+Beside the headline the same JSON line carries: the block-sparse Hubbard
+DMRG2 (L = 64, D ramp to 2048; matvec per precision, one sweep per
+precision), J1-J2 gradients (L = 40, D = 256), the TFIM TDVP2 quench
+(L = 100, 40 steps, vs the free-fermion closed form and FP64), the XXZ
+L = 20 D = 64 DMRG2 against exact diagonalisation, and the CPU baseline.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -288,6 +294,47 @@ def gradients_converged_j1j2(L=40, D=256, J2=0.25, nsweeps=5, h=1e-5):
             "dE_dA_norm": float(sum(torch.sum(a * a).item() for a in res.dA) ** 0.5)}
 
 
+def tdvp_tfim_quench(L=100, nsteps=40, dt=0.05, D=512, precs=("f64", "f32", "bf16x3")):
+    """TDVP2 quench of the transverse-field Ising chain (J = 1, h = 0 -> 1)
+    from |+>^L, ComplexF64 storage, truncation D <= 512 at eps = 1e-8, Lanczos
+    exponentiate (dimension <= 12, FP64 re-orthogonalisation in the mixed
+    forms): <sz_i(t)> at every step against the free-fermion closed form and,
+    for the mixed precisions, against the FP64 trajectory."""
+    import torch
+    from finitemptools_amd import models, tdvp
+    from finitemptools_amd.envs import environments, local_expectations, mpo_to_device
+    from finitemptools_amd.mps import prodmps
+    from finitemptools_amd.truncation import MPSTruncation
+    from oracle import ed
+    from oracle.freefermion import tfim_quench_sz
+    mpo = mpo_to_device(models.tfim_mpo(L, 1.0, 1.0, dtype=np.complex128), torch.complex128)
+    exact = tfim_quench_sz(L, 1.0, 1.0, [dt * (k + 1) for k in range(nsteps)])
+    res, traj = {"workload": f"TFIM L={L} J=1 quench h=0->1 from |+>^L, TDVP2 dt={dt}, {nsteps} steps, "
+                             f"C64, D<={D} eps=1e-8"}, {}
+    for prec in precs:
+        psi = prodmps([np.ones(2) / np.sqrt(2)] * L, dtype=torch.complex128)
+        alg = tdvp.TDVP2(stepsize=-1j * dt, trunc=MPSTruncation(D=D, eps=1e-8), prec=prec)
+        env = environments(mpo, psi, prec)
+        rows, sec = [], 0.0
+        for _ in range(nsteps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tdvp.sweep(env, alg)
+            torch.cuda.synchronize()
+            sec += time.perf_counter() - t0
+            rows.append(local_expectations(psi, ed.SZ))
+        traj[prec] = np.array(rows)
+        ent = {"s": round(sec, 2), "s_per_step": round(sec / nsteps, 3),
+               "D_max": max(t.shape[2] for t in psi.data),
+               "max_abs_err_vs_closed_form": float(np.abs(traj[prec] - exact).max())}
+        if prec != "f64" and "f64" in traj:
+            ent["max_abs_diff_vs_f64"] = float(np.abs(traj[prec] - traj["f64"]).max())
+            ent["speedup_vs_f64"] = round(res["f64"]["s"] / sec, 3)
+        res[prec] = ent
+        print(f"  tdvp {prec}: {sec:.1f} s", file=sys.stderr, flush=True)
+    return res
+
+
 def gradients_j1j2(L=40, D=256, J2=0.25):
     """dE/dJ2 and dE/dA of E = <psi|H|psi>/<psi|psi> for the J1-J2 chain on a
     random D = 256 MPS (the cost does not depend on convergence), with stored
@@ -328,6 +375,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-blocksparse", action="store_true")
     ap.add_argument("--no-gradients", action="store_true")
+    ap.add_argument("--no-tdvp", action="store_true")
     ap.add_argument("--only", choices=("all", "blocksparse"), default="all",
                     help="blocksparse: run only the block-sparse section (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -377,7 +425,7 @@ def main():
     eng.check(eng.lib.mh_probe_mfma_f64(eng.bind_stream(), ctypes.byref(peak_meas)))
     gms, gn, gfl = r2["prof"]["gemm"]
     avg_ms = gms / max(gn, 1)
-    per_launch = gfl / max(gn, 1)  # 2*M*N*K = 2 D^3 w d^2 for both GEMMs here
+    per_launch = gfl / max(gn, 1)  # 2*M*N*K = 2 D^3 w d^2 for both launches here
     achieved = per_launch / (avg_ms * 1e-3) / 1e12
     wms, wn, _ = r2["prof"]["wstage"]
     out = {
@@ -399,7 +447,9 @@ def main():
                                "rel_err_vs_oracle": err2},
         "ac_prime_D1024_f64": {"tflops": round(r1["tflops"], 3), "ms": round(r1["sec_per_matvec"] * 1e3, 4)},
         "ac2_prime_D512_mixed": mixed,
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (both GEMM stages of ac2_prime)",
+        "roofline": {"bound": "mfma",
+                     "kernel": "the two launches of ac2_prime: k_heff_step1<5,4,5> (GEMM 1 with the "
+                               "mode product in its epilogue) and k_gemm (GEMM 3)",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": pmc_traffic(),
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
@@ -417,6 +467,9 @@ def main():
         print("gradients ...", file=sys.stderr, flush=True)
         out["gradients_j1j2_L40_D256"] = gradients_j1j2()
         out["gradients_j1j2_L40_D256_converged"] = gradients_converged_j1j2()
+    if not args.no_tdvp:
+        print("tdvp TFIM L=100 ...", file=sys.stderr, flush=True)
+        out["tdvp_tfim_L100_quench"] = tdvp_tfim_quench()
     print("xxz L=20 D=64 ...", file=sys.stderr, flush=True)
     out["dmrg_xxz_L20_D64"] = xxz_dmrg_L20_D64(cpu=not args.no_cpu)
     if not args.no_cpu:

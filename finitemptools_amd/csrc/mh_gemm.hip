@@ -444,7 +444,7 @@ template <int WL, int DS, int WR>
 __global__ void __launch_bounds__(256)
     k_heff_step1(const double *__restrict__ L, const double *__restrict__ X,
                  const double *__restrict__ Mt, double *__restrict__ T2, int Dl, int Dr,
-                 int tiles_a, int tiles_b, int xcd_chunk) {
+                 int tiles_a, int tiles_b, int xcd_chunk, int group) {
   constexpr int BA = 32, BM = BA * WL, BN = 64, BK = 16;
   constexpr int NI = WL * DS, NJ = DS * WR;
   constexpr int WTM = BM / 2, WTN = BN / 2, TM = WTM / 16, TN = WTN / 16;
@@ -463,11 +463,15 @@ __global__ void __launch_bounds__(256)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   // XCD-contiguous tile order: block id -> XCD id % 8 -> a contiguous range
-  // of tiles (a fastest), so one XCD's blocks share x columns in its L2.
+  // of a grouped order (group a-tiles swept together along b'), so the tiles
+  // of one XCD form a compact block of (a, b') whose L rows and x columns its
+  // own L2 can hold.
   const int local = blockIdx.x >> 3;
   const int t = (blockIdx.x & 7) * xcd_chunk + local;
   if (local >= xcd_chunk || t >= tiles_a * tiles_b) return;
-  const int a0 = (t % tiles_a) * BA, c0 = (t / tiles_a) * BN;
+  const int span = group * tiles_b, first = (t / span) * group;
+  const int gsz = min(tiles_a - first, group), wt = t % span;
+  const int a0 = (first + wt % gsz) * BA, c0 = (wt / gsz) * BN;
   const int K = Dl, NC = DS * Dr;
   const int64_t ldL = (int64_t)Dl * WL;
   const int lr = lane & 15, lk = lane >> 4;
@@ -632,8 +636,9 @@ static void launch_step1(int Dl, int Dr, const double *L, const double *X, const
                          double *T2, hipStream_t s) {
   const int tiles_a = (Dl + 31) / 32, tiles_b = (DS * Dr + 63) / 64;
   const int chunk = (tiles_a * tiles_b + 7) / 8;
+  const int group = min(tiles_a, 8);
   hipLaunchKernelGGL((k_heff_step1<WL, DS, WL>), dim3(8 * chunk), dim3(256), 0, s, L, X, Mt, T2,
-                     Dl, Dr, tiles_a, tiles_b, chunk);
+                     Dl, Dr, tiles_a, tiles_b, chunk, group);
 }
 
 // T2 = mode(L x) in one launch for FP64 real operators with wl = wr in

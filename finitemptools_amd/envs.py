@@ -74,6 +74,30 @@ def expectation(mpo, mps):
     return hold.reshape(-1)[0].item()
 
 
+def local_expectations(mps, op):
+    """<psi|op_i|psi> / <psi|psi> at every site i: one left and one right pass
+    of identity-MPO environment updates (mpo/transfer.jl:37-55 with
+    hj::Nothing), then per site one update with the 1 x d x 1 x d operator
+    tensor closed against the right environment -- O(L) contractions instead
+    of one full expectation (mpo/arithmetics.jl:267-275) per site."""
+    L, d = len(mps), mps[0].shape[1]
+    dt, dev = mps[0].dtype, mps[0].device
+    W = np.zeros((1, d, 1, d), dtype=np.complex128 if mps[0].is_complex() else np.float64)
+    W[0, :, 0, :] = np.asarray(op)
+    Wd = to_device(W, dev, dt)
+    left = [_boundary(dt, dev)]
+    for i in range(L):
+        left.append(transfer.updateleft(left[-1], mps[i], None, mps[i]))
+    right = [None] * (L + 1)
+    right[L] = _boundary(dt, dev)
+    for i in range(L - 1, 0, -1):
+        right[i] = transfer.updateright(right[i + 1], mps[i], None, mps[i])
+    norm = left[L].reshape(-1)[0]
+    vals = torch.stack([torch.sum(transfer.updateleft(left[i], mps[i], Wd, mps[i]) * right[i + 1])
+                        for i in range(L)])
+    return (vals / norm).real.cpu().numpy()
+
+
 def mpo_to_device(mpo_host, dtype=torch.float64, device="cuda"):
     return [to_device(W, device, dtype) for W in mpo_host]
 

@@ -17,4 +17,6 @@ oracle is pinned by closed forms, brute force and exact diagonalisation in
 - exponentiate: scipy.linalg.expm on the dense H_eff.         -> pinned
 - DMRG2 sweeps + MPSTruncation: ED energies (scipy eigsh), Bethe-ansatz
   trend, truncation semantics on hand-made spectra.           -> pinned
+- freefermion.tfim_quench_sz (closed form of the TFIM quench used to check
+  the L = 100 TDVP run): exact state-vector evolution at L = 8. -> pinned
 """

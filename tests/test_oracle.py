@@ -228,3 +228,23 @@ def test_hubbard_sector_energy_minimum_is_the_ground_energy():
     assert sum(comb(L, a) * comb(L, b) for a in range(L + 1) for b in range(L + 1)) == 4 ** L
     # one electron: the lowest free-particle band energy -2t cos(pi/(L+1))
     assert abs(ed.hubbard_sector_energy(L, 1, 0) + 2 * np.cos(np.pi / (L + 1))) < 1e-12
+
+
+@pytest.mark.parametrize("h", [1.0, 0.4])
+def test_tfim_quench_closed_form_matches_ed(h):
+    """The free-fermion <sz_j(t)> of the TFIM quench from |+>^L (the oracle for
+    the L = 100 TDVP measurement) equals exact evolution of the L = 8 state
+    vector; at t = 0 it vanishes; a flipped field sign moves it."""
+    import scipy.sparse.linalg as spla
+    from oracle.freefermion import tfim_quench_sz
+    L = 8
+    H = ed.tfim_hamiltonian(L, 1.0, h)
+    psi0 = np.full(2 ** L, 2 ** (-L / 2), dtype=np.complex128)
+    times = [0.0, 0.3, 1.0, 2.5]
+    ff = tfim_quench_sz(L, 1.0, h, times)
+    for t, row in zip(times, ff):
+        psi = spla.expm_multiply(-1j * t * H, psi0)
+        ref = np.array([np.vdot(psi, ed._site(ed.SZ, i, L, 2) @ psi).real for i in range(L)])
+        assert np.allclose(row, ref, atol=1e-12)
+    assert np.abs(ff[0]).max() < 1e-14
+    assert np.abs(tfim_quench_sz(L, 1.0, -h, [1.0])[0] - ff[2]).max() > 0.1

@@ -98,3 +98,29 @@ def test_mixed_precision_quench_trajectory(gpu, prec, tol):
     assert 0.0 < err < tol
     ref = exact_sz(L, h, dt * nsteps, plus_state(L))
     assert np.max(np.abs(traj[prec][-1] - ref)) < max(1e-4, 2 * tol)
+
+
+def test_local_expectations_all_sites(gpu):
+    """One-pass <sz_i> at every site equals the per-site expectation and the
+    exact state's values (random complex MPS, not normalised)."""
+    import torch
+    from finitemptools_amd.envs import local_expectation, local_expectations
+    from finitemptools_amd.mps import FiniteMPS
+    rng = np.random.default_rng(2)
+    L, D = 7, 5
+    dims = [min(D, 2 ** i, 2 ** (L - i)) for i in range(L + 1)]
+    from finitemptools_amd.engine import to_device
+    ts = [to_device(rng.standard_normal((dims[i], 2, dims[i + 1]))
+                    + 1j * rng.standard_normal((dims[i], 2, dims[i + 1])), "cuda", torch.complex128)
+          for i in range(L)]
+    psi = FiniteMPS(ts)
+    got = local_expectations(psi, ed.SZ)
+    one = np.array([local_expectation(psi, ed.SZ, i).real for i in range(L)])
+    assert np.allclose(got, one, atol=1e-12)
+    v = ts[0].cpu().numpy().reshape(2, -1)
+    for t in ts[1:]:
+        a = t.cpu().numpy()
+        v = np.einsum("pa,asb->psb", v, a).reshape(-1, a.shape[2])
+    v = v.reshape(-1)
+    exact = np.array([np.vdot(v, ed._site(ed.SZ, i, L, 2) @ v).real for i in range(L)]) / np.vdot(v, v).real
+    assert np.allclose(got, exact, atol=1e-12)
