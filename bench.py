@@ -294,12 +294,14 @@ def gradients_converged_j1j2(L=40, D=256, J2=0.25, nsweeps=5, h=1e-5):
             "dE_dA_norm": float(sum(torch.sum(a * a).item() for a in res.dA) ** 0.5)}
 
 
-def tdvp_tfim_quench(L=100, nsteps=40, dt=0.05, D=512, precs=("f64", "f32", "bf16x3")):
+def tdvp_tfim_quench(L=100, nsteps=40, dt=0.05, D=512, precs=("f64", "f32")):
     """TDVP2 quench of the transverse-field Ising chain (J = 1, h = 0 -> 1)
     from |+>^L, ComplexF64 storage, truncation D <= 512 at eps = 1e-8, Lanczos
     exponentiate (dimension <= 12, FP64 re-orthogonalisation in the mixed
     forms): <sz_i(t)> at every step against the free-fermion closed form and,
-    for the mixed precisions, against the FP64 trajectory."""
+    for the mixed precision, against the FP64 trajectory.  (Split-BF16 is left
+    out: its ~1e-5 relative noise lies above the 1e-8 truncation threshold, so
+    the bond dimension runs to the cap and the run takes minutes.)"""
     import torch
     from finitemptools_amd import models, tdvp
     from finitemptools_amd.envs import environments, local_expectations, mpo_to_device
