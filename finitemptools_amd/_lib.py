@@ -65,6 +65,8 @@ SIGNATURES = {
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mh_updateright": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mh_ac_prime_open": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                 c_void_p, c_void_p, c_void_p]),
     "mh_heff_size": (c_int, [P(HeffOp), P(c_size_t)]),
     "mh_heff_apply": (c_int, [c_void_p, P(HeffOp), c_void_p, c_void_p]),
     "mh_lanczos_eig": (c_int, [c_void_p, P(HeffOp), c_void_p, c_int, c_double, c_int,

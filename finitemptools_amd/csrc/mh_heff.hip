@@ -475,6 +475,32 @@ int mh_ac2_prime(mh_handle h, int dtype, int prec, int Dl, int d1, int d2,
   return mh_heff_apply(h, &op, x, y);
 }
 
+int mh_ac_prime_open(mh_handle h, int dtype, int Dl, int d, int Dr, int wl, int wr,
+                     const void *x, const void *W, const void *L, void *P) {
+  if (!h) return MH_ERR_ARG;
+  // the checks of ac_prime with a placeholder right environment
+  mh_heff_op op{MH_HEFF1, dtype, MH_PREC_F64, Dl, d, 1, Dr, wl, 0, wr, W, nullptr, L, L};
+  MH_TRY(heff_check(h, op));
+  if (!x || !P) return fail(h, MH_ERR_ARG, "null tensor");
+  if (x == P) return fail(h, MH_ERR_ARG, "x and P must not alias");
+  const size_t b = dtype_bytes(dtype);
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  const size_t t1 = (size_t)Dl * wl * d * Dr, mt = (size_t)wl * d * d * wr;
+  MH_TRY(ensure_ws(h, al(t1 * b) + al(mt * b)));
+  char *T1 = (char *)h->ws;
+  char *Mt = T1 + al(t1 * b);
+  MH_TRY(build_mode_matrix(h, dtype, MODE_SITE, wl, 0, wr, d, 1, W, nullptr, Mt));
+  if (h->fusion && dtype == MH_F64) {
+    const int st = heff_step1_fused(h, Dl, Dr, wl, d, wr, L, x, Mt, P);
+    if (st != MH_ERR_UNSUPPORTED) return st;
+  }
+  GemmDesc g1 = gd(Dl * wl, d * Dr, Dl, L, 1, (int64_t)Dl * wl, 0, x, 1, Dl, T1, 1,
+                   (int64_t)Dl * wl);
+  MH_TRY(gemm(h, dtype, MH_PREC_F64, g1));
+  return mode_product(h, dtype, Dl, wl * d, d * wr, Dr, T1, (int64_t)Dl * wl * d, Mt, P,
+                      (int64_t)Dl * d * wr);
+}
+
 int mh_c_prime(mh_handle h, int dtype, int prec, int Dl, int Dr, int w,
                const void *x, const void *L, const void *R, void *y) {
   mh_heff_op op{MH_HEFF0, dtype, prec, Dl, 1, 1, Dr, w, 0, w, nullptr, nullptr, L, R};

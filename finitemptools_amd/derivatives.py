@@ -122,6 +122,24 @@ def ac_prime(x, m, hleft, hright, prec="f64"):
     return HeffOperator(1, hleft, hright, m, prec=prec).apply(x)
 
 
+def ac_prime_open(x, m, hleft):
+    """P[a,s',w',b'] = L[a,w,a'] x[a',s,b'] W[w,s',w',s]: ac_prime with its
+    right environment left open (FP64 / C64), the expansion term of
+    expansion/chexpand.jl:23-25.  Returns a (Dl, d, wr, Dr) tensor."""
+    _same([x, m, hleft])
+    x, m, hleft = as_colmajor(x), as_colmajor(m), as_colmajor(hleft)
+    Dl, d, Dr = x.shape
+    wl, _, wr, _ = m.shape
+    if tuple(hleft.shape) != (Dl, wl, Dl) or tuple(m.shape) != (wl, d, wr, d):
+        raise _dm(f"ac_prime_open: L {tuple(hleft.shape)}, W {tuple(m.shape)}, x {tuple(x.shape)}")
+    P = colmajor_empty((Dl, d, wr, Dr), x.dtype, x.device)
+    e = engine(x.device)
+    h = e.bind_stream()
+    e.check(e.lib.mh_ac_prime_open(h, dtype_code(x), Dl, d, Dr, wl, wr, ptr(x), ptr(m),
+                                   ptr(hleft), ptr(P)), "ac_prime_open")
+    return P
+
+
 def ac2_prime(x, h1, h2, hleft, hright, prec="f64"):
     """y = (((L x) W1) W2) R on a two-site tensor x[a,s1,s2,b] (derivatives.jl:20-22)"""
     _same([x, h1, h2, hleft, hright])

@@ -159,12 +159,15 @@ class DMRG1:
 
 
 class DMRG1S:
-    """dmrg.jl:148-156: one-site DMRG with the optimal subspace expansion."""
+    """dmrg.jl:148-156: one-site DMRG with subspace expansion (``expan``:
+    expansion.OptimalExpansion, the default, or expansion.CHExpansion)."""
 
     def __init__(self, maxiter=MAXITER, tol=TOL, eig_maxiter=10, eig_tol=EIG_TOL,
-                 verbosity=0, trunc=None, prec="f64", reorth=None):
+                 verbosity=0, trunc=None, prec="f64", reorth=None, expan=None):
+        from .expansion import OptimalExpansion
         self.maxiter, self.tol = maxiter, tol
         self.eig_maxiter, self.eig_tol, self.verbosity = eig_maxiter, eig_tol, verbosity
+        self.expan = expan or OptimalExpansion()
         self.trunc = trunc or MPSTruncation()
         self.prec = prec
         self.reorth = (prec != "f64") if reorth is None else reorth
@@ -201,7 +204,7 @@ def _leftsweep1(m, alg, expand):
     energies, delta = [], 0.0
     for site in range(len(mps) - 1):
         if expand:
-            right_expansion(m, site, alg.trunc, prec=alg.prec)
+            right_expansion(m, site, alg.trunc, prec=alg.prec, expan=alg.expan)
         ev, vecs = _site_eig(m, alg, site)
         energies.append(ev)
         delta = max(delta, calc_galerkin(m, site, alg.prec))
@@ -220,7 +223,7 @@ def _rightsweep1(m, alg, expand):
     energies, delta = [], 0.0
     for site in range(len(mps) - 1, 0, -1):
         if expand:
-            left_expansion(m, site, alg.trunc, prec=alg.prec)
+            left_expansion(m, site, alg.trunc, prec=alg.prec, expan=alg.expan)
         ev, vecs = _site_eig(m, alg, site)
         energies.append(ev)
         delta = max(delta, calc_galerkin(m, site, alg.prec))

@@ -1,12 +1,19 @@
-"""Optimal subspace expansion (algorithms/expansion/optimalexpand.jl:4-69):
-enlarge the bond between two sites by the dominant directions of the
-two-site H_eff residual projected onto the complements of both site spaces.
-The two-site H_eff application runs on the engine; the null spaces and the
-small SVDs are FP64 dense linear algebra.
+"""Subspace expansion of the single-site algorithms (DMRG1S, TDVP1S):
+
+- optimal expansion (algorithms/expansion/optimalexpand.jl:4-69, the
+  reference's default): enlarge the bond between two sites by the dominant
+  directions of the two-site H_eff residual projected onto the complements of
+  both site spaces;
+- CH expansion (expansion/chexpand.jl:1-63, Hubig et al., PRB 91, 155115):
+  append alpha * L.A.W (the site's H_eff with its far bond left open) to the
+  bond and truncate by an SVD.
+
+The H_eff contractions run on the engine; the null spaces and the small SVDs
+are FP64 dense linear algebra.
 """
 import torch
 
-from .derivatives import ac2_prime
+from .derivatives import ac2_prime, ac_prime_open
 from .linalg import as_matrix, from_matrix, gemm, two_site
 from .truncation import MPSTruncation
 
@@ -61,7 +68,85 @@ class OptimalExpansion:
     """optimalexpand.jl:4-6 (the reference's default expansion)."""
 
 
-def right_expansion(m, site, trunc, prec="f64"):
+class CHExpansion:
+    """chexpand.jl:5-17: expansion term alpha * L.A.W; ``momentum`` is kept
+    (and checked) as in the reference, which does not use it further."""
+
+    def __init__(self, eps=0.01, momentum=0.9):
+        if not (0.0 < momentum <= 1.0):
+            raise ValueError("momentum should be in (0, 1].")
+        self.eps, self.momentum = float(eps), float(momentum)
+
+
+def _merge_last(P):
+    """(n0, n1, n2, n3) column-major -> (n0, n1, n2*n3), index k2 + n2*k3."""
+    n0, n1, n2, n3 = P.shape
+    return P.permute(3, 2, 1, 0).contiguous().reshape(n3 * n2, n1, n0).permute(2, 1, 0)
+
+
+def _ch_left(m, site, expan, trunc):
+    """left_expansion! for CHExpansion (chexpand.jl:20-29, 42-51): bond
+    (site-1, site); A1' = [alpha L.A1.W | A1], A2' = [0; A2], truncated SVD
+    of A1', then A1 = u, A2 = s v A2'."""
+    mpo, mps, hs = m.mpo, m.mps, m.hstorage
+    A1, A2 = mps[site - 1], mps[site]
+    Dl, d, Dm = A1.shape
+    _, d2, Dr = A2.shape
+    if expan.eps == 0.0:
+        a, nz = as_matrix(A1, 2), 0
+    else:
+        P = _merge_last(ac_prime_open(A1, mpo[site - 1], hs[site - 1]))   # (Dl, d, wr Dm)
+        nz = P.shape[2]
+        a = torch.cat([as_matrix(P, 2) * expan.eps, as_matrix(A1, 2)], dim=1)
+    u, s, v = _tsvd_mat(a, trunc)
+    k = u.shape[1]
+    mps[site - 1] = from_matrix(u, (Dl, d, k))
+    sv = v[:, nz:] * s.to(v.dtype).reshape(-1, 1)
+    mps[site] = from_matrix(gemm(sv.contiguous(), as_matrix(A2, 1)), (k, d2, Dr))
+    m.updateleft(site - 1)
+
+
+def _ch_right(m, site, expan, trunc):
+    """right_expansion! for CHExpansion (chexpand.jl:31-40, 52-61): bond
+    (site, site+1); A2' = [alpha W.A2.R (left bond (w, l) fused, w fastest);
+    A2], A1' = [0 | A1], truncated SVD of A2', then A2 = v, A1 = A1' u s.
+    The open-left contraction is ac_prime_open of the bond-transposed
+    tensors: x~[r2,s,l] = A2[l,s,r2], W~[w',s',w,s] = W[w,s',w',s], L~ = R."""
+    mpo, mps, hs = m.mpo, m.mps, m.hstorage
+    A1, A2 = mps[site], mps[site + 1]
+    Dl, d, Dm = A1.shape
+    _, d2, Dr = A2.shape
+    if expan.eps == 0.0:
+        b, nz = as_matrix(A2, 1), 0
+    else:
+        Pt = ac_prime_open(A2.permute(2, 1, 0), mpo[site + 1].permute(2, 1, 0, 3), hs[site + 2])
+        Q = _merge_last(Pt).permute(2, 1, 0)                                # (wl Dm, d2, Dr)
+        nz = Q.shape[0]
+        b = torch.cat([as_matrix(Q, 1) * expan.eps, as_matrix(A2, 1)], dim=0)
+    u, s, v = _tsvd_mat(b, trunc)
+    k = v.shape[0]
+    mps[site + 1] = from_matrix(v, (k, d2, Dr))
+    us = u[nz:, :] * s.to(u.dtype).reshape(1, -1)
+    mps[site] = from_matrix(gemm(as_matrix(A1, 2), us.contiguous()), (Dl, d, k))
+    m.updateright(site + 1)
+
+
+def right_expansion(m, site, trunc, prec="f64", expan=None):
+    """right_expansion!: expand bond (site, site+1) while sweeping right with
+    the optimal (default) or the CH scheme."""
+    if isinstance(expan, CHExpansion):
+        return _ch_right(m, site, expan, trunc)
+    return _optimal_right(m, site, trunc, prec)
+
+
+def left_expansion(m, site, trunc, prec="f64", expan=None):
+    """left_expansion!: expand bond (site-1, site) while sweeping left."""
+    if isinstance(expan, CHExpansion):
+        return _ch_left(m, site, expan, trunc)
+    return _optimal_left(m, site, trunc, prec)
+
+
+def _optimal_right(m, site, trunc, prec="f64"):
     """optimalexpand.jl:42-69: expand bond (site, site+1) while sweeping right."""
     mpo, mps, hs = m.mpo, m.mps, m.hstorage
     A, ACi = mps[site], mps[site + 1]
@@ -84,7 +169,7 @@ def right_expansion(m, site, trunc, prec="f64"):
     m.updateright(site + 1)
 
 
-def left_expansion(m, site, trunc, prec="f64"):
+def _optimal_left(m, site, trunc, prec="f64"):
     """optimalexpand.jl:12-40: expand bond (site-1, site) while sweeping left."""
     mpo, mps, hs = m.mpo, m.mps, m.hstorage
     ACi, B = mps[site - 1], mps[site]

@@ -1,5 +1,5 @@
 """TDVP on the device: algorithms/tdvp.jl (TDVP1 :8-69, TDVP2 :71-142,
-sweep! :219-222).
+TDVP1S :145-213, sweep! :219-222).
 
 One ``sweep`` is a left sweep followed by a right sweep and advances the state
 by ``stepsize`` (a real stepsize evolves in imaginary time, -1j*dt in real
@@ -88,6 +88,19 @@ class TDVP1:
         self.prec, self.kdim, self.verbosity = prec, kdim, verbosity
 
 
+class TDVP1S:
+    """tdvp.jl:145-155: one-site TDVP with subspace expansion before every
+    forward step (``expan``: OptimalExpansion, the default, or CHExpansion)."""
+
+    def __init__(self, stepsize, exptol=TDVP_EXP_TOL, trunc=None, expan=None, prec="f64",
+                 kdim=12, verbosity=0):
+        from .expansion import OptimalExpansion
+        self.stepsize, self.exptol = complex(stepsize), exptol
+        self.trunc = trunc or default_truncation()
+        self.expan = expan or OptimalExpansion()
+        self.prec, self.kdim, self.verbosity = prec, kdim, verbosity
+
+
 def _dt(alg, x):
     """stepsize as the tensors' scalar type (a real stepsize on real data)."""
     return alg.stepsize.real if not x.is_complex() and alg.stepsize.imag == 0 else alg.stepsize
@@ -170,11 +183,58 @@ def _rightsweep1(m, alg):
         mps[site] = _exp(op1, dt / 2, mps[site], alg)
 
 
+def _leftsweep1s(m, alg):
+    """tdvp.jl:158-180: expand bond (site, site+1), forward one-site step,
+    QR, backward bond step; the last site takes the full step."""
+    from .expansion import right_expansion
+    mpo, mps, hs = m.mpo, m.mps, m.hstorage
+    L = len(mps)
+    for site in range(L - 1):
+        right_expansion(m, site, alg.trunc, prec=alg.prec, expan=alg.expan)
+        dt = _dt(alg, mps[site])
+        op1 = HeffOperator(1, hs[site], hs[site + 1], mpo[site], prec=alg.prec)
+        tmp = _exp(op1, dt / 2, mps[site], alg)
+        mps[site], v = leftorth(tmp)
+        hnew = transfer.updateleft(hs[site], mps[site], mpo[site], mps[site], prec=alg.prec)
+        op0 = HeffOperator(0, hnew, hs[site + 1], prec=alg.prec)
+        v = _exp(op0, -dt / 2, v, alg)
+        mps[site + 1] = _mul_left(v, mps[site + 1])
+        hs[site + 1] = hnew
+    site = L - 1
+    dt = _dt(alg, mps[site])
+    op1 = HeffOperator(1, hs[site], hs[site + 1], mpo[site], prec=alg.prec)
+    mps[site] = _exp(op1, dt, mps[site], alg)
+
+
+def _rightsweep1s(m, alg):
+    """tdvp.jl:182-213: LQ, backward bond step, expand bond (site-1, site),
+    forward one-site step."""
+    from .expansion import left_expansion
+    mpo, mps, hs = m.mpo, m.mps, m.hstorage
+    L = len(mps)
+    for site in range(L - 2, -1, -1):
+        dt = _dt(alg, mps[site])
+        v, Q = rightorth(mps[site + 1])
+        mps[site + 1] = Q
+        hnew = transfer.updateright(hs[site + 2], Q, mpo[site + 1], Q, prec=alg.prec)
+        op0 = HeffOperator(0, hs[site + 1], hnew, prec=alg.prec)
+        v = _exp(op0, -dt / 2, v, alg)
+        hs[site + 1] = hnew
+        mps[site] = _mul_right(mps[site], v)
+        if site > 0:
+            left_expansion(m, site, alg.trunc, prec=alg.prec, expan=alg.expan)
+        op1 = HeffOperator(1, hs[site], hs[site + 1], mpo[site], prec=alg.prec)
+        mps[site] = _exp(op1, dt / 2, mps[site], alg)
+
+
 def sweep(m, alg):
     """sweep! (tdvp.jl:219-222): left sweep, then right sweep."""
     if isinstance(alg, TDVP2):
         _leftsweep2(m, alg)
         _rightsweep2(m, alg)
+    elif isinstance(alg, TDVP1S):
+        _leftsweep1s(m, alg)
+        _rightsweep1s(m, alg)
     elif isinstance(alg, TDVP1):
         _leftsweep1(m, alg)
         _rightsweep1(m, alg)

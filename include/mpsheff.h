@@ -84,6 +84,14 @@ int mh_ac2_prime(mh_handle h, int dtype, int prec, int Dl, int d1, int d2,
                  int Dr, int wl, int wm, int wr, const void *x, const void *W1,
                  const void *W2, const void *L, const void *R, void *y);
 
+/* The right bond left open: P[a,s',w',b'] = sum L[a,w,a'] x[a',s,b'] W[w,s',w',s]
+ * (FP64 / C64 arithmetic), i.e. ac_prime before its last contraction with R.
+ * P: (Dl, d, wr, Dr) = the CH subspace-expansion term L.A.W with (w', b')
+ * fused, w' fastest (expansion/chexpand.jl:23-25); the left-open term of
+ * chexpand.jl:34-36 is the same contraction of the bond-transposed tensors. */
+int mh_ac_prime_open(mh_handle h, int dtype, int Dl, int d, int Dr, int wl, int wr,
+                     const void *x, const void *W, const void *L, void *P);
+
 /* y[a,b] = sum L[a,w,a'] x[a',b'] R[b,w,b']
  * replaces c_prime(x::MPSBondTensor, hleft, hright), algorithms/derivatives.jl:52-54. */
 int mh_c_prime(mh_handle h, int dtype, int prec, int Dl, int Dr, int w,

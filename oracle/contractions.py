@@ -15,6 +15,26 @@ def ac_prime(x, m, hleft, hright):
     return np.tensordot(t, hright, axes=([1, 3], [2, 1]))  # [a, s', b]
 
 
+def ch_expansion_term_left(hleft, A, m, alpha):
+    """expansion/chexpand.jl:23-25:
+    alpha * hleft[-1,1,2] A[2,3,4] m[1,-2,5,3] ft[5,4,-3] with ft the
+    isomorphism (w' (x) b') -> fused index w' + wr*b'.  Returns (Dl, d, wr*Dr)."""
+    t = np.tensordot(hleft, A, axes=([2], [0]))            # [a, w, s, b']
+    t = np.tensordot(t, m, axes=([1, 2], [0, 3]))          # [a, b', s', w']
+    t = alpha * t.transpose(0, 2, 3, 1)                    # [a, s', w', b']
+    return t.reshape(t.shape[0], t.shape[1], -1, order="F")
+
+
+def ch_expansion_term_right(A, m, hright, alpha):
+    """expansion/chexpand.jl:34-36:
+    alpha * ft[-1,1,2] A[2,3,4] m[1,-2,5,3] hright[-3,5,4] with ft the
+    isomorphism fused index w + wl*a <- (w (x) a).  Returns (wl*Dl, d, Dr)."""
+    t = np.tensordot(A, hright, axes=([2], [2]))           # [a, s, b, w']
+    t = np.tensordot(m, t, axes=([2, 3], [3, 1]))          # [w, s', a, b]
+    t = alpha * t.transpose(0, 2, 1, 3)                    # [w, a, s', b]
+    return t.reshape(-1, t.shape[2], t.shape[3], order="F")
+
+
 def ac2_prime(x, h1, h2, hleft, hright):
     """derivatives.jl:20-22:
     (((hleft[-1,1,2] x[2,3,4,5]) h1[1,-2,6,3]) h2[6,-3,7,4]) hright[-4,7,5]"""

@@ -248,3 +248,29 @@ def test_tfim_quench_closed_form_matches_ed(h):
         assert np.allclose(row, ref, atol=1e-12)
     assert np.abs(ff[0]).max() < 1e-14
     assert np.abs(tfim_quench_sz(L, 1.0, -h, [1.0])[0] - ff[2]).max() > 0.1
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_ch_expansion_terms_pinned_by_ac_prime(cplx):
+    """The CH expansion terms are ac_prime with one environment left open:
+    closing the fused bond of the left term with any R[b, w', b'] (and the
+    right term with any L[a, w, a']) gives alpha * ac_prime(A, W, L, R); a
+    closing environment of full rank pins every entry."""
+    rng = np.random.default_rng(17)
+    Dl, d, Dr, wl, wr, alpha = 3, 2, 4, 3, 5, 0.37
+    A = rand(rng, (Dl, d, Dr), cplx)
+    W = rand(rng, (wl, d, wr, d), cplx)
+    L = rand(rng, (Dl, wl, Dl), cplx)
+    R = rand(rng, (Dr, wr, Dr), cplx)
+    P = C.ch_expansion_term_left(L, A, W, alpha)
+    assert P.shape == (Dl, d, wr * Dr)
+    Rfull = rand(rng, (wr * Dr, wr, Dr), cplx)              # rank wr*Dr on the fused bond
+    ref = alpha * C.ac_prime(A, W, L, Rfull)
+    closed = np.einsum("asf,bf->asb", P, Rfull.reshape(wr * Dr, wr * Dr, order="F"))
+    assert np.allclose(closed, ref, atol=1e-12)
+    Q = C.ch_expansion_term_right(A, W, R, alpha)
+    assert Q.shape == (wl * Dl, d, Dr)
+    Lfull = rand(rng, (wl * Dl, wl, Dl), cplx)
+    ref = alpha * C.ac_prime(A, W, Lfull, R)
+    closed = np.einsum("af,fsb->asb", Lfull.reshape(wl * Dl, wl * Dl, order="F"), Q)
+    assert np.allclose(closed, ref, atol=1e-12)
