@@ -315,7 +315,8 @@ def tdvp_tfim_quench(L=100, nsteps=40, dt=0.05, D=512, precs=("f64", "f32")):
                              f"C64, D<={D} eps=1e-8"}, {}
     for prec in precs:
         psi = prodmps([np.ones(2) / np.sqrt(2)] * L, dtype=torch.complex128)
-        alg = tdvp.TDVP2(stepsize=-1j * dt, trunc=MPSTruncation(D=D, eps=1e-8), prec=prec)
+        alg = tdvp.TDVP2(stepsize=-1j * dt, trunc=MPSTruncation(D=D, eps=1e-8), prec=prec,
+                         ishermitian=True)
         env = environments(mpo, psi, prec)
         rows, sec = [], 0.0
         for _ in range(nsteps):

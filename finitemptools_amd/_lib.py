@@ -73,6 +73,8 @@ SIGNATURES = {
                                P(c_double), c_void_p, P(c_int), P(c_int)]),
     "mh_lanczos_expm": (c_int, [c_void_p, P(HeffOp), c_void_p, c_double, c_double, c_int,
                                 c_double, c_void_p, P(c_int)]),
+    "mh_arnoldi_expm": (c_int, [c_void_p, P(HeffOp), c_void_p, c_double, c_double, c_int,
+                                c_double, c_void_p, P(c_int)]),
     "mh_set_graphs": (c_int, [c_void_p, c_int]),
     "mh_set_fusion": (c_int, [c_void_p, c_int]),
     "mh_dot": (c_int, [c_void_p, c_int, c_size_t, c_void_p, c_void_p, P(c_double)]),

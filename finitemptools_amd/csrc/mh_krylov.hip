@@ -14,6 +14,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <functional>
 
 #include "mh_internal.h"
 
@@ -134,17 +135,19 @@ __global__ void k_axpby(size_t n, double ar, double ai, const double *__restrict
 // st[3] initial norm, st[4..5] scratch (re, im of the last dot),
 // st[8 .. 8+kmax) alpha, st[8+kmax .. 8+2kmax) beta,
 // st[8+2kmax .. 8+2kmax+2*(kmax+1)) combination coefficients (re, im),
-// then 2*MAXPART partials and (kmax+1)*MAXPART*2 multi-dot partials.
+// then 2*MAXPART partials, (kmax+1)*MAXPART*2 multi-dot partials and the
+// Arnoldi Hessenberg columns H[j, i] (re, im) at o_h + 2*(j + (kmax+1)*i).
 struct LanLayout {
   int kmax;
-  int o_alpha, o_beta, o_coef, o_part, o_mpart, total;
+  int o_alpha, o_beta, o_coef, o_part, o_mpart, o_h, total;
   explicit LanLayout(int k) : kmax(k) {
     o_alpha = 8;
     o_beta = o_alpha + k;
     o_coef = o_beta + k;
     o_part = o_coef + 2 * (k + 1);
     o_mpart = o_part + 2 * MAXPART;
-    total = o_mpart + 2 * MAXPART * (k + 1);
+    o_h = o_mpart + 2 * MAXPART * (k + 1);
+    total = o_h + 2 * (k + 1) * k;
   }
 };
 
@@ -263,6 +266,21 @@ __global__ void __launch_bounds__(RB) k_mdot_reduce(const double *__restrict__ p
   if (threadIdx.x == 0) {
     coef[2 * j] = re;
     coef[2 * j + 1] = im;
+  }
+}
+
+// Arnoldi: H[0..nv, i] = coef (pass 0) or += coef (pass 1), frozen once done.
+__global__ void k_hcol(double *__restrict__ st, int o_h, int ld, int i, int nv,
+                       const double *__restrict__ coef, int pass) {
+  const int j = threadIdx.x;
+  if (j >= nv || st[0] != 0.0) return;
+  double *hc = st + o_h + 2 * ((size_t)j + (size_t)ld * i);
+  if (pass == 0) {
+    hc[0] = coef[2 * j];
+    hc[1] = coef[2 * j + 1];
+  } else {
+    hc[0] += coef[2 * j];
+    hc[1] += coef[2 * j + 1];
   }
 }
 
@@ -387,6 +405,81 @@ void sym_eig(int k, const double *A, double *evals, double *Z) {
   std::memcpy(Z, Zs.data(), sizeof(double) * k * k);
 }
 
+// exp(A) of a small complex k x k matrix (column-major) by scaling and
+// squaring with the diagonal [8/8] Pade approximant (Moler & Van Loan,
+// "Nineteen dubious ways", method 3): A / 2^s with |A/2^s|_1 <= 1/2,
+// N = sum c_j X^j, D = sum (-1)^j c_j X^j, exp(X) ~ D^-1 N (Gaussian
+// elimination with partial pivoting), then s squarings.
+typedef std::complex<double> cplx_t;
+static void expm_small(int k, const std::vector<cplx_t> &A, std::vector<cplx_t> &E) {
+  auto idx = [k](int r, int c) { return r + (size_t)k * c; };
+  double nrm = 0.0;
+  for (int c = 0; c < k; ++c) {
+    double s = 0.0;
+    for (int r = 0; r < k; ++r) s += std::abs(A[idx(r, c)]);
+    nrm = std::max(nrm, s);
+  }
+  int sq = 0;
+  while (nrm > 0.5 && sq < 200) {
+    nrm *= 0.5;
+    ++sq;
+  }
+  const double scale = std::ldexp(1.0, -sq);
+  std::vector<cplx_t> X(A.size()), P((size_t)k * k), N((size_t)k * k, 0.0), D((size_t)k * k, 0.0);
+  for (size_t i = 0; i < A.size(); ++i) X[i] = A[i] * scale;
+  auto mul = [&](const std::vector<cplx_t> &a, const std::vector<cplx_t> &b, std::vector<cplx_t> &o) {
+    std::vector<cplx_t> t((size_t)k * k, 0.0);
+    for (int c = 0; c < k; ++c)
+      for (int m = 0; m < k; ++m) {
+        const cplx_t bv = b[idx(m, c)];
+        if (bv == 0.0) continue;
+        for (int r = 0; r < k; ++r) t[idx(r, c)] += a[idx(r, m)] * bv;
+      }
+    o.swap(t);
+  };
+  const int q = 8;
+  double cj = 1.0;
+  for (int r = 0; r < k; ++r) P[idx(r, r)] = 1.0;  // X^0
+  for (int j = 0; j <= q; ++j) {
+    if (j > 0) {
+      cj *= (double)(q - j + 1) / (double)(j * (2 * q - j + 1));
+      mul(P, X, P);
+    }
+    const double sg = (j % 2) ? -1.0 : 1.0;
+    for (size_t i = 0; i < P.size(); ++i) {
+      N[i] += cj * P[i];
+      D[i] += sg * cj * P[i];
+    }
+  }
+  // Solve D E = N.
+  std::vector<int> piv(k);
+  for (int c = 0; c < k; ++c) {
+    int p = c;
+    for (int r = c + 1; r < k; ++r)
+      if (std::abs(D[idx(r, c)]) > std::abs(D[idx(p, c)])) p = r;
+    piv[c] = p;
+    if (p != c) {
+      for (int m = 0; m < k; ++m) std::swap(D[idx(c, m)], D[idx(p, m)]);
+      for (int m = 0; m < k; ++m) std::swap(N[idx(c, m)], N[idx(p, m)]);
+    }
+    const cplx_t d = D[idx(c, c)];
+    for (int r = c + 1; r < k; ++r) {
+      const cplx_t f = D[idx(r, c)] / d;
+      if (f == 0.0) continue;
+      for (int m = c; m < k; ++m) D[idx(r, m)] -= f * D[idx(c, m)];
+      for (int m = 0; m < k; ++m) N[idx(r, m)] -= f * N[idx(c, m)];
+    }
+  }
+  E.assign((size_t)k * k, 0.0);
+  for (int m = 0; m < k; ++m)
+    for (int r = k - 1; r >= 0; --r) {
+      cplx_t s = N[idx(r, m)];
+      for (int c = r + 1; c < k; ++c) s -= D[idx(r, c)] * E[idx(c, m)];
+      E[idx(r, m)] = s / D[idx(r, r)];
+    }
+  for (int i = 0; i < sq; ++i) mul(E, E, E);
+}
+
 // ---------------------------------------------------------- drivers
 
 struct Krylov {
@@ -403,6 +496,7 @@ struct Krylov {
   void *Mt;
   double *st;
   int G;
+  bool arnoldi = false;  // full orthogonalisation, Hessenberg H (non-Hermitian H_eff)
 
   Krylov(mh_handle hh, const mh_heff_op &o, int k)
       : h(hh), op(o), cplx(o.dtype == MH_C64), kmax(k), lay(k) {
@@ -467,8 +561,38 @@ struct Krylov {
     return check_launch(h, "lanczos start");
   }
 
+  // One Arnoldi step i: w = A V_i, two classical Gram-Schmidt passes against
+  // V_0..V_i whose coefficients sum to the Hessenberg column H[0..i, i],
+  // H[i+1, i] = |w| (kept as beta_i), V_{i+1} = w / |w|.
+  int step_arnoldi(int i, double btol, bool write_next) {
+    double *part = st + lay.o_part;
+    MH_TRY(apply(vec(i), w));
+    int slot;
+    prof_begin(h, MH_K_VEC, 0.0, &slot);
+    double *mpart = st + lay.o_mpart;
+    double *coef = st + lay.o_coef;
+    for (int pass = 0; pass < 2; ++pass) {
+      if (cplx) hipLaunchKernelGGL(k_mdot_part<true>, dim3(G), dim3(RB), 0, h->stream, V, vstride_n(), i + 1, w, mpart);
+      else hipLaunchKernelGGL(k_mdot_part<false>, dim3(G), dim3(RB), 0, h->stream, V, vstride_n(), i + 1, w, mpart);
+      hipLaunchKernelGGL(k_mdot_reduce, dim3(i + 1), dim3(RB), 0, h->stream, mpart, G, coef);
+      hipLaunchKernelGGL(k_hcol, dim3(1), dim3(128), 0, h->stream, st, lay.o_h, kmax + 1, i, i + 1, coef, pass);
+      double *pp = (pass == 1) ? part : nullptr;
+      if (cplx) hipLaunchKernelGGL(k_msub<true>, dim3(G), dim3(RB), 0, h->stream, vstride_n(), i + 1, V, coef, w, st, pp);
+      else hipLaunchKernelGGL(k_msub<false>, dim3(G), dim3(RB), 0, h->stream, vstride_n(), i + 1, V, coef, w, st, pp);
+    }
+    hipLaunchKernelGGL(k_take_beta, dim3(1), dim3(RB), 0, h->stream, part, G, st, lay.o_beta, i, btol);
+    if (write_next) {
+      unsigned gb = (unsigned)((nd + 255) / 256);
+      if (cplx) hipLaunchKernelGGL(k_div<true>, dim3(gb), dim3(256), 0, h->stream, n, w, vec(i + 1), st + lay.o_beta + i, st);
+      else hipLaunchKernelGGL(k_div<false>, dim3(gb), dim3(256), 0, h->stream, n, w, vec(i + 1), st + lay.o_beta + i, st);
+    }
+    prof_end(h, slot);
+    return check_launch(h, "arnoldi step");
+  }
+
   // One Lanczos step i: w = H V_i, alpha_i, w -= ..., (reorth), beta_i, V_{i+1}.
   int step(int i, double btol, int reorth, bool write_next) {
+    if (arnoldi) return step_arnoldi(i, btol, write_next);
     double *part = st + lay.o_part;
     MH_TRY(apply(vec(i), w));
     int slot;
@@ -510,6 +634,16 @@ struct Krylov {
     MH_HIP(h, hipMemcpyAsync(h->red_host, st, sizeof(double) * lay.o_coef, hipMemcpyDeviceToHost, h->stream));
     MH_HIP(h, hipStreamSynchronize(h->stream));
     std::memcpy(hs.data(), h->red_host, sizeof(double) * lay.o_coef);
+    return MH_OK;
+  }
+
+  // Arnoldi: the Hessenberg columns, hc[2*(j + (kmax+1)*i) + {0,1}].
+  int read_hessenberg(std::vector<double> &hc) {
+    const size_t nh = 2 * (size_t)(kmax + 1) * kmax;
+    hc.resize(nh);
+    MH_HIP(h, hipMemcpyAsync(h->red_host, st + lay.o_h, sizeof(double) * nh, hipMemcpyDeviceToHost, h->stream));
+    MH_HIP(h, hipStreamSynchronize(h->stream));
+    std::memcpy(hc.data(), h->red_host, sizeof(double) * nh);
     return MH_OK;
   }
 
@@ -626,6 +760,120 @@ static int lanczos_eig_run(mh_handle h, Krylov &K, const void *x0, int kmax, dou
   return MH_OK;
 }
 
+// exp(dt H_eff) x0 by Krylov sub-steps: Lanczos (Hermitian H_eff, the
+// tridiagonal exponentiated through its eigen-decomposition) or Arnoldi
+// (any H_eff, the Hessenberg matrix exponentiated by Pade).  Each sub-step
+// takes the largest tau (halving from the remaining time) whose a-posteriori
+// error beta0 * h_{k,k-1} * |e_k^T exp(tau H_k) e1| is below tol * |tau|.
+static int krylov_expm(mh_handle h, const mh_heff_op *op, const void *x0, double dt_re,
+                       double dt_im, int kdim, double tol, bool arnoldi, void *xout,
+                       int *info) {
+  if (!h || !op) return MH_ERR_ARG;
+  MH_TRY(heff_check(h, *op));
+  if (!x0 || !xout) return fail(h, MH_ERR_ARG, "null vector");
+  if (kdim < 2 || kdim > 64) return fail(h, MH_ERR_ARG, "kdim must be in [2, 64]");
+  if (op->dtype == MH_F64 && dt_im != 0.0)
+    return fail(h, MH_ERR_DTYPE, "complex time step needs MH_C64 data");
+  if (!(tol > 0.0)) return fail(h, MH_ERR_ARG, "tol must be positive");
+  const std::complex<double> dt_total(dt_re, dt_im);
+  const double adt = std::abs(dt_total);
+  if (adt == 0.0) {
+    size_t n = heff_vec_len(*op);
+    MH_HIP(h, hipMemcpyAsync(xout, x0, n * dtype_bytes(op->dtype), hipMemcpyDeviceToDevice, h->stream));
+    if (info) *info = 0;
+    return MH_OK;
+  }
+  Krylov K(h, *op, kdim);
+  K.arnoldi = arnoldi;
+  MH_TRY(K.setup());
+  StepGraph graph;
+  const size_t n = heff_vec_len(*op);
+  const void *src = x0;
+  std::complex<double> done_t(0.0, 0.0);
+  int nsub = 0, status = 0;
+  while (true) {
+    std::complex<double> rem = dt_total - done_t;
+    if (std::abs(rem) <= 1e-14 * adt) break;
+    if (++nsub > 4096) { status = -1; break; }
+    // Rebuild the Krylov state for this sub-step (st is cleared by setup()).
+    if (nsub > 1) {
+      hipLaunchKernelGGL(k_fill, dim3((K.lay.total + 255) / 256), dim3(256), 0, h->stream, K.st, K.lay.total, 0.0);
+    }
+    MH_TRY(K.start(src));
+    // Happy breakdown threshold relative to the vector norm 1.  Steps
+    // 1..kdim-1 are the same launches in every sub-step: one graph.
+    MH_TRY(K.step(0, 1e-13, 1, kdim > 1));
+    MH_TRY(run_steps(h, K, kdim, 1e-13, 1, graph));
+    std::vector<double> hs;
+    MH_TRY(K.read_state(hs));
+    const double b0 = hs[3];
+    if (!(b0 > 0.0)) {  // zero vector stays zero
+      MH_HIP(h, hipMemcpyAsync(xout, src, n * dtype_bytes(op->dtype), hipMemcpyDeviceToDevice, h->stream));
+      break;
+    }
+    int k = (int)hs[1];
+    if (k < 1) k = 1;
+    const bool invariant = hs[0] != 0.0;
+    const double bk = hs[K.lay.o_beta + k - 1];
+    std::function<void(std::complex<double>, std::vector<std::complex<double>> &)> expT_e1;
+    std::vector<double> T, ev, Z, hc;
+    std::vector<cplx_t> Hk;
+    if (!arnoldi) {
+      T.assign((size_t)k * k, 0.0);
+      ev.resize(k);
+      Z.resize((size_t)k * k);
+      for (int i = 0; i < k; ++i) {
+        T[i + k * i] = hs[K.lay.o_alpha + i];
+        if (i + 1 < k) T[i + 1 + k * i] = T[i + k * (i + 1)] = hs[K.lay.o_beta + i];
+      }
+      sym_eig(k, T.data(), ev.data(), Z.data());
+      expT_e1 = [&](std::complex<double> tau, std::vector<std::complex<double>> &f) {
+        f.assign(k, 0.0);
+        for (int c = 0; c < k; ++c) {
+          std::complex<double> e = std::exp(tau * ev[c]) * Z[0 + k * c];
+          for (int r = 0; r < k; ++r) f[r] += Z[r + k * c] * e;
+        }
+      };
+    } else {
+      MH_TRY(K.read_hessenberg(hc));
+      Hk.assign((size_t)k * k, 0.0);
+      for (int i = 0; i < k; ++i) {
+        for (int j = 0; j <= i; ++j) {
+          const size_t o = 2 * ((size_t)j + (size_t)(kdim + 1) * i);
+          Hk[j + (size_t)k * i] = cplx_t(hc[o], op->dtype == MH_C64 ? hc[o + 1] : 0.0);
+        }
+        if (i + 1 < k) Hk[i + 1 + (size_t)k * i] = hs[K.lay.o_beta + i];
+      }
+      expT_e1 = [&](std::complex<double> tau, std::vector<std::complex<double>> &f) {
+        std::vector<cplx_t> A(Hk.size()), E;
+        for (size_t q = 0; q < Hk.size(); ++q) A[q] = tau * Hk[q];
+        expm_small(k, A, E);
+        f.assign(E.begin(), E.begin() + k);  // first column
+      };
+    }
+    std::complex<double> tau = rem;
+    std::vector<std::complex<double>> f;
+    for (int tries = 0; tries < 60; ++tries) {
+      expT_e1(tau, f);
+      double err = invariant ? 0.0 : b0 * bk * std::abs(f[k - 1]);
+      if (err <= tol * std::abs(tau)) break;
+      tau *= 0.5;
+    }
+    std::vector<double> c2(2 * k);
+    for (int i = 0; i < k; ++i) {
+      c2[2 * i] = b0 * f[i].real();
+      c2[2 * i + 1] = b0 * f[i].imag();
+    }
+    if (op->dtype == MH_F64)
+      for (int i = 0; i < k; ++i) c2[2 * i + 1] = 0.0;
+    MH_TRY(K.combine(k, c2.data(), xout));
+    src = xout;
+    done_t += tau;
+  }
+  if (info) *info = status;
+  return status == 0 ? MH_OK : fail(h, MH_ERR_NOCONV, "exponentiate: too many sub-steps");
+}
+
 }  // namespace mh
 
 using namespace mh;
@@ -710,86 +958,12 @@ int mh_bs_lanczos_eig(mh_handle h, mh_bs_plan plan, const void *p1, const void *
 
 int mh_lanczos_expm(mh_handle h, const mh_heff_op *op, const void *x0, double dt_re,
                     double dt_im, int kdim, double tol, void *xout, int *info) {
-  if (!h || !op) return MH_ERR_ARG;
-  MH_TRY(heff_check(h, *op));
-  if (!x0 || !xout) return fail(h, MH_ERR_ARG, "null vector");
-  if (kdim < 2 || kdim > 64) return fail(h, MH_ERR_ARG, "kdim must be in [2, 64]");
-  if (op->dtype == MH_F64 && dt_im != 0.0)
-    return fail(h, MH_ERR_DTYPE, "complex time step needs MH_C64 data");
-  if (!(tol > 0.0)) return fail(h, MH_ERR_ARG, "tol must be positive");
-  const std::complex<double> dt_total(dt_re, dt_im);
-  const double adt = std::abs(dt_total);
-  if (adt == 0.0) {
-    size_t n = heff_vec_len(*op);
-    MH_HIP(h, hipMemcpyAsync(xout, x0, n * dtype_bytes(op->dtype), hipMemcpyDeviceToDevice, h->stream));
-    if (info) *info = 0;
-    return MH_OK;
-  }
-  Krylov K(h, *op, kdim);
-  MH_TRY(K.setup());
-  StepGraph graph;
-  const size_t n = heff_vec_len(*op);
-  const void *src = x0;
-  std::complex<double> done_t(0.0, 0.0);
-  int nsub = 0, status = 0;
-  while (true) {
-    std::complex<double> rem = dt_total - done_t;
-    if (std::abs(rem) <= 1e-14 * adt) break;
-    if (++nsub > 4096) { status = -1; break; }
-    // Rebuild the Krylov state for this sub-step (st is cleared by setup()).
-    if (nsub > 1) {
-      hipLaunchKernelGGL(k_fill, dim3((K.lay.total + 255) / 256), dim3(256), 0, h->stream, K.st, K.lay.total, 0.0);
-    }
-    MH_TRY(K.start(src));
-    // Happy breakdown threshold relative to the vector norm 1.  Steps
-    // 1..kdim-1 are the same launches in every sub-step: one graph.
-    MH_TRY(K.step(0, 1e-13, 1, kdim > 1));
-    MH_TRY(run_steps(h, K, kdim, 1e-13, 1, graph));
-    std::vector<double> hs;
-    MH_TRY(K.read_state(hs));
-    const double b0 = hs[3];
-    if (!(b0 > 0.0)) {  // zero vector stays zero
-      MH_HIP(h, hipMemcpyAsync(xout, src, n * dtype_bytes(op->dtype), hipMemcpyDeviceToDevice, h->stream));
-      break;
-    }
-    int k = (int)hs[1];
-    if (k < 1) k = 1;
-    const bool invariant = hs[0] != 0.0;
-    std::vector<double> T((size_t)k * k, 0.0), ev(k), Z((size_t)k * k);
-    for (int i = 0; i < k; ++i) {
-      T[i + k * i] = hs[K.lay.o_alpha + i];
-      if (i + 1 < k) T[i + 1 + k * i] = T[i + k * (i + 1)] = hs[K.lay.o_beta + i];
-    }
-    sym_eig(k, T.data(), ev.data(), Z.data());
-    const double bk = hs[K.lay.o_beta + k - 1];
-    auto expT_e1 = [&](std::complex<double> tau, std::vector<std::complex<double>> &f) {
-      f.assign(k, 0.0);
-      for (int c = 0; c < k; ++c) {
-        std::complex<double> e = std::exp(tau * ev[c]) * Z[0 + k * c];
-        for (int r = 0; r < k; ++r) f[r] += Z[r + k * c] * e;
-      }
-    };
-    std::complex<double> tau = rem;
-    std::vector<std::complex<double>> f;
-    for (int tries = 0; tries < 60; ++tries) {
-      expT_e1(tau, f);
-      double err = invariant ? 0.0 : b0 * bk * std::abs(f[k - 1]);
-      if (err <= tol * std::abs(tau)) break;
-      tau *= 0.5;
-    }
-    std::vector<double> c2(2 * k);
-    for (int i = 0; i < k; ++i) {
-      c2[2 * i] = b0 * f[i].real();
-      c2[2 * i + 1] = b0 * f[i].imag();
-    }
-    if (op->dtype == MH_F64)
-      for (int i = 0; i < k; ++i) c2[2 * i + 1] = 0.0;
-    MH_TRY(K.combine(k, c2.data(), xout));
-    src = xout;
-    done_t += tau;
-  }
-  if (info) *info = status;
-  return status == 0 ? MH_OK : fail(h, MH_ERR_NOCONV, "exponentiate: too many sub-steps");
+  return krylov_expm(h, op, x0, dt_re, dt_im, kdim, tol, false, xout, info);
+}
+
+int mh_arnoldi_expm(mh_handle h, const mh_heff_op *op, const void *x0, double dt_re,
+                    double dt_im, int kdim, double tol, void *xout, int *info) {
+  return krylov_expm(h, op, x0, dt_re, dt_im, kdim, tol, true, xout, info);
 }
 
 }  // extern "C"

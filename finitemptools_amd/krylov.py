@@ -38,24 +38,27 @@ def eig_solver(A, x, maxiter, tol, reorth=False):
     return val, vec
 
 
-def exponentiate(A, dt, x, tol=1e-8, kdim=12):
-    """exp(dt * A) x with a Lanczos space of dimension <= kdim, sub-stepped so
-    that each sub-step's Krylov error estimate is below tol * |sub-step|."""
+def exponentiate(A, dt, x, tol=1e-8, kdim=12, ishermitian=True):
+    """exp(dt * A) x with a Krylov space of dimension <= kdim, sub-stepped so
+    that each sub-step's Krylov error estimate is below tol * |sub-step|:
+    Lanczos for a Hermitian A, Arnoldi otherwise (KrylovKit's choice behind
+    exponentiate(...; ishermitian), tdvp.jl:4)."""
     x = A.check_vector(x)
     dt = complex(dt)
     out = colmajor_empty(A.shape, A.dtype, x.device)
     e = A.engine
     h = e.bind_stream()
     info = ctypes.c_int()
-    e.check(e.lib.mh_lanczos_expm(h, ctypes.byref(A.op), ptr(x), dt.real, dt.imag, int(kdim),
-                                  float(tol), ptr(out), ctypes.byref(info)), "lanczos_expm")
+    fn = e.lib.mh_lanczos_expm if ishermitian else e.lib.mh_arnoldi_expm
+    e.check(fn(h, ctypes.byref(A.op), ptr(x), dt.real, dt.imag, int(kdim), float(tol), ptr(out),
+               ctypes.byref(info)), "lanczos_expm" if ishermitian else "arnoldi_expm")
     return out, info.value
 
 
-def exp_evolve(A, dt, x, tol):
+def exp_evolve(A, dt, x, tol, ishermitian=True):
     """``_exp_evolve(f, dt, x, isherm, tol)`` (tdvp.jl:3-6) passes tol/|dt| to
     KrylovKit as the accuracy per unit time; the engine takes the same."""
-    y, _ = exponentiate(A, dt, x, tol=tol / abs(dt) if dt != 0 else tol)
+    y, _ = exponentiate(A, dt, x, tol=tol / abs(dt) if dt != 0 else tol, ishermitian=ishermitian)
     return y
 
 

@@ -20,7 +20,9 @@ DEFAULT_D = 50         # Defaults.D
 
 
 def _exp(op, dt, x, alg):
-    y, _ = exponentiate(op, dt, x, tol=alg.exptol / abs(dt), kdim=alg.kdim)
+    """_exp_evolve (tdvp.jl:3-6): Lanczos when alg.ishermitian, else Arnoldi."""
+    y, _ = exponentiate(op, dt, x, tol=alg.exptol / abs(dt), kdim=alg.kdim,
+                        ishermitian=alg.ishermitian)
     return y
 
 
@@ -59,10 +61,12 @@ def _scale_rows(v, s):
 
 
 class TDVP2:
-    """tdvp.jl:71-79 (exptol 1e-8, default truncation)."""
+    """tdvp.jl:71-79 (exptol 1e-8, default truncation, ishermitian = false:
+    Arnoldi exponentiate unless the caller declares H Hermitian)."""
 
     def __init__(self, stepsize, exptol=TDVP_EXP_TOL, trunc=None, prec="f64", kdim=12, verbosity=0,
-                 split="rho"):
+                 split="rho", ishermitian=False):
+        self.ishermitian = bool(ishermitian)
         self.stepsize, self.exptol = complex(stepsize), exptol
         self.trunc = trunc or default_truncation()
         self.prec, self.kdim, self.verbosity = prec, kdim, verbosity
@@ -81,9 +85,12 @@ def _split2(two, alg, absorb):
 
 
 class TDVP1:
-    """tdvp.jl:8-17 (bond dimension D reached by zero-padding, increase_bond!)."""
+    """tdvp.jl:8-17 (bond dimension D reached by zero-padding, increase_bond!;
+    ishermitian = false: Arnoldi exponentiate)."""
 
-    def __init__(self, stepsize, D=DEFAULT_D, exptol=TDVP_EXP_TOL, prec="f64", kdim=12, verbosity=0):
+    def __init__(self, stepsize, D=DEFAULT_D, exptol=TDVP_EXP_TOL, prec="f64", kdim=12, verbosity=0,
+                 ishermitian=False):
+        self.ishermitian = bool(ishermitian)
         self.stepsize, self.exptol, self.D = complex(stepsize), exptol, D
         self.prec, self.kdim, self.verbosity = prec, kdim, verbosity
 
@@ -93,8 +100,9 @@ class TDVP1S:
     forward step (``expan``: OptimalExpansion, the default, or CHExpansion)."""
 
     def __init__(self, stepsize, exptol=TDVP_EXP_TOL, trunc=None, expan=None, prec="f64",
-                 kdim=12, verbosity=0):
+                 kdim=12, verbosity=0, ishermitian=False):
         from .expansion import OptimalExpansion
+        self.ishermitian = bool(ishermitian)
         self.stepsize, self.exptol = complex(stepsize), exptol
         self.trunc = trunc or default_truncation()
         self.expan = expan or OptimalExpansion()

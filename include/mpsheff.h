@@ -152,6 +152,15 @@ int mh_lanczos_expm(mh_handle h, const mh_heff_op *op, const void *x0,
                     double dt_re, double dt_im, int kdim, double tol,
                     void *xout, int *info);
 
+/* The same with an Arnoldi Krylov space (full FP64 classical Gram-Schmidt,
+ * two passes; the Hessenberg matrix exponentiated on the host by scaling and
+ * squaring of the [8/8] Pade approximant): any H_eff, Hermitian or not.
+ * Replaces exponentiate(...; ishermitian=false), KrylovKit's Arnoldi driver,
+ * the reference's default in algorithms/tdvp.jl:3-6,16,78,153. */
+int mh_arnoldi_expm(mh_handle h, const mh_heff_op *op, const void *x0,
+                    double dt_re, double dt_im, int kdim, double tol,
+                    void *xout, int *info);
+
 /* enable = 1 (default; environment MH_GRAPHS=0 turns it off): the Lanczos
  * steps after the first of mh_lanczos_eig / mh_bs_lanczos_eig /
  * mh_lanczos_expm are captured into one hipGraph and launched as a whole

@@ -79,3 +79,21 @@ def test_exponentiate_graph_matches_eager(gpu):
     (y0, i0), (y1, i1), (y2, i2) = res
     assert i0 == i1 == i2 == 0
     assert torch.equal(y0, y1) and torch.equal(y0, y2)
+
+
+def test_arnoldi_exponentiate_graph_matches_eager(gpu):
+    """The Arnoldi steps (Hessenberg columns in device memory) captured as a
+    graph give bitwise the eager result."""
+    from finitemptools_amd.derivatives import HeffOperator
+    from finitemptools_amd.krylov import exponentiate
+    rng = np.random.default_rng(6)
+    D, w = 20, 3
+    W = models.tfim_mpo(4, 1.0, 1.0, dtype=np.complex128)[1]
+    L = to_device(hermitian_env(rng, D, w, cplx=True, scale=1.0 / D))
+    R = to_device(hermitian_env(rng, D, w, cplx=True, scale=1.0 / D))
+    op = HeffOperator(2, L, R, to_device(W), to_device(W))
+    x = to_device(rng.standard_normal((D, 2, 2, D)) + 1j * rng.standard_normal((D, 2, 2, D)))
+    res = both(lambda: exponentiate(op, -1.0j, x, tol=1e-10, kdim=8, ishermitian=False))
+    (y0, i0), (y1, i1), (y2, i2) = res
+    assert i0 == i1 == i2 == 0
+    assert torch.equal(y0, y1) and torch.equal(y0, y2)

@@ -327,3 +327,48 @@ def test_fused_first_step_matches_unfused_and_oracle(gpu, kind, Wk, Dl, Dr):
     assert relerr(ys[0], ref) < TOL
     assert relerr(ys[1], ref) < TOL
     assert relerr(ys[0], ys[1]) < 1e-13
+
+
+@pytest.mark.parametrize("dt,cplx", [(-0.05j, True), (0.03, False), (-0.3j, True)])
+def test_arnoldi_expm_non_hermitian(gpu, dt, cplx):
+    """exp(dt H_eff) x for a NON-Hermitian H_eff (random MPO tensor and
+    environments) by the Arnoldi exponentiator, against scipy's expm of the
+    dense H_eff; the Lanczos exponentiator is not valid here and differs."""
+    import scipy.linalg as sla
+    from finitemptools_amd.derivatives import HeffOperator
+    from finitemptools_amd.krylov import exponentiate
+    rng = np.random.default_rng(13)
+    Dl, Dr, d, w = 4, 3, 2, 3
+    W = rand(rng, (w, d, w, d), cplx) * 0.5
+    L = rand(rng, (Dl, w, Dl), cplx) * 0.5
+    R = rand(rng, (Dr, w, Dr), cplx) * 0.5
+    x = rand(rng, (Dl, d, d, Dr), cplx)
+    H = C.dense_heff2(W, W, L, R)
+    assert np.linalg.norm(H - H.conj().T) > 0.1 * np.linalg.norm(H)
+    ref = sla.expm(dt * H) @ C.vec(x)
+    op = HeffOperator(2, dev(L), dev(R), dev(W), dev(W))
+    y, info = exponentiate(op, dt, dev(x), tol=1e-12, kdim=12, ishermitian=False)
+    assert info == 0
+    assert relerr(C.vec(host(y)), ref) < 1e-9
+    yl, _ = exponentiate(op, dt, dev(x), tol=1e-12, kdim=12, ishermitian=True)
+    assert relerr(C.vec(host(yl)), ref) > 1e-6
+
+
+def test_arnoldi_expm_equals_lanczos_on_hermitian(gpu):
+    """On a Hermitian H_eff both exponentiators give exp(dt H) x."""
+    import scipy.linalg as sla
+    from finitemptools_amd.derivatives import HeffOperator
+    from finitemptools_amd.krylov import exponentiate
+    rng = np.random.default_rng(21)
+    W = mpo_tensor("tfim", True)  # Hermitian blocks per channel: H_eff Hermitian
+    w = W.shape[0]
+    Dl, Dr, d = 5, 4, 2
+    L = hermitian_env(rng, Dl, w, True)
+    R = hermitian_env(rng, Dr, w, True)
+    x = rand(rng, (Dl, d, d, Dr), True)
+    op = HeffOperator(2, dev(L), dev(R), dev(W), dev(W))
+    ref = sla.expm(-0.1j * C.dense_heff2(W, W, L, R)) @ C.vec(x)
+    ya, _ = exponentiate(op, -0.1j, dev(x), tol=1e-12, kdim=10, ishermitian=False)
+    yl, _ = exponentiate(op, -0.1j, dev(x), tol=1e-12, kdim=10, ishermitian=True)
+    assert relerr(C.vec(host(ya)), ref) < 1e-9
+    assert relerr(C.vec(host(yl)), ref) < 1e-9
