@@ -71,6 +71,8 @@ SIGNATURES = {
     "mh_heff_apply": (c_int, [c_void_p, P(HeffOp), c_void_p, c_void_p]),
     "mh_lanczos_eig": (c_int, [c_void_p, P(HeffOp), c_void_p, c_int, c_double, c_int,
                                P(c_double), c_void_p, P(c_int), P(c_int)]),
+    "mh_lanczos_eig_proj": (c_int, [c_void_p, P(HeffOp), c_int, P(c_void_p), c_void_p, c_int,
+                                    c_double, c_int, P(c_double), c_void_p, P(c_int), P(c_int)]),
     "mh_lanczos_expm": (c_int, [c_void_p, P(HeffOp), c_void_p, c_double, c_double, c_int,
                                 c_double, c_void_p, P(c_int)]),
     "mh_arnoldi_expm": (c_int, [c_void_p, P(HeffOp), c_void_p, c_double, c_double, c_int,

@@ -137,9 +137,12 @@ __global__ void k_axpby(size_t n, double ar, double ai, const double *__restrict
 // st[8+2kmax .. 8+2kmax+2*(kmax+1)) combination coefficients (re, im),
 // then 2*MAXPART partials, (kmax+1)*MAXPART*2 multi-dot partials and the
 // Arnoldi Hessenberg columns H[j, i] (re, im) at o_h + 2*(j + (kmax+1)*i).
+// Projected operators (excited states): up to NPMAX projector vectors, their
+// multi-dot partials at o_ppart and coefficients at o_pcoef.
+constexpr int NPMAX = 8;
 struct LanLayout {
   int kmax;
-  int o_alpha, o_beta, o_coef, o_part, o_mpart, o_h, total;
+  int o_alpha, o_beta, o_coef, o_part, o_mpart, o_h, o_ppart, o_pcoef, total;
   explicit LanLayout(int k) : kmax(k) {
     o_alpha = 8;
     o_beta = o_alpha + k;
@@ -147,7 +150,9 @@ struct LanLayout {
     o_part = o_coef + 2 * (k + 1);
     o_mpart = o_part + 2 * MAXPART;
     o_h = o_mpart + 2 * MAXPART * (k + 1);
-    total = o_h + 2 * (k + 1) * k;
+    o_ppart = o_h + 2 * (k + 1) * k;
+    o_pcoef = o_ppart + 2 * MAXPART * NPMAX;
+    total = o_pcoef + 2 * NPMAX;
   }
 };
 
@@ -267,6 +272,13 @@ __global__ void __launch_bounds__(RB) k_mdot_reduce(const double *__restrict__ p
     coef[2 * j] = re;
     coef[2 * j + 1] = im;
   }
+}
+
+// coef_j -> conj(coef_j): the projection of dmrgexcited.jl:1-6 subtracts
+// dot(y, p) p with dot conjugate-linear in y, i.e. conj(<p, y>) p.
+__global__ void k_conj_coef(double *coef, int nv) {
+  const int j = threadIdx.x;
+  if (j < nv) coef[2 * j + 1] = -coef[2 * j + 1];
 }
 
 // Arnoldi: H[0..nv, i] = coef (pass 0) or += coef (pass 1), frozen once done.
@@ -497,6 +509,9 @@ struct Krylov {
   double *st;
   int G;
   bool arnoldi = false;  // full orthogonalisation, Hessenberg H (non-Hermitian H_eff)
+  int nproj = 0;                        // projectors applied after every H_eff x
+  const void *const *proj_src = nullptr;
+  double *P = nullptr;                  // their copies, stride nd
 
   Krylov(mh_handle hh, const mh_heff_op &o, int k)
       : h(hh), op(o), cplx(o.dtype == MH_C64), kmax(k), lay(k) {
@@ -512,8 +527,25 @@ struct Krylov {
   }
 
   int apply(const double *x, double *y) {
-    if (bs) return bs_apply_ws(h, *bs, x, bp1, bp2, y, heff_ws);
-    return heff_apply_ws(h, op, x, y, heff_ws, op.kind == MH_HEFF0 ? nullptr : Mt, true);
+    if (bs) MH_TRY(bs_apply_ws(h, *bs, x, bp1, bp2, y, heff_ws));
+    else MH_TRY(heff_apply_ws(h, op, x, y, heff_ws, op.kind == MH_HEFF0 ? nullptr : Mt, true));
+    return project(y);
+  }
+
+  // for l in order: y -= dot(y, p_l) p_l (dmrgexcited.jl:1-6), each with
+  // the y left by the previous projector, as the reference loops.
+  int project(double *y) {
+    double *pp = st + lay.o_ppart, *pc = st + lay.o_pcoef;
+    for (int l = 0; l < nproj; ++l) {
+      const double *pl = P + (size_t)l * nd;
+      if (cplx) hipLaunchKernelGGL(k_mdot_part<true>, dim3(G), dim3(RB), 0, h->stream, pl, n, 1, y, pp);
+      else hipLaunchKernelGGL(k_mdot_part<false>, dim3(G), dim3(RB), 0, h->stream, pl, n, 1, y, pp);
+      hipLaunchKernelGGL(k_mdot_reduce, dim3(1), dim3(RB), 0, h->stream, pp, G, pc);
+      if (cplx) hipLaunchKernelGGL(k_conj_coef, dim3(1), dim3(64), 0, h->stream, pc, 1);
+      if (cplx) hipLaunchKernelGGL(k_msub<true>, dim3(G), dim3(RB), 0, h->stream, n, 1, pl, pc, y, nullptr, nullptr);
+      else hipLaunchKernelGGL(k_msub<false>, dim3(G), dim3(RB), 0, h->stream, n, 1, pl, pc, y, nullptr, nullptr);
+    }
+    return nproj ? check_launch(h, "projection") : MH_OK;
   }
 
   int setup() {
@@ -523,11 +555,16 @@ struct Krylov {
     // index V_j at j * nd), then room for the MPO mode matrix.
     size_t vb = al(nd * sizeof(double) * (kmax + 1));
     size_t mtb = al(sizeof(double) * 2 * 4096 * 4);
-    MH_TRY(ensure_ws(h, hb + vb + mtb));
+    size_t pb = al(nd * sizeof(double) * nproj);
+    MH_TRY(ensure_ws(h, hb + vb + mtb + pb));
     heff_ws = (char *)h->ws;
     V = (double *)(heff_ws + hb);
     w = V + (size_t)kmax * nd;
     Mt = (char *)V + vb;
+    P = (double *)((char *)Mt + mtb);
+    for (int l = 0; l < nproj; ++l)
+      MH_HIP(h, hipMemcpyAsync(P + (size_t)l * nd, proj_src[l], nd * sizeof(double),
+                               hipMemcpyDeviceToDevice, h->stream));
     MH_TRY(ensure_red(h, lay.total));
     st = h->red;
     // Mixed forms: the environments are packed once for the whole solve.
@@ -550,9 +587,16 @@ struct Krylov {
     return check_launch(h, "nrm");
   }
 
-  // normalised start vector into V_0, norm into st[3]
+  // normalised start vector into V_0, norm into st[3] (projected first when
+  // the operator carries projectors, as the reference projects its start
+  // vector, dmrgexcited.jl:90)
   int start(const void *x0) {
     double *part = st + lay.o_part;
+    if (nproj) {
+      MH_HIP(h, hipMemcpyAsync(w, x0, nd * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      MH_TRY(project(w));
+      x0 = w;
+    }
     MH_TRY(launch_nrm((const double *)x0, part));
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RB), 0, h->stream, part, G, st + 3, 1);
     unsigned gb = (unsigned)((nd + 255) / 256);
@@ -941,6 +985,23 @@ int mh_lanczos_eig(mh_handle h, const mh_heff_op *op, const void *x0, int kmax,
   if (!x0 || !xout) return fail(h, MH_ERR_ARG, "null vector");
   if (kmax < 1 || kmax > 128) return fail(h, MH_ERR_ARG, "kmax must be in [1, 128]");
   Krylov K(h, *op, kmax);
+  return lanczos_eig_run(h, K, x0, kmax, btol, reorth, eigval, xout, info, niter);
+}
+
+int mh_lanczos_eig_proj(mh_handle h, const mh_heff_op *op, int nproj, const void *const *proj,
+                        const void *x0, int kmax, double btol, int reorth, double *eigval,
+                        void *xout, int *info, int *niter) {
+  if (!h || !op || !eigval) return MH_ERR_ARG;
+  MH_TRY(heff_check(h, *op));
+  if (!x0 || !xout) return fail(h, MH_ERR_ARG, "null vector");
+  if (kmax < 1 || kmax > 128) return fail(h, MH_ERR_ARG, "kmax must be in [1, 128]");
+  if (nproj < 0 || nproj > NPMAX) return fail(h, MH_ERR_ARG, "0 to %d projectors", NPMAX);
+  if (nproj > 0 && !proj) return fail(h, MH_ERR_ARG, "null projector list");
+  for (int l = 0; l < nproj; ++l)
+    if (!proj[l]) return fail(h, MH_ERR_ARG, "null projector %d", l);
+  Krylov K(h, *op, kmax);
+  K.nproj = nproj;
+  K.proj_src = proj;
   return lanczos_eig_run(h, K, x0, kmax, btol, reorth, eigval, xout, info, niter);
 }
 

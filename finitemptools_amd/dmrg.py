@@ -58,8 +58,11 @@ def _bond_step(m, alg, site, absorb):
     two = two_site(mps[site], mps[site + 1])
     op = HeffOperator(2, hs[site], hs[site + 2], mpo[site], mpo[site + 1], prec=alg.prec)
     t0 = time.perf_counter()
+    # excited states: the lower states through the overlap environments
+    # (dmrgexcited.jl:85-90); the solver projects the start vector once
+    proj = m.bond_projectors(site) if hasattr(m, "bond_projectors") else None
     ev, vecs, info = simple_lanczos_solver(op, two, "SR", alg.eig_maxiter, alg.eig_tol,
-                                           reorth=alg.reorth)
+                                           reorth=alg.reorth, projectors=proj)
     t1 = time.perf_counter()
     alg.stats["lanczos_s"] += t1 - t0
     alg.stats["matvec_flops"] += op.flops() * alg.eig_maxiter
@@ -141,7 +144,16 @@ def compute(m, alg, nsweeps=None):
 def ground_state(state, h, alg=None, nsweeps=None):
     """ground_state!(state, h, alg) (dmrg.jl:240); returns (energies, delta)."""
     alg = alg or DMRG2()
-    return compute(environments(h, state, alg.prec), alg, nsweeps)
+    return compute(environments(h, state, prec=alg.prec), alg, nsweeps)
+
+
+def excited_state(state, h, lower_states, alg=None, nsweeps=None):
+    """The lowest state orthogonal (through the projected solve) to
+    ``lower_states``: compute! on environments(h, state, projectors)
+    (dmrgexcited.jl, runtests.jl:494-535); returns (energies, delta)."""
+    alg = alg or DMRG2()
+    return compute(environments(h, state, projectors=list(lower_states), prec=alg.prec), alg,
+                   nsweeps)
 
 
 # ------------------------------------------------------------ one-site DMRG
@@ -189,8 +201,9 @@ def calc_galerkin(m, site, prec="f64"):
 
 def _site_eig(m, alg, site):
     op = HeffOperator(1, m.hstorage[site], m.hstorage[site + 1], m.mpo[site], prec=alg.prec)
+    proj = m.site_projectors(site) if hasattr(m, "site_projectors") else None  # dmrgexcited.jl:22-25
     ev, vecs, _ = simple_lanczos_solver(op, m.mps[site], "SR", alg.eig_maxiter, alg.eig_tol,
-                                        reorth=alg.reorth)
+                                        reorth=alg.reorth, projectors=proj)
     return ev, vecs
 
 

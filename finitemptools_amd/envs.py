@@ -61,8 +61,108 @@ class ExpectationCache:
         self.hstorage[site] = transfer.updateright(self.hstorage[site + 1], A, self.mpo[site], A, prec=self.prec)
 
 
-def environments(mpo, mps, prec="f64"):
-    """environments(mpo, mps) (finiteenv.jl:57)"""
+def overlap_updateleft(hold, psiA, psiB):
+    """hnew[b, b'] = conj(psiA[a,s,b]) hold[a,a'] psiB[a',s,b'] (mps/transfer.jl:23-31),
+    two engine GEMMs; psiA and psiB may have different bond dimensions."""
+    from .linalg import as_matrix, cm_view, gemm
+    Dla, d, Dra = psiA.shape
+    U = cm_view(gemm(hold, as_matrix(psiB, 1)), (Dla, d, psiB.shape[2]))       # [a, s, b']
+    return gemm(as_matrix(psiA, 2).t(), as_matrix(U, 2), conjA=True)           # [b, b']
+
+
+def overlap_updateright(hold, psiA, psiB):
+    """hnew[a, a'] = conj(psiA[a,s,b]) hold[b,b'] psiB[a',s,b'] (mps/transfer.jl:8-16)."""
+    from .linalg import as_matrix, cm_view, gemm
+    Dlb, d, Drb = psiB.shape
+    U = cm_view(gemm(as_matrix(psiB, 2), hold.t()), (Dlb, d, psiA.shape[2]))  # [a', s, b]
+    return gemm(as_matrix(psiA, 1), as_matrix(U, 1).t(), conjA=True)           # [a, a']
+
+
+def c_proj(x, cleft, cright):
+    """tmp[a,s,b] = cleft[a,x] x[x,s,y] cright[b,y] (derivatives.jl:65-67): a
+    lower state's site tensor seen through the overlap environments."""
+    from .linalg import as_matrix, cm_view, gemm
+    Dl, Dr = cleft.shape[0], cright.shape[0]
+    t = cm_view(gemm(cleft, as_matrix(x, 1)), (Dl, x.shape[1], x.shape[2]))
+    return cm_view(gemm(as_matrix(t, 2), cright.t()), (Dl, x.shape[1], Dr))
+
+
+def c2_proj(x1, x2, cleft, cright):
+    """tmp[a,s1,s2,b] = cleft[a,1] x1[1,s1,2] x2[2,s2,3] cright[b,3]
+    (_c2_prime_2, dmrgexcited.jl:70-72)."""
+    from .linalg import as_matrix, cm_view, gemm, two_site
+    x = two_site(x1, x2)
+    _, d1, d2, _ = x.shape
+    Dl, Dr = cleft.shape[0], cright.shape[0]
+    t = cm_view(gemm(cleft, as_matrix(x, 1)), (Dl, d1, d2, x.shape[3]))
+    return cm_view(gemm(as_matrix(t, 3), cright.t()), (Dl, d1, d2, Dr))
+
+
+class ProjectedExpectationCache(ExpectationCache):
+    """Environments of an excited-state search (envs/finiteenv.jl:97-168):
+    the MPO environments plus, for every lower state ("projector") l, the
+    overlap environments cstorages[l][i] between the state and l
+    (cstorages[l][i] is the left environment of site i, [i+1] the right)."""
+
+    def __init__(self, mpo, mps, projectors, prec="f64"):
+        self.projectors = list(projectors)
+        for p in self.projectors:
+            if len(p) != len(mps):
+                raise _lib_dm("projector length differs from the state")
+        self.cstorages = [[None] * (len(mps) + 1) for _ in self.projectors]
+        super().__init__(mpo, mps, prec)
+
+    @property
+    def cenvs(self):
+        return self.cstorages
+
+    def init_hstorage(self, center=0):
+        """init_hstorage_right + init_cstorage_right! (finiteenv.jl:97-117): the
+        overlap boundaries (l_LL, r_RR), then the base class's passes, whose
+        updateright / updateleft calls (below) fill both kinds of slots."""
+        L = len(self.mps)
+        dt, dev = self.mps[0].dtype, self.mps[0].device
+        for cs in self.cstorages:
+            cs[0] = torch.ones((1, 1), dtype=dt, device=dev)   # l_LL
+            cs[L] = torch.ones((1, 1), dtype=dt, device=dev)   # r_RR
+        super().init_hstorage(center)
+
+    def updateleft(self, site):
+        """updateleft! (finiteenv.jl:151-156)"""
+        super().updateleft(site)
+        for l, cs in enumerate(self.cstorages):
+            cs[site + 1] = overlap_updateleft(cs[site], self.mps[site], self.projectors[l][site])
+
+    def updateright(self, site):
+        """updateright! (finiteenv.jl:159-164)"""
+        super().updateright(site)
+        for l, cs in enumerate(self.cstorages):
+            cs[site] = overlap_updateright(cs[site + 1], self.mps[site], self.projectors[l][site])
+
+    def site_projectors(self, site):
+        """[c_proj(projectors[l][site], cstorages[l][site], cstorages[l][site+1])]"""
+        return [c_proj(self.projectors[l][site], cs[site], cs[site + 1])
+                for l, cs in enumerate(self.cstorages)]
+
+    def bond_projectors(self, site):
+        """[_c2_prime_2(projectors[l][site], projectors[l][site+1], cstorages[l][site],
+        cstorages[l][site+2])]"""
+        return [c2_proj(self.projectors[l][site], self.projectors[l][site + 1], cs[site], cs[site + 2])
+                for l, cs in enumerate(self.cstorages)]
+
+
+def _lib_dm(msg):
+    from . import _lib
+    return _lib.DimensionMismatch(2, msg)
+
+
+def environments(mpo, mps, projectors=None, prec="f64"):
+    """environments(mpo, mps) (finiteenv.jl:57) and, with lower states,
+    environments(mpo, mps, projectors) (finiteenv.jl:119-120)."""
+    if isinstance(projectors, str):   # environments(mpo, mps, prec)
+        projectors, prec = None, projectors
+    if projectors:
+        return ProjectedExpectationCache(mpo, mps, projectors, prec)
     return ExpectationCache(mpo, mps, prec)
 
 

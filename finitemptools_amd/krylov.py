@@ -11,10 +11,14 @@ from . import _lib
 from .engine import as_colmajor, colmajor_empty, dtype_code, ptr
 
 
-def simple_lanczos_solver(A, v, wh="SR", kmax=10, btol=1.0e-8, reorth=False, verbosity=0):
+def simple_lanczos_solver(A, v, wh="SR", kmax=10, btol=1.0e-8, reorth=False, verbosity=0,
+                          projectors=None):
     """Lowest eigenpair of the operator ``A`` (an HeffOperator) started from ``v``.
     Returns (eigenvalue, eigenvector, info) with info = 0 when beta < btol stopped
-    the recurrence and -1 when kmax steps ran (simple_lanczos.jl:108-113)."""
+    the recurrence and -1 when kmax steps ran (simple_lanczos.jl:108-113).
+    ``projectors``: vectors p_l of v's shape; the operator becomes
+    x -> P(A x) and the start vector P(v), with P(y): y -= dot(y, p_l) p_l in
+    order (the excited-state solve, dmrgexcited.jl:1-6)."""
     if wh not in ("SR", "SA"):
         raise ValueError("only the lowest eigenpair ('SR'/'SA') is supported")
     v = A.check_vector(v)
@@ -24,9 +28,17 @@ def simple_lanczos_solver(A, v, wh="SR", kmax=10, btol=1.0e-8, reorth=False, ver
     ev = ctypes.c_double()
     info = ctypes.c_int()
     nit = ctypes.c_int()
-    e.check(e.lib.mh_lanczos_eig(h, ctypes.byref(A.op), ptr(v), int(kmax), float(btol),
-                                 1 if reorth else 0, ctypes.byref(ev), ptr(out),
-                                 ctypes.byref(info), ctypes.byref(nit)), "lanczos_eig")
+    if projectors:
+        ps = [A.check_vector(p) for p in projectors]
+        arr = (ctypes.c_void_p * len(ps))(*[ctypes.c_void_p(p.data_ptr()) for p in ps])
+        e.check(e.lib.mh_lanczos_eig_proj(h, ctypes.byref(A.op), len(ps), arr, ptr(v), int(kmax),
+                                          float(btol), 1 if reorth else 0, ctypes.byref(ev),
+                                          ptr(out), ctypes.byref(info), ctypes.byref(nit)),
+                "lanczos_eig_proj")
+    else:
+        e.check(e.lib.mh_lanczos_eig(h, ctypes.byref(A.op), ptr(v), int(kmax), float(btol),
+                                     1 if reorth else 0, ctypes.byref(ev), ptr(out),
+                                     ctypes.byref(info), ctypes.byref(nit)), "lanczos_eig")
     if verbosity >= 2:
         print(f"lanczos: {nit.value} steps, info {info.value}")
     return ev.value, out, info.value

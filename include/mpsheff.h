@@ -144,6 +144,16 @@ int mh_lanczos_eig(mh_handle h, const mh_heff_op *op, const void *x0, int kmax,
                    double btol, int reorth, double *eigval, void *xout,
                    int *info, int *niter);
 
+/* mh_lanczos_eig on the projected operator x -> P(H_eff x) with
+ * P(y): for l = 0..nproj-1 in order, y -= dot(y, p_l) p_l (dot conjugate-
+ * linear in y), applied to the start vector too: the excited-state solve of
+ * algorithms/dmrgexcited.jl:1-6,22-25,86-90 (projector vectors p_l are the
+ * lower states contracted with the overlap environments, c_proj,
+ * derivatives.jl:65-67).  proj: nproj <= 8 device vectors of x's size. */
+int mh_lanczos_eig_proj(mh_handle h, const mh_heff_op *op, int nproj, const void *const *proj,
+                        const void *x0, int kmax, double btol, int reorth, double *eigval,
+                        void *xout, int *info, int *niter);
+
 /* xout = exp(dt * H_eff) x0 by a Lanczos (Hermitian H_eff) Krylov space of
  * dimension <= kdim, repeated in sub-steps until the Krylov error estimate of
  * each sub-step is below tol.  Replaces _exp_evolve -> KrylovKit.exponentiate,
