@@ -475,6 +475,19 @@ int mh_ac2_prime(mh_handle h, int dtype, int prec, int Dl, int d1, int d2,
   return mh_heff_apply(h, &op, x, y);
 }
 
+int mh_mode_product(mh_handle h, int dtype, int P, int ni, int nj, int Q, const void *in,
+                    const void *Mt, void *out) {
+  if (!h) return MH_ERR_ARG;
+  if (dtype != MH_F64 && dtype != MH_C64) return fail(h, MH_ERR_DTYPE, "unknown dtype %d", dtype);
+  if (P < 1 || Q < 1 || ni < 1 || nj < 1 || ni > 4096 || nj > 4096)
+    return fail(h, MH_ERR_SHAPE, "mode product extents P=%d ni=%d nj=%d Q=%d", P, ni, nj, Q);
+  if ((int64_t)P * ni * Q >= ((int64_t)1 << 31) || (int64_t)P * nj * Q >= ((int64_t)1 << 31))
+    return fail(h, MH_ERR_SHAPE, "mode product exceeds 2^31 elements");
+  if (!in || !Mt || !out) return fail(h, MH_ERR_ARG, "null tensor");
+  if (in == out) return fail(h, MH_ERR_ARG, "in and out must not alias");
+  return mode_product(h, dtype, P, ni, nj, Q, in, (int64_t)P * ni, Mt, out, (int64_t)P * nj);
+}
+
 int mh_ac_prime_open(mh_handle h, int dtype, int Dl, int d, int Dr, int wl, int wr,
                      const void *x, const void *W, const void *L, void *P) {
   if (!h) return MH_ERR_ARG;

@@ -84,6 +84,14 @@ int mh_ac2_prime(mh_handle h, int dtype, int prec, int Dl, int d1, int d2,
                  int Dr, int wl, int wm, int wr, const void *x, const void *W1,
                  const void *W2, const void *L, const void *R, void *y);
 
+/* out[p, j, q] = sum_i in[p, i, q] Mt[i, j]: a small matrix on the middle
+ * index of a 3-index tensor (in: P x ni x Q, out: P x nj x Q, Mt: ni x nj,
+ * all column-major on the device; ni, nj <= 4096).  With Mt = G^T it applies
+ * a gate G[s', s] to the physical legs of a one-, two- or three-site tensor
+ * (bond_evolution, _apply_impl, circuit/apply_gates.jl:9-31,74-79). */
+int mh_mode_product(mh_handle h, int dtype, int P, int ni, int nj, int Q, const void *in,
+                    const void *Mt, void *out);
+
 /* The right bond left open: P[a,s',w',b'] = sum L[a,w,a'] x[a',s,b'] W[w,s',w',s]
  * (FP64 / C64 arithmetic), i.e. ac_prime before its last contraction with R.
  * P: (Dl, d, wr, Dr) = the CH subspace-expansion term L.A.W with (w', b')
