@@ -66,10 +66,16 @@ struct Stage {
 // not depend on the order the two adds land (only used for beta == 0).
 // NW = 4 or 8 waves per block (2 x NW/2 wave grid): eight waves give two
 // waves per SIMD to GEMMs with few output tiles (one block per CU).
-template <bool CPLX, int BM, int BN, int BK, bool AMC, bool BKC, bool VEC, int NW = 4>
+// KG = 2 (with NW = 8, BK a multiple of 32): the waves form two 2x2 grids;
+// grid g computes the 16-deep K slices with (slice index % 2) == g of every
+// staged step, so each wave keeps a 2x-larger tile (more MFMAs per LDS
+// fragment) while two waves still share each SIMD; the second grid's tile is
+// added to the first's through LDS after the K loop (a fixed order).
+template <bool CPLX, int BM, int BN, int BK, bool AMC, bool BKC, bool VEC, int NW = 4, int KG = 1>
 __global__ void __launch_bounds__(64 * NW)
     k_gemm(const GemmP p) {
-  constexpr int WGM = 2, WGN = NW / 2;
+  static_assert(KG == 1 || (KG == 2 && NW == 8 && BK % 32 == 0), "K-grouped waves");
+  constexpr int WGM = 2, WGN = NW / (2 * KG);
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   constexpr int NP = CPLX ? 2 : 1;
@@ -82,7 +88,8 @@ __global__ void __launch_bounds__(64 * NW)
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WGM, wn = wave / WGM;
+  const int kgrp = wave / (NW / KG), wv = wave % (NW / KG);
+  const int wm = wv % WGM, wn = wv / WGM;
   int tm, tn;
   if (!tile_of_block(p, tm, tn)) return;  // whole block: padding of the grouped grid
   const int m0 = tm * BM, n0 = tn * BN;
@@ -240,6 +247,7 @@ __global__ void __launch_bounds__(64 * NW)
   auto compute = [&](int buf) {
 #pragma unroll
     for (int kb = 0; kb < BK; kb += 16) {
+      if (KG == 2 && ((kb / 16) & 1) != kgrp) continue;
       double a[NP][TM][4], b[NP][TN][4];
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
@@ -305,6 +313,37 @@ __global__ void __launch_bounds__(64 * NW)
     compute((nk - 1) & 1);
   }
 
+  if constexpr (KG == 2) {
+    // grid 1 hands its tile to grid 0 through LDS (the staging buffers are
+    // free once every wave is past its last compute)
+    constexpr int NACC = NP * TM * TN * 4;
+    static_assert((size_t)NACC * 256 <= sizeof(sA) / sizeof(double), "LDS for the K-group reduction");
+    double *red = &sA[0][0][0];
+    __syncthreads();
+    if (kgrp == 1) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              red[((((q * TM + i) * TN + j) * 4 + r) * 4 + wv) * 64 + lane] = acc[q][i][j][r];
+    }
+    __syncthreads();
+    if (kgrp == 1) return;
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            acc[q][i][j][r] += red[((((q * TM + i) * TN + j) * 4 + r) * 4 + wv) * 64 + lane];
+  }
+
   const bool split = gridDim.z > 1;
   const bool has_beta = (p.beta_re != 0.0 || p.beta_im != 0.0);
 #pragma unroll
@@ -360,6 +399,9 @@ static const Cand kCands[] = {
     // output tile per CU
     {64, 64, 16, 1, 0, 8}, {64, 64, 32, 1, 0, 8}, {128, 64, 16, 1, 0, 8},
     {64, 64, 16, 1, 8, 8}, {64, 64, 32, 2, 0, 8},
+    // eight waves as two K-slice groups of a 2x2 wave grid
+    {64, 64, 32, 1, 0, 8, 2}, {64, 64, 32, 1, 8, 8, 2}, {128, 64, 32, 1, 0, 8, 2},
+    {64, 64, 64, 1, 0, 8, 2},
 };
 constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
 
@@ -370,7 +412,15 @@ static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
   hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC>), grid, dim3(256), 0, s, p)
 #define MH_L8(BM, BN, BK) \
   hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC, 8>), grid, dim3(512), 0, s, p)
-  if (c.waves == 8) {
+#define MH_L8K(BM, BN, BK) \
+  hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC, 8, 2>), grid, dim3(512), 0, s, p)
+  if (c.kgroups == 2) {
+    if constexpr (!CPLX) {  // real only (LDS and registers)
+      if (c.bm == 128) MH_L8K(128, 64, 32);
+      else if (c.bk == 64) MH_L8K(64, 64, 64);
+      else MH_L8K(64, 64, 32);
+    }
+  } else if (c.waves == 8) {
     if (c.bm == 128) MH_L8(128, 64, 16);
     else if (c.bk == 32) MH_L8(64, 64, 32);
     else MH_L8(64, 64, 16);
@@ -382,6 +432,7 @@ static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
   else if constexpr (!CPLX) MH_L(128, 128, 16);  // complex 128x128 is never admitted
 #undef MH_L
 #undef MH_L8
+#undef MH_L8K
 }
 
 template <bool CPLX, bool VEC>
@@ -419,6 +470,7 @@ static bool cand_ok_f64(const Cand &c, const GemmP &p, bool cplx) {
   if (c.split > 1 && (p.beta_re != 0.0 || p.beta_im != 0.0)) return false;
   if (c.split > 1 && p.K < 256 * c.split) return false;
   if (cplx && c.bk == 32) return false;              // LDS: keep <= 80 KB
+  if (cplx && c.kgroups == 2) return false;
   if (cplx && c.bm * c.bn > 64 * 128) return false;  // register pressure
   return true;
 }
@@ -778,9 +830,9 @@ static int tune(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx, b
     MH_HIP(h, hipEventElapsedTime(&ms, e0, e1));
     ms /= 3.f;
     if (h->tune_log)
-      fprintf(stderr, "[mh tune] %s%s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d waves=%d: %.1f us (%.2f TFLOP/s)\n",
+      fprintf(stderr, "[mh tune] %s%s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d waves=%d kgroups=%d: %.1f us (%.2f TFLOP/s)\n",
               fam.name, cplx ? "/c" : "", p.M, p.N, p.K, (int)amc, (int)bkc, c.bm, c.bn, c.bk,
-              c.split, c.group, c.waves ? c.waves : 4, ms * 1e3,
+              c.split, c.group, c.waves ? c.waves : 4, c.kgroups ? c.kgroups : 1, ms * 1e3,
               2.0 * p.M * (double)p.N * p.K * (cplx ? 4 : 1) / (ms * 1e-3) / 1e12);
     if (ms < best) {
       best = ms;
@@ -805,6 +857,15 @@ int gemm_dispatch(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx,
   snprintf(key, sizeof(key), "%s:%d:%d:%d:%d:%d%d:%d:%d", tag, cplx ? 1 : 0, p.M, p.N, p.K,
            (int)amc, (int)bkc, (p.beta_re != 0.0 || p.beta_im != 0.0) ? 1 : 0, p.c_f32);
   int choice = -1;
+  // a forced configuration (tests), if this family has it and admits it
+  if (h->gemm_force >= 0 && fam.cands == kCands && h->gemm_force < fam.n &&
+      fam.ok(fam.cands[h->gemm_force], p, cplx)) {
+    int slot;
+    prof_begin(h, MH_K_GEMM, flops, &slot);
+    int st = run_cand(h, fam, fam.cands[h->gemm_force], p, cplx, amc, bkc);
+    prof_end(h, slot);
+    return st;
+  }
   auto it = h->tune.find(key);
   if (it != h->tune.end()) {
     choice = it->second;
@@ -828,6 +889,19 @@ int gemm_dispatch(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx,
 }
 
 }  // namespace mh
+
+extern "C" int mh_gemm_candidates(int *n) {
+  if (!n) return MH_ERR_ARG;
+  *n = mh::kNCand;
+  return MH_OK;
+}
+
+extern "C" int mh_gemm_force_candidate(mh_handle h, int cand) {
+  if (!h) return MH_ERR_ARG;
+  if (cand < -1 || cand >= mh::kNCand) return mh::fail(h, MH_ERR_ARG, "no GEMM configuration %d", cand);
+  h->gemm_force = cand;
+  return MH_OK;
+}
 
 extern "C" int mh_gemm(mh_handle h, int dtype, int prec, int M, int N, int K,
                        const void *A, int64_t sam, int64_t sak, int conjA,

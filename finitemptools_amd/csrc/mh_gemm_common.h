@@ -38,6 +38,8 @@ struct GemmP {
 struct Cand {
   int bm, bn, bk, split, group;
   int waves;  // waves per block; 0 means the family default (4)
+  int kgroups;  // 2: the 8 waves form two 2x2 wave grids that take alternate
+                // 16-deep K slices and add their tiles at the end (0/1: off)
 };
 
 struct GemmFamily {

@@ -56,6 +56,7 @@ struct mh_context {
   int fusion = 1;
   hipStream_t cap_stream = nullptr;
   int autotune = 1;
+  int gemm_force = -1;  // FP64 GEMM configuration forced for every call (-1: tuned)
   int tune_log = 0;
 };
 

@@ -372,3 +372,36 @@ def test_arnoldi_expm_equals_lanczos_on_hermitian(gpu):
     yl, _ = exponentiate(op, -0.1j, dev(x), tol=1e-12, kdim=10, ishermitian=True)
     assert relerr(C.vec(host(ya)), ref) < 1e-9
     assert relerr(C.vec(host(yl)), ref) < 1e-9
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_every_gemm_configuration(gpu, cplx):
+    """Each FP64 GEMM configuration the tuner may choose (tile, K depth,
+    split-K, tile order, waves, K-grouped waves), forced in turn, on ragged
+    shapes in the layouts of the two H_eff steps (C = A B with A m-contiguous
+    and B k- or n-contiguous), against numpy; configurations a shape does not
+    admit fall back to tuning."""
+    import ctypes
+    import torch
+    from finitemptools_amd.engine import engine, to_device
+    from finitemptools_amd.linalg import gemm
+    eng = engine()
+    n = ctypes.c_int()
+    eng.check(eng.lib.mh_gemm_candidates(ctypes.byref(n)))
+    rng = np.random.default_rng(17)
+    dt = torch.complex128 if cplx else torch.float64
+    shapes = [(130, 70, 600), (64, 64, 1100), (200, 90, 40)]
+    try:
+        for c in range(n.value):
+            eng.check(eng.lib.mh_gemm_force_candidate(eng.h, c))
+            for M, N, K in shapes:
+                A = rand(rng, (M, K), cplx)
+                B = rand(rng, (K, N), cplx)
+                Bt = rand(rng, (N, K), cplx)
+                Ad = to_device(A, "cuda", dt)
+                C1 = gemm(Ad, to_device(B, "cuda", dt)).cpu().numpy()
+                C2 = gemm(Ad, to_device(Bt, "cuda", dt).t()).cpu().numpy()
+                assert relerr(C1, A @ B) < 1e-13, (c, M, N, K)
+                assert relerr(C2, A @ Bt.T) < 1e-13, (c, M, N, K)
+    finally:
+        eng.check(eng.lib.mh_gemm_force_candidate(eng.h, -1))
