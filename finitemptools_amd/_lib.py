@@ -81,7 +81,7 @@ SIGNATURES = {
                                 c_double, c_void_p, P(c_int)]),
     "mh_set_graphs": (c_int, [c_void_p, c_int]),
     "mh_set_fusion": (c_int, [c_void_p, c_int]),
-    "mh_gemm_candidates": (c_int, [P(c_int)]),
+    "mh_gemm_candidates": (c_int, [c_int, P(c_int)]),
     "mh_gemm_force_candidate": (c_int, [c_void_p, c_int]),
     "mh_dot": (c_int, [c_void_p, c_int, c_size_t, c_void_p, c_void_p, P(c_double)]),
     "mh_nrm2": (c_int, [c_void_p, c_int, c_size_t, c_void_p, P(c_double)]),

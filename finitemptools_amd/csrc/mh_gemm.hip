@@ -858,8 +858,7 @@ int gemm_dispatch(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx,
            (int)amc, (int)bkc, (p.beta_re != 0.0 || p.beta_im != 0.0) ? 1 : 0, p.c_f32);
   int choice = -1;
   // a forced configuration (tests), if this family has it and admits it
-  if (h->gemm_force >= 0 && fam.cands == kCands && h->gemm_force < fam.n &&
-      fam.ok(fam.cands[h->gemm_force], p, cplx)) {
+  if (h->gemm_force >= 0 && h->gemm_force < fam.n && fam.ok(fam.cands[h->gemm_force], p, cplx)) {
     int slot;
     prof_begin(h, MH_K_GEMM, flops, &slot);
     int st = run_cand(h, fam, fam.cands[h->gemm_force], p, cplx, amc, bkc);
@@ -890,15 +889,21 @@ int gemm_dispatch(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx,
 
 }  // namespace mh
 
-extern "C" int mh_gemm_candidates(int *n) {
+namespace mh {
+const GemmFamily &mixed_family(int prec);
+}
+
+extern "C" int mh_gemm_candidates(int prec, int *n) {
   if (!n) return MH_ERR_ARG;
-  *n = mh::kNCand;
+  if (prec == MH_PREC_F64) *n = mh::kNCand;
+  else if (prec == MH_PREC_F32 || prec == MH_PREC_BF16X3) *n = mh::mixed_family(prec).n;
+  else return MH_ERR_ARG;
   return MH_OK;
 }
 
 extern "C" int mh_gemm_force_candidate(mh_handle h, int cand) {
   if (!h) return MH_ERR_ARG;
-  if (cand < -1 || cand >= mh::kNCand) return mh::fail(h, MH_ERR_ARG, "no GEMM configuration %d", cand);
+  if (cand < -1 || cand >= 64) return mh::fail(h, MH_ERR_ARG, "no GEMM configuration %d", cand);
   h->gemm_force = cand;
   return MH_OK;
 }

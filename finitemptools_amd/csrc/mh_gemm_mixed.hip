@@ -62,16 +62,16 @@ __global__ void k_pack(const double *__restrict__ src, uint32_t *__restrict__ ds
   if (i < n) dst[i] = pack_word(src[i], PM);
 }
 
-template <int PM, bool CPLX, int BR, bool RMAJ>
+template <int PM, bool CPLX, int BR, bool RMAJ, int NT = 256>
 struct MxStage {
   static constexpr int BK = 32;
   static constexpr int NC = CPLX ? 2 : 1;        // components
   static constexpr int NS = PM == PM_BF16 ? 2 : 1;  // planes per component (hi, lo)
   static constexpr int PITCH = PM == PM_BF16 ? BK + 8 : BK + 4;  // LDS elements per row
-  static constexpr int NPAIR = BR * BK / 512;
+  static constexpr int NPAIR = BR * BK / (2 * NT);
   static_assert(NPAIR >= 1, "tile");
   __device__ static void coord(int t, int i, int &r, int &k) {
-    const int e = t + 256 * i;
+    const int e = t + NT * i;
     if (RMAJ) {
       r = e % BR;
       k = 2 * (e / BR);
@@ -82,16 +82,19 @@ struct MxStage {
   }
 };
 
-template <int PM, bool CPLX, int BM, int BN, bool AMC, bool BKC, bool VEC, bool OUTF32>
-__global__ void __launch_bounds__(256)
+// NW = 4 or 8 waves (2 x NW/2 wave grid); eight give two waves per SIMD to
+// GEMMs with about one output tile per CU.
+template <int PM, bool CPLX, int BM, int BN, bool AMC, bool BKC, bool VEC, bool OUTF32, int NW = 4>
+__global__ void __launch_bounds__(64 * NW)
     k_gemm_mx(const GemmP p) {
   constexpr int BK = 32;
-  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int WTM = BM / 2, WTN = BN / (NW / 2);
   constexpr int TM = WTM / 16, TN = WTN / 16;
+  static_assert(TM >= 1 && TN >= 1, "tile");
   constexpr int NC = CPLX ? 2 : 1;
   constexpr int NS = PM == PM_BF16 ? 2 : 1;
-  using SA = MxStage<PM, CPLX, BM, AMC>;
-  using SB = MxStage<PM, CPLX, BN, !BKC>;
+  using SA = MxStage<PM, CPLX, BM, AMC, 64 * NW>;
+  using SB = MxStage<PM, CPLX, BN, !BKC, 64 * NW>;
   using E = typename std::conditional<PM == PM_BF16, uint16_t, float>::type;
   constexpr int PITCH = SA::PITCH;
   __shared__ __align__(16) E sA[NC * NS][2][BM * PITCH];
@@ -390,6 +393,9 @@ static const Cand kMxCands[] = {
     {64, 64, 32, 1, 0}, {128, 64, 32, 1, 0}, {64, 128, 32, 1, 0}, {128, 128, 32, 1, 0},
     {64, 64, 32, 2, 0}, {128, 64, 32, 2, 0}, {128, 128, 32, 2, 0},
     {64, 64, 32, 1, 8}, {128, 64, 32, 1, 8}, {128, 128, 32, 1, 8},
+    // eight waves per block
+    {64, 64, 32, 1, 0, 8}, {64, 64, 32, 2, 0, 8}, {128, 64, 32, 1, 0, 8}, {128, 64, 32, 2, 0, 8},
+    {64, 64, 32, 1, 8, 8},
 };
 constexpr int kNMx = sizeof(kMxCands) / sizeof(kMxCands[0]);
 
@@ -403,11 +409,23 @@ static void launch_mx_cfg(const Cand &c, GemmP p, hipStream_t s) {
     else                                                                                  \
       hipLaunchKernelGGL((k_gemm_mx<PM, CPLX, BM, BN, AMC, BKC, VEC, false>), grid, dim3(256), 0, s, p); \
   } while (0)
-  if (c.bm == 64 && c.bn == 64) MH_L(64, 64);
+#define MH_L8(BM, BN)                                                                     \
+  do {                                                                                    \
+    if (p.c_f32)                                                                          \
+      hipLaunchKernelGGL((k_gemm_mx<PM, CPLX, BM, BN, AMC, BKC, VEC, true, 8>), grid, dim3(512), 0, s, p); \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_gemm_mx<PM, CPLX, BM, BN, AMC, BKC, VEC, false, 8>), grid, dim3(512), 0, s, p); \
+  } while (0)
+  if (c.waves == 8) {
+    if (c.bm == 128) MH_L8(128, 64);
+    else MH_L8(64, 64);
+  }
+  else if (c.bm == 64 && c.bn == 64) MH_L(64, 64);
   else if (c.bm == 128 && c.bn == 64) MH_L(128, 64);
   else if (c.bm == 64 && c.bn == 128) MH_L(64, 128);
   else if constexpr (!CPLX) MH_L(128, 128);
 #undef MH_L
+#undef MH_L8
 }
 
 // 8-byte pair loads along k need an even K, an even stride along rows and an

@@ -189,11 +189,12 @@ int mh_set_graphs(mh_handle h, int enable);
  * followed by a separate mode-product launch instead of the fused kernel
  * (default on; environment MH_FUSION=0). */
 int mh_set_fusion(mh_handle h, int enable);
-/* FP64 GEMM configurations (tile shape, K depth, split-K, tile order, waves)
- * the per-shape tuner chooses from; force_candidate fixes one for every
- * FP64 GEMM of the handle that admits it (-1: back to tuning).  Used by the
- * tests to check each configuration. */
-int mh_gemm_candidates(int *n);
+/* GEMM configurations (tile shape, K depth, split-K, tile order, waves) the
+ * per-shape tuner of each precision chooses from; force_candidate fixes
+ * configuration `cand` of the calling precision's list for every GEMM of the
+ * handle that admits it (-1: back to tuning).  Used by the tests to check
+ * each configuration. */
+int mh_gemm_candidates(int prec, int *n);
 int mh_gemm_force_candidate(mh_handle h, int cand);
 
 /* ---- Krylov vector primitives (simple_lanczos.jl:28-38,50-63) ------------ */

@@ -387,7 +387,7 @@ def test_every_gemm_configuration(gpu, cplx):
     from finitemptools_amd.linalg import gemm
     eng = engine()
     n = ctypes.c_int()
-    eng.check(eng.lib.mh_gemm_candidates(ctypes.byref(n)))
+    eng.check(eng.lib.mh_gemm_candidates(0, ctypes.byref(n)))
     rng = np.random.default_rng(17)
     dt = torch.complex128 if cplx else torch.float64
     shapes = [(130, 70, 600), (64, 64, 1100), (200, 90, 40)]

@@ -97,3 +97,33 @@ def test_mixed_benchmark_size(gpu, prec):
     x = rng.standard_normal((D, 2, 2, D))
     op = HeffOperator(2, dev(L), dev(R), dev(W), dev(W), prec=prec)
     assert relerr(host(op(dev(x))), C.ac2_prime(x, W, W, L, R)) < TOLS[prec]
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_every_mixed_gemm_configuration(gpu, prec, cplx):
+    """Each configuration of the mixed families (tile, split-K, tile order,
+    four or eight waves), forced in turn, within the precision's tolerance;
+    configurations a shape does not admit fall back to tuning."""
+    import torch
+    rng = np.random.default_rng(9)
+    n = ctypes.c_int()
+    gpu.check(gpu.lib.mh_gemm_candidates(PRECS[prec], ctypes.byref(n)))
+    h = gpu.bind_stream()
+    try:
+        for c in range(n.value):
+            gpu.check(gpu.lib.mh_gemm_force_candidate(h, c))
+            for M, N, K in [(130, 70, 1100), (64, 192, 40)]:
+                A, B = rand(rng, (M, K), cplx), rand(rng, (K, N), cplx)
+                Ad, Bd = dev(A), dev(np.ascontiguousarray(B.T)).t()
+                Cd = dev(np.zeros((M, N), dtype=A.dtype))
+                st = gpu.lib.mh_gemm(h, 1 if cplx else 0, PRECS[prec], M, N, K,
+                                     ctypes.c_void_p(Ad.data_ptr()), Ad.stride(0), Ad.stride(1), 0,
+                                     ctypes.c_void_p(Bd.data_ptr()), Bd.stride(0), Bd.stride(1),
+                                     ctypes.c_void_p(Cd.data_ptr()), Cd.stride(0), Cd.stride(1),
+                                     1.0, 0.0, 0.0, 0.0)
+                gpu.check(st)
+                torch.cuda.synchronize()
+                assert relerr(host(Cd), A @ B) < TOLS[prec], (c, M, N, K)
+    finally:
+        gpu.check(gpu.lib.mh_gemm_force_candidate(h, -1))
