@@ -25,6 +25,13 @@ namespace mh {
 namespace {
 
 constexpr int MAXQ = 4;
+constexpr int QLIM = 16384;  // |charge component| < QLIM: sums of two stay in 16 bits
+
+bool charges_ok(const int32_t *q, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (q[i] <= -QLIM || q[i] >= QLIM) return false;
+  return true;
+}
 
 // A charge packed into 64 bits (16 bits per component, offset binary).
 inline int64_t qkey(const int32_t *q, int nq) {
@@ -103,6 +110,8 @@ int mk_space(mh_handle h, const mh_bs_sectors *s, int nq, Sp &out) {
   if (!s || s->n < 0 || (s->n > 0 && (!s->charges || !s->dims)))
     return fail(h, MH_ERR_ARG, "bad sector list");
   if (s->nq != nq) return fail(h, MH_ERR_ARG, "charge components differ");
+  if (!charges_ok(s->charges, (size_t)s->n * nq))
+    return fail(h, MH_ERR_ARG, "sector charges must lie in (-%d, %d)", QLIM, QLIM);
   out.q.resize(s->n);
   out.dim.resize(s->n);
   for (int i = 0; i < s->n; ++i) {
@@ -272,8 +281,16 @@ int finish(mh_handle h, int prec, const int64_t sizes[6], const std::vector<int>
 
 int read_common(mh_handle h, int d, const int32_t *phys, int nq, std::vector<int64_t> &ph) {
   if (d <= 0 || !phys) return fail(h, MH_ERR_ARG, "no physical charges");
+  if (!charges_ok(phys, (size_t)d * nq))
+    return fail(h, MH_ERR_ARG, "physical charges must lie in (-%d, %d)", QLIM, QLIM);
   ph.resize(d);
   for (int s = 0; s < d; ++s) ph[s] = qkey(phys + (size_t)s * nq, nq);
+  return MH_OK;
+}
+
+int check_qw(mh_handle h, const int32_t *q, int w, int nq) {
+  if (!charges_ok(q, (size_t)w * nq))
+    return fail(h, MH_ERR_ARG, "MPO channel charges must lie in (-%d, %d)", QLIM, QLIM);
   return MH_OK;
 }
 
@@ -303,6 +320,8 @@ int mh_bs_plan_heff2(mh_handle h, int prec, const mh_bs_sectors *left,
   MH_TRY(mk_space(h, left, nq, Sl));
   MH_TRY(mk_space(h, right, nq, Sr));
   MH_TRY(read_common(h, d, phys, nq, ph));
+  MH_TRY(check_qw(h, qwl, wl, nq));
+  MH_TRY(check_qw(h, qwr, wr, nq));
   Lay x;
   mk_layout(Sl, Sr, ph, 2, x, nq);
   Env Le, Re;
@@ -394,6 +413,8 @@ int mh_bs_plan_updateleft(mh_handle h, int prec, const mh_bs_sectors *left,
   MH_TRY(mk_space(h, left, nq, Sl));
   MH_TRY(mk_space(h, right, nq, Sr));
   MH_TRY(read_common(h, d, phys, nq, ph));
+  MH_TRY(check_qw(h, qwl, wl, nq));
+  MH_TRY(check_qw(h, qwr, wr, nq));
   Lay al;
   mk_layout(Sl, Sr, ph, 1, al, nq);
   Env Le, Ln;
@@ -467,6 +488,8 @@ int mh_bs_plan_updateright(mh_handle h, int prec, const mh_bs_sectors *left,
   MH_TRY(mk_space(h, left, nq, Sl));
   MH_TRY(mk_space(h, right, nq, Sr));
   MH_TRY(read_common(h, d, phys, nq, ph));
+  MH_TRY(check_qw(h, qwl, wl, nq));
+  MH_TRY(check_qw(h, qwr, wr, nq));
   Lay al;
   mk_layout(Sl, Sr, ph, 1, al, nq);
   Env Re, Rn;

@@ -297,7 +297,7 @@ int mh_bs_apply(mh_handle h, mh_bs_plan plan, const void *x, const void *p1, con
  *     D_q x K_q column-major matrix.
  * These are the layouts of the Python planner (blocksparse.py), which lists
  * the sectors sorted by charge; both produce the same tables.
- * charges: nq ints per sector (nq = 1..4 U(1) factors, |q| < 32768).
+ * charges: nq ints per sector (nq = 1..4 U(1) factors, |q| < 16384).
  * phys: d charges of the physical index (nq ints each).  qwl / qwr: charge
  * of every MPO channel on the left / right MPO bond (nq ints each); MPO
  * tensors W[wl, s_out, wr, s_in] column-major FP64.  Errors: MH_ERR_ARG for

@@ -352,3 +352,20 @@ def test_native_planner_rejects_charge_breaking_mpo():
         bs.host_native_tables(kind, xl, Lay, Ray, bad, W2, prec)
     with pytest.raises(ValueError):
         bs.plan_heff2(xl, Lay, Ray, bad, W2)
+
+
+def test_native_planner_rejects_out_of_range_charges():
+    """Charge components are packed 16 bits each: sectors, physical charges
+    or MPO channel charges outside (-16384, 16384) are refused rather than
+    wrapped into another sector."""
+    from finitemptools_amd import _lib
+    _, kind, args = NATIVE_CASES[7]     # xxz heff2 at an inner bond, U(1) charges
+    xl, Lay, Ray, W1, W2, prec = args
+    big = 20000
+    shifted = bs.Space([((q[0] + big,), d) for q, d in xl.lspace.sectors])
+    xbad = bs.BlockLayout(shifted, xl.phys, 2, xl.rspace)
+    with pytest.raises(_lib.MHError):
+        bs.host_native_tables(kind, xbad, Lay, Ray, W1, W2, prec)
+    Lbad = bs.EnvLayout(xl.lspace, [(q[0] + big,) for q in Lay.qw])
+    with pytest.raises(_lib.MHError):
+        bs.host_native_tables(kind, xl, Lbad, Ray, W1, W2, prec)
