@@ -588,6 +588,276 @@ int mode_product_packed(mh_handle h, int dtype, int prec, int P, int ni, int nj,
   return check_launch(h, "mode_product_packed");
 }
 
+// ------------------------------------------------ fused first H_eff step, packed operands
+//
+// Packed-operand form of the fused FP64 step (k_heff_step1, mh_gemm.hip), for
+// FP64 real operators:
+//   T1[a, w, s, b'] = sum_a' L[a, w, a'] x[a', s, b']    (MFMA, FP32 accumulators)
+//   T2[a, j, b']    = sum_i T1[a, i, b'] Mt[i, j]         (FP32, epilogue)
+// with L, x and T2 packed words of the family's precision (i = w + WL*s,
+// j = s' + DS*w').  T1 never leaves the block and is not rounded to packed
+// words (the unfused path rounds it once more).  A block owns a = a0..a0+31
+// for every channel w (BM = 32*WL rows: row r is a0 + r%32 of channel r/32)
+// and 64 columns c = s + DS*b'; four waves 2 x 2, BK = 32, double-buffered
+// k-major LDS staging as in k_gemm_mx.  The epilogue stages the 64-column
+// tile in LDS ([column][row], pitch BM + 4, so an accumulator's four rows are
+// one conflict-free 16-byte write), then each thread forms T2 for 2-4 (a, b')
+// pairs with Mt read from LDS as FP32.  Needs Dl even (8-byte x pairs).
+template <int PM, int WL, int DS, int WR>
+__global__ void __launch_bounds__(256)
+    k_heff_step1_mx(const uint32_t *__restrict__ L, const uint32_t *__restrict__ X,
+                    const double *__restrict__ Mt, uint32_t *__restrict__ T2, int Dl, int Dr,
+                    int tiles_a, int tiles_b, int xcd_chunk, int group) {
+  constexpr int BA = 32, BM = BA * WL, BN = 64, BK = 32;
+  constexpr int NI = WL * DS, NJ = DS * WR;
+  constexpr int WTM = BM / 2, WTN = BN / 2, TM = WTM / 16, TN = WTN / 16;
+  constexpr int BB = BN / DS;  // b' values per block
+  constexpr int NPQ = BA * BB / 256;  // (a, b') pairs per thread
+  static_assert(BN % DS == 0 && (BA * BB) % 256 == 0 && WTM % 16 == 0, "tile");
+  using SA = MxStage<PM, false, BM, true>;   // L rows (contiguous along a), [r][k]
+  using SB = MxStage<PM, false, BN, false>;  // x columns (contiguous along a'), [c][k]
+  constexpr int NS = PM == PM_BF16 ? 2 : 1;
+  using E = typename std::conditional<PM == PM_BF16, uint16_t, float>::type;
+  constexpr int PITCH = SA::PITCH;
+  constexpr int SZA = BM * PITCH, SZB = BN * PITCH;  // elements per plane and buffer
+  constexpr int STAGE_BYTES = NS * 2 * (SZA + SZB) * (int)sizeof(E);
+  constexpr int CP = BM + 4;
+  constexpr int EPI_BYTES = BN * CP * (int)sizeof(float);
+  constexpr int SMEM = STAGE_BYTES > EPI_BYTES ? STAGE_BYTES : EPI_BYTES;
+  __shared__ __align__(16) char smem[SMEM];
+  __shared__ __align__(16) float sM[NI * NJ];
+  E *const sA = reinterpret_cast<E *>(smem);  // [plane][buf][BM * PITCH]
+  E *const sB = sA + NS * 2 * SZA;            // [plane][buf][BN * PITCH]
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int local = blockIdx.x >> 3;
+  const int t = (blockIdx.x & 7) * xcd_chunk + local;
+  if (local >= xcd_chunk || t >= tiles_a * tiles_b) return;
+  const int span = group * tiles_b, first = (t / span) * group;
+  const int gsz = min(tiles_a - first, group), wt = t % span;
+  const int a0 = (first + wt % gsz) * BA, c0 = (wt / gsz) * BN;
+  const int K = Dl, NC = DS * Dr;
+  const int64_t ldL = (int64_t)Dl * WL;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  for (int e = tid; e < NI * NJ; e += 256) sM[e] = (float)Mt[e];
+
+  int64_t aoff[SA::NPAIR], boff[SB::NPAIR];
+  bool aok[SA::NPAIR], bok[SB::NPAIR];
+#pragma unroll
+  for (int i = 0; i < SA::NPAIR; ++i) {
+    int r, k;
+    SA::coord(tid, i, r, k);
+    const int a = a0 + r % BA, w = r / BA;
+    aok[i] = a < Dl;
+    aoff[i] = (int64_t)min(a, Dl - 1) + (int64_t)Dl * w;
+  }
+#pragma unroll
+  for (int i = 0; i < SB::NPAIR; ++i) {
+    int r, k;
+    SB::coord(tid, i, r, k);
+    const int c = c0 + r;
+    bok[i] = c < NC;
+    boff[i] = (int64_t)min(c, NC - 1) * Dl;
+  }
+
+  uint32_t ra[SA::NPAIR][2], rb[SB::NPAIR][2];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int kc = min(k0 + k + e, K - 1);
+        ra[i][e] = L[aoff[i] + (int64_t)kc * ldL];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      const int kc = min(k0 + k, K - 2);  // even: k0, k even and K even
+      const uint2 v = *reinterpret_cast<const uint2 *>(X + boff[i] + kc);
+      rb[i][0] = v.x;
+      rb[i][1] = v.y;
+    }
+  };
+  auto put = [&](E *dst, int buf, int sz, int off, uint32_t w0, uint32_t w1) {
+    if constexpr (PM == PM_BF16) {
+      *reinterpret_cast<uint32_t *>(&dst[buf * sz + off]) = (w0 & 0xffffu) | (w1 << 16);
+      *reinterpret_cast<uint32_t *>(&dst[(2 + buf) * sz + off]) = (w0 >> 16) | (w1 & 0xffff0000u);
+    } else {
+      *reinterpret_cast<uint2 *>(&dst[buf * sz + off]) = make_uint2(w0, w1);
+    }
+  };
+  auto store = [&](int buf, int k0) {
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      const uint32_t w0 = (aok[i] && k0 + k < K) ? ra[i][0] : 0u;
+      const uint32_t w1 = (aok[i] && k0 + k + 1 < K) ? ra[i][1] : 0u;
+      put(sA, buf, SZA, r * PITCH + k, w0, w1);
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      const uint32_t w0 = (bok[i] && k0 + k < K) ? rb[i][0] : 0u;
+      const uint32_t w1 = (bok[i] && k0 + k + 1 < K) ? rb[i][1] : 0u;
+      put(sB, buf, SZB, r * PITCH + k, w0, w1);
+    }
+  };
+
+  f4_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  auto compute = [&](int buf) {
+    const E *A = sA + buf * SZA;
+    const E *B = sB + buf * SZB;
+    if constexpr (PM == PM_F32) {
+#pragma unroll
+      for (int kb = 0; kb < BK; kb += 16) {
+        f4_t a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          a[i] = *reinterpret_cast<const f4_t *>(&A[(wm * WTM + i * 16 + lr) * PITCH + kb + 4 * lk]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          b[j] = *reinterpret_cast<const f4_t *>(&B[(wn * WTN + j * 16 + lr) * PITCH + kb + 4 * lk]);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+      }
+    } else {
+      bf8_t ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int o = (wm * WTM + i * 16 + lr) * PITCH + 8 * lk;
+        ah[i] = *reinterpret_cast<const bf8_t *>(&A[o]);
+        al[i] = *reinterpret_cast<const bf8_t *>(&A[2 * SZA + o]);
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int o = (wn * WTN + j * 16 + lr) * PITCH + 8 * lk;
+        bh[j] = *reinterpret_cast<const bf8_t *>(&B[o]);
+        bl[j] = *reinterpret_cast<const bf8_t *>(&B[2 * SZB + o]);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  load(0);
+  store(0, 0);
+  __syncthreads();
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int k1 = (kt + 1) * BK;
+    load(k1);
+    compute(kt & 1);
+    store((kt + 1) & 1, k1);
+    __syncthreads();
+  }
+  compute((nk - 1) & 1);
+
+  // Epilogue.  f32 accumulator layout: register r of lane l is
+  // C[4*(l/16) + r][l%16], so a lane's four values are four adjacent rows.
+  __syncthreads();  // every wave is done with the staging buffers
+  float *const Cs = reinterpret_cast<float *>(smem);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int m = wm * WTM + i * 16 + 4 * lk;
+      const int n = wn * WTN + j * 16 + lr;
+      *reinterpret_cast<f4_t *>(&Cs[n * CP + m]) = acc[i][j];
+    }
+  __syncthreads();
+  const int bq0 = c0 / DS;  // first b' of the block
+  float v[NPQ][NI];
+#pragma unroll
+  for (int q = 0; q < NPQ; ++q) {
+    const int pr = tid + 256 * q;
+    const int al = pr % BA, bl = pr / BA;
+#pragma unroll
+    for (int s = 0; s < DS; ++s)
+#pragma unroll
+      for (int w = 0; w < WL; ++w) v[q][w + WL * s] = Cs[(bl * DS + s) * CP + al + BA * w];
+  }
+  const int a = a0 + (tid % BA);
+#pragma unroll 1
+  for (int j = 0; j < NJ; ++j) {
+    float o[NPQ];
+#pragma unroll
+    for (int q = 0; q < NPQ; ++q) o[q] = 0.f;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const float m = sM[i + NI * j];
+#pragma unroll
+      for (int q = 0; q < NPQ; ++q) o[q] = __builtin_fmaf(v[q][i], m, o[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < NPQ; ++q) {
+      const int bq = bq0 + (tid + 256 * q) / BA;
+      if (a < Dl && bq < Dr) T2[a + (int64_t)Dl * (j + (int64_t)NJ * bq)] = pack_word_f(o[q], PM);
+    }
+  }
+}
+
+template <int PM, int WL, int DS>
+static void launch_step1_mx(int Dl, int Dr, const uint32_t *L, const uint32_t *X,
+                            const double *Mt, uint32_t *T2, hipStream_t s) {
+  const int tiles_a = (Dl + 31) / 32, tiles_b = (DS * Dr + 63) / 64;
+  const int chunk = (tiles_a * tiles_b + 7) / 8;
+  const int group = min(tiles_a, 8);
+  hipLaunchKernelGGL((k_heff_step1_mx<PM, WL, DS, WL>), dim3(8 * chunk), dim3(256), 0, s, L, X,
+                     Mt, T2, Dl, Dr, tiles_a, tiles_b, chunk, group);
+}
+
+// Packed T2 = mode(L x) in one launch (FP64 real operators, packed L and x,
+// wl = wr in {3,4,5,6,8}, physical extent 2 or 4, Dl even);
+// MH_ERR_UNSUPPORTED otherwise (the caller runs GEMM + packed mode product).
+int heff_step1_fused_mx(mh_handle h, int prec, int Dl, int Dr, int wl, int ds, int wr,
+                        const void *L, const void *x, const void *Mt, void *T2) {
+  auto al8 = [](const void *q) { return ((uintptr_t)q & 7) == 0; };
+  if (prec != MH_PREC_F32 && prec != MH_PREC_BF16X3) return MH_ERR_UNSUPPORTED;
+  if (wl != wr || Dl < 2 || Dl % 2 != 0 || !al8(x)) return MH_ERR_UNSUPPORTED;
+  const uint32_t *Lp = (const uint32_t *)L, *Xp = (const uint32_t *)x;
+  const double *Mp = (const double *)Mt;
+  uint32_t *Tp = (uint32_t *)T2;
+  hipStream_t s = h->stream;
+  const double flops = 2.0 * Dl * (double)wl * ds * Dr * Dl;
+  int slot;
+#define MH_S1(W, D)                                                        \
+  if (wl == W && ds == D) {                                                \
+    prof_begin(h, MH_K_GEMM, flops, &slot);                                \
+    if (prec == MH_PREC_F32) launch_step1_mx<PM_F32, W, D>(Dl, Dr, Lp, Xp, Mp, Tp, s);  \
+    else launch_step1_mx<PM_BF16, W, D>(Dl, Dr, Lp, Xp, Mp, Tp, s);        \
+    prof_end(h, slot);                                                     \
+    return check_launch(h, "fused packed H_eff step 1");                   \
+  }
+  MH_S1(5, 4) MH_S1(5, 2) MH_S1(3, 4) MH_S1(3, 2) MH_S1(4, 4) MH_S1(4, 2)
+  MH_S1(6, 4) MH_S1(6, 2) MH_S1(8, 4) MH_S1(8, 2)
+#undef MH_S1
+  return MH_ERR_UNSUPPORTED;
+}
+
 // GEMM on operands that are already packed; c_packed selects packed words
 // (1) or FP64 (0) for C.
 int gemm_packed(mh_handle h, int dtype, int prec, const GemmDesc &g, int c_packed) {

@@ -255,7 +255,9 @@ int heff_prepare_mixed(mh_handle h, const mh_heff_op &op, void *ws) {
 
 // Mixed forms of H_eff.x: x is packed, step 1 writes packed T1, the mode
 // product reads and writes packed words (FP32 arithmetic), step 3 writes the
-// FP64 result.  L, R must already be packed (heff_prepare_mixed).
+// FP64 result.  For FP64 real operators of the supported shapes the first
+// two run as one launch (heff_step1_fused_mx), with T1 kept in FP32 inside
+// the block.  L, R must already be packed (heff_prepare_mixed).
 static int heff_apply_mixed(mh_handle h, const mh_heff_op &op, const void *x, void *y, void *ws,
                             const void *Mt_prebuilt) {
   const MixedLayout lay(op);
@@ -278,10 +280,18 @@ static int heff_apply_mixed(mh_handle h, const mh_heff_op &op, const void *x, vo
   if (!Mt_prebuilt)
     MH_TRY(build_mode_matrix(h, op.dtype, op.kind == MH_HEFF2 ? MODE_TWO : MODE_SITE, wl, op.wm,
                              wr, d1, op.d2, op.W1, op.W2, Mt));
-  GemmDesc g1 = gd(Dl * wl, ds * Dr, Dl, Lp, 1, (int64_t)Dl * wl, 0, xp, 1, Dl, T1, 1, (int64_t)Dl * wl);
-  MH_TRY(gemm_packed(h, op.dtype, op.prec, g1, 1));
-  MH_TRY(mode_product_packed(h, op.dtype, op.prec, Dl, ni, nj, Dr, T1, (int64_t)Dl * ni, Mt, T2,
-                             (int64_t)Dl * nj));
+  int fused = MH_ERR_UNSUPPORTED;
+  if (h->fusion && op.dtype == MH_F64) {
+    fused = heff_step1_fused_mx(h, op.prec, Dl, Dr, wl, ds, wr, Lp, xp, Mt, T2);
+    if (fused != MH_OK && fused != MH_ERR_UNSUPPORTED) return fused;
+  }
+  if (fused != MH_OK) {
+    GemmDesc g1 = gd(Dl * wl, ds * Dr, Dl, Lp, 1, (int64_t)Dl * wl, 0, xp, 1, Dl, T1, 1,
+                     (int64_t)Dl * wl);
+    MH_TRY(gemm_packed(h, op.dtype, op.prec, g1, 1));
+    MH_TRY(mode_product_packed(h, op.dtype, op.prec, Dl, ni, nj, Dr, T1, (int64_t)Dl * ni, Mt,
+                               T2, (int64_t)Dl * nj));
+  }
   GemmDesc g3 = gd(Dl * ds, Dr, wr * Dr, T2, 1, (int64_t)Dl * ds, 0, Rp, Dr, 1, y, 1, (int64_t)Dl * ds);
   return gemm_packed(h, op.dtype, op.prec, g3, 0);
 }

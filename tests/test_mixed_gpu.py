@@ -127,3 +127,54 @@ def test_every_mixed_gemm_configuration(gpu, prec, cplx):
                 assert relerr(host(Cd), A @ B) < TOLS[prec], (c, M, N, K)
     finally:
         gpu.check(gpu.lib.mh_gemm_force_candidate(h, -1))
+
+
+FUSED_MX_CASES = [  # (kind, (w, d) or model, Dl, Dr): shapes the packed fused first step takes
+    (2, "xxz", 2, 2), (2, "xxz", 34, 50), (2, "xxz", 96, 130), (2, (4, 2), 40, 66),
+    (2, (8, 2), 36, 20), (2, (3, 2), 64, 64), (1, "hubbard", 50, 64), (1, (8, 4), 32, 33),
+    (1, (6, 2), 130, 70), (1, (3, 4), 2, 7),
+]
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("kind,Wk,Dl,Dr", FUSED_MX_CASES)
+def test_fused_mixed_first_step(gpu, prec, kind, Wk, Dl, Dr):
+    """The packed fused GEMM + mode-product kernel (FP64 real operators,
+    wl = wr in {3,4,5,6,8}, physical extent 2 or 4, even Dl) and the
+    two-launch packed path, each against the FP64 oracle within the
+    precision's tolerance; ragged Dl, Dr off the 32 x 64/ds block tile."""
+    from finitemptools_amd import models
+    from finitemptools_amd.derivatives import ac2_prime, ac_prime
+    from finitemptools_amd.engine import engine
+    rng = np.random.default_rng(Dl * 37 + Dr)
+    if Wk == "xxz":
+        W = models.heisenberg_xxz_mpo(4, Jzz=0.9, hz=0.1)[1]
+    elif Wk == "hubbard":
+        W = models.hubbard_mpo(4, 1.0, 4.0)[1]
+    else:
+        w, d = Wk
+        W = rand(rng, (w, d, w, d))
+    w, d = W.shape[0], W.shape[1]
+    L = rand(rng, (Dl, w, Dl)) / np.sqrt(Dl * w)
+    R = rand(rng, (Dr, w, Dr)) / np.sqrt(Dr * w)
+    x = rand(rng, (Dl, d, d, Dr) if kind == 2 else (Dl, d, Dr))
+    ref = C.ac2_prime(x, W, W, L, R) if kind == 2 else C.ac_prime(x, W, L, R)
+    eng = engine()
+    errs, modes = [], []
+    eng.profile_enable(True)
+    for fused in (True, False):
+        eng.set_fusion(fused)
+        try:
+            eng.profile_reset()
+            if kind == 2:
+                y = ac2_prime(dev(x), dev(W), dev(W), dev(L), dev(R), prec=prec)
+            else:
+                y = ac_prime(dev(x), dev(W), dev(L), dev(R), prec=prec)
+            errs.append(relerr(host(y), ref))
+            modes.append(eng.profile_read(1)[1])   # separate mode-product launches
+        finally:
+            eng.set_fusion(True)
+    eng.profile_enable(False)
+    assert modes == [0, 1], modes
+    assert errs[0] < TOLS[prec], errs
+    assert errs[1] < TOLS[prec], errs
