@@ -80,6 +80,7 @@ SIGNATURES = {
     "mh_arnoldi_expm": (c_int, [c_void_p, P(HeffOp), c_void_p, c_double, c_double, c_int,
                                 c_double, c_void_p, P(c_int)]),
     "mh_set_graphs": (c_int, [c_void_p, c_int]),
+    "mh_set_krylov_small": (c_int, [c_void_p, c_int]),
     "mh_set_fusion": (c_int, [c_void_p, c_int]),
     "mh_gemm_candidates": (c_int, [c_int, P(c_int)]),
     "mh_gemm_force_candidate": (c_int, [c_void_p, c_int]),

@@ -54,6 +54,8 @@ struct mh_context {
   int graphs = 1;
   // Fused first H_eff step (GEMM with the mode product in its epilogue).
   int fusion = 1;
+  // Krylov steps on short vectors as one single-workgroup launch.
+  int krylov_small = 1;
   hipStream_t cap_stream = nullptr;
   int autotune = 1;
   int gemm_force = -1;  // FP64 GEMM configuration forced for every call (-1: tuned)

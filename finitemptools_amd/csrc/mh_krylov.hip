@@ -357,6 +357,213 @@ __global__ void k_combine(size_t n, int nv, const double *__restrict__ V,
   }
 }
 
+// ------------------------------------------------ one-workgroup Krylov step
+// For short vectors (at most SMALL_ND doubles) the vector part of a Lanczos
+// or Arnoldi step -- alpha, the three-term update, the Gram-Schmidt passes,
+// beta and the next basis vector -- is ONE launch of one 1024-thread
+// workgroup instead of ten to fourteen launches, which at these sizes cost
+// launch latency, not bandwidth (sixteen waves keep enough loads in flight
+// to cover the L2 latency of each pass).  Same recurrence, state layout and
+// early-exit rules as the multi-block kernels above.  Every thread owns the
+// vector elements e = t, t + 1024, ... in every pass, so w needs no barrier
+// between passes; sums are per thread in index order, then a fixed
+// LDS/shuffle tree, so results are reproducible run to run.
+constexpr int SB_T = 1024;
+constexpr int SB_W = SB_T / 64;
+constexpr size_t SMALL_ND = 16384;
+constexpr int SB_J = 8;      // basis vectors per multi-dot batch
+constexpr int SB_VMAX = 128; // basis vectors per step (kmax limit)
+
+// out[q] = workgroup-wide sum of v[q], q < NV (a shared array), for NV = 1,
+// 2 (wave shuffles, then the SB_W wave sums in order) or NV = 2*SB_J (the
+// partials transposed through LDS: SB_T/NV consecutive threads per value add
+// NV entries each, then shuffle within their group).  sh holds NV*SB_T doubles.
+template <int NV>
+__device__ __forceinline__ void sb_sum(double (&v)[NV], double *sh, double *out) {
+  const int t = threadIdx.x;
+  if constexpr (NV <= 2) {
+    const int lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double x = v[q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+      if (lane == 0) sh[wv * NV + q] = x;
+    }
+    __syncthreads();
+    if (t < NV) {
+      double x = 0.0;
+      for (int k = 0; k < SB_W; ++k) x += sh[k * NV + t];
+      out[t] = x;
+    }
+  } else {
+    constexpr int TPV = SB_T / NV;  // threads per value (64 for NV = 16)
+    static_assert(SB_T % NV == 0 && TPV <= 64 && (TPV & (TPV - 1)) == 0, "reduction shape");
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[q * SB_T + t] = v[q];
+    __syncthreads();
+    const int q = t / TPV, r = t % TPV;
+    double x = 0.0;
+#pragma unroll
+    for (int k = 0; k < SB_T / TPV; ++k) x += sh[q * SB_T + r + k * TPV];
+#pragma unroll
+    for (int o = TPV / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if (r == 0) out[q] = x;
+  }
+  __syncthreads();
+}
+
+// mode 0: Lanczos step; 1: Lanczos step with two Gram-Schmidt passes
+// against V_0..V_i; 2: Arnoldi step (the passes give the Hessenberg column
+// H[0..i, i] at st[o_h + 2*(j + ldh*i)]).  next (may be null) receives
+// V_{i+1}.  n counts scalars (complex: pairs of doubles).
+template <bool CPLX>
+__global__ void __launch_bounds__(SB_T)
+    k_step_small(size_t n, const double *__restrict__ V, double *__restrict__ w,
+                 double *__restrict__ next, double *__restrict__ st, int o_alpha, int o_beta,
+                 int o_h, int ldh, int i, int mode, double btol) {
+  __shared__ double sh[2 * SB_J * SB_T];
+  __shared__ double res[2 * SB_J];
+  __shared__ double coef[2 * SB_VMAX];
+  __shared__ double scal[4];
+  const int t = threadIdx.x;
+  const size_t nd = CPLX ? 2 * n : n;
+  const bool done = st[0] != 0.0;  // written (by thread 0) only after the last read below
+  const double *vi = V + (size_t)i * nd;
+  double s2 = 0.0;
+  if (mode != 2) {
+    double p[2] = {0.0, 0.0};
+    for (size_t e = t; e < n; e += SB_T) {
+      if (CPLX) {
+        const double2 a = reinterpret_cast<const double2 *>(vi)[e];
+        const double2 b = reinterpret_cast<const double2 *>(w)[e];
+        p[0] += a.x * b.x + a.y * b.y;
+        p[1] += a.x * b.y - a.y * b.x;
+      } else {
+        p[0] += vi[e] * w[e];
+      }
+    }
+    sb_sum<2>(p, sh, res);
+    if (t == 0) {
+      double a = st[o_alpha + i];
+      if (!done) {
+        a = res[0];
+        st[o_alpha + i] = a;
+        const double ai = fabs(res[1]);
+        if (ai > st[2]) st[2] = ai;
+      }
+      scal[0] = a;
+      scal[1] = i > 0 ? st[o_beta + i - 1] : 0.0;
+    }
+    __syncthreads();
+    const double a = scal[0], b = scal[1];
+    const double *vm = i > 0 ? V + (size_t)(i - 1) * nd : vi;
+    for (size_t e = t; e < nd; e += SB_T) {
+      double v = w[e] - a * vi[e];
+      if (i > 0) v -= b * vm[e];
+      w[e] = v;
+      s2 += v * v;
+    }
+  }
+  if (mode >= 1) {
+    const int nv = i + 1;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int j0 = 0; j0 < nv; j0 += SB_J) {
+        double p[2 * SB_J];
+#pragma unroll
+        for (int q = 0; q < 2 * SB_J; ++q) p[q] = 0.0;
+        for (size_t e = t; e < n; e += SB_T) {
+          if (CPLX) {
+            const double2 b = reinterpret_cast<const double2 *>(w)[e];
+#pragma unroll
+            for (int q = 0; q < SB_J; ++q)
+              if (j0 + q < nv) {
+                const double2 a = reinterpret_cast<const double2 *>(V + (size_t)(j0 + q) * nd)[e];
+                p[2 * q] += a.x * b.x + a.y * b.y;
+                p[2 * q + 1] += a.x * b.y - a.y * b.x;
+              }
+          } else {
+            const double b = w[e];
+#pragma unroll
+            for (int q = 0; q < SB_J; ++q)
+              if (j0 + q < nv) p[2 * q] += V[(size_t)(j0 + q) * nd + e] * b;
+          }
+        }
+        sb_sum<2 * SB_J>(p, sh, res);
+        if (t < 2 * SB_J && j0 + t / 2 < nv) coef[2 * j0 + t] = res[t];
+      }
+      __syncthreads();
+      if (mode == 2 && t < nv && !done) {
+        double *hc = st + o_h + 2 * ((size_t)t + (size_t)ldh * i);
+        if (pass == 0) {
+          hc[0] = coef[2 * t];
+          hc[1] = coef[2 * t + 1];
+        } else {
+          hc[0] += coef[2 * t];
+          hc[1] += coef[2 * t + 1];
+        }
+      }
+      double s = 0.0;
+      for (size_t e = t; e < n; e += SB_T) {
+        if (CPLX) {
+          double2 v = reinterpret_cast<double2 *>(w)[e];
+          if (!done)
+            for (int j = 0; j < nv; ++j) {
+              const double2 x = reinterpret_cast<const double2 *>(V + (size_t)j * nd)[e];
+              const double cr = coef[2 * j], ci = coef[2 * j + 1];
+              v.x -= cr * x.x - ci * x.y;
+              v.y -= cr * x.y + ci * x.x;
+            }
+          reinterpret_cast<double2 *>(w)[e] = v;
+          s += v.x * v.x + v.y * v.y;
+        } else {
+          double v = w[e];
+          if (!done)
+            for (int j = 0; j < nv; ++j) v -= coef[2 * j] * V[(size_t)j * nd + e];
+          w[e] = v;
+          s += v * v;
+        }
+      }
+      s2 = s;
+      __syncthreads();  // coef is rewritten by the next pass
+    }
+  }
+  double pb[1] = {s2};
+  sb_sum<1>(pb, sh, res);
+  if (t == 0) {
+    if (!done) {
+      const double b = sqrt(res[0]);
+      st[o_beta + i] = b;
+      st[1] = (double)(i + 1);
+      if (b < btol) st[0] = 1.0;
+    }
+    scal[2] = st[0];
+    scal[3] = st[o_beta + i];
+  }
+  __syncthreads();
+  if (next) {
+    const double sv = scal[3];
+    const bool z = scal[2] != 0.0 || sv == 0.0;
+    for (size_t e = t; e < nd; e += SB_T) next[e] = z ? 0.0 : w[e] / sv;
+  }
+}
+
+// Start vector for short vectors: st[3] = |x0|, V_0 = x0 / |x0| (zeros for
+// a zero vector), one launch.
+__global__ void __launch_bounds__(SB_T)
+    k_start_small(size_t nd, const double *__restrict__ x0, double *__restrict__ v0,
+                  double *__restrict__ st) {
+  __shared__ double sh[SB_W];
+  __shared__ double res[1];  // sb_sum<1> uses SB_W entries of sh
+  const int t = threadIdx.x;
+  double p[1] = {0.0};
+  for (size_t e = t; e < nd; e += SB_T) p[0] += x0[e] * x0[e];
+  sb_sum<1>(p, sh, res);
+  const double sv = sqrt(res[0]);
+  if (t == 0) st[3] = sv;
+  for (size_t e = t; e < nd; e += SB_T) v0[e] = sv == 0.0 ? 0.0 : x0[e] / sv;
+}
+
 // ---------------------------------------------------------- host helpers
 
 // Eigen-decomposition of a real symmetric k x k matrix (column-major) by the
@@ -597,6 +804,11 @@ struct Krylov {
       MH_TRY(project(w));
       x0 = w;
     }
+    if (small()) {
+      hipLaunchKernelGGL(k_start_small, dim3(1), dim3(SB_T), 0, h->stream, nd,
+                         (const double *)x0, vec(0), st);
+      return check_launch(h, "lanczos start");
+    }
     MH_TRY(launch_nrm((const double *)x0, part));
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RB), 0, h->stream, part, G, st + 3, 1);
     unsigned gb = (unsigned)((nd + 255) / 256);
@@ -634,8 +846,23 @@ struct Krylov {
     return check_launch(h, "arnoldi step");
   }
 
+  bool small() const { return h->krylov_small && nd <= SMALL_ND && kmax <= SB_VMAX; }
+
+  // Any step on short vectors: the matvec, then one workgroup for the rest.
+  int step_small(int i, double btol, int mode, bool write_next) {
+    MH_TRY(apply(vec(i), w));
+    int slot;
+    prof_begin(h, MH_K_VEC, 0.0, &slot);
+    double *nx = write_next ? vec(i + 1) : nullptr;
+    if (cplx) hipLaunchKernelGGL(k_step_small<true>, dim3(1), dim3(SB_T), 0, h->stream, n, V, w, nx, st, lay.o_alpha, lay.o_beta, lay.o_h, kmax + 1, i, mode, btol);
+    else hipLaunchKernelGGL(k_step_small<false>, dim3(1), dim3(SB_T), 0, h->stream, n, V, w, nx, st, lay.o_alpha, lay.o_beta, lay.o_h, kmax + 1, i, mode, btol);
+    prof_end(h, slot);
+    return check_launch(h, "krylov step");
+  }
+
   // One Lanczos step i: w = H V_i, alpha_i, w -= ..., (reorth), beta_i, V_{i+1}.
   int step(int i, double btol, int reorth, bool write_next) {
+    if (small()) return step_small(i, btol, arnoldi ? 2 : (reorth ? 1 : 0), write_next);
     if (arnoldi) return step_arnoldi(i, btol, write_next);
     double *part = st + lay.o_part;
     MH_TRY(apply(vec(i), w));

@@ -136,6 +136,8 @@ int mh_create(int device, mh_handle *out) {
   if (const char *s = getenv("MH_GRAPHS")) h->graphs = atoi(s) != 0;
   // MH_FUSION=0: run the first H_eff step as GEMM + separate mode product.
   if (const char *s = getenv("MH_FUSION")) h->fusion = atoi(s) != 0;
+  // MH_KRYLOV_SMALL=0: multi-block vector kernels for every Krylov step.
+  if (const char *s = getenv("MH_KRYLOV_SMALL")) h->krylov_small = atoi(s) != 0;
   *out = h;
   return MH_OK;
 }
@@ -168,6 +170,12 @@ int mh_set_graphs(mh_handle h, int enable) {
 int mh_set_fusion(mh_handle h, int enable) {
   if (!h) return MH_ERR_ARG;
   h->fusion = enable ? 1 : 0;
+  return MH_OK;
+}
+
+int mh_set_krylov_small(mh_handle h, int enable) {
+  if (!h) return MH_ERR_ARG;
+  h->krylov_small = enable ? 1 : 0;
   return MH_OK;
 }
 

@@ -114,6 +114,11 @@ class Engine:
         """Krylov recurrences as hipGraphs (default) or eager launches."""
         self.check(self.lib.mh_set_graphs(self.h, 1 if on else 0))
 
+    def set_krylov_small(self, on=True):
+        """Single-workgroup Krylov steps for short vectors (default) or the
+        multi-block vector kernels at every size."""
+        self.check(self.lib.mh_set_krylov_small(self.h, 1 if on else 0))
+
     def set_fusion(self, on=True):
         """Fused first H_eff step (default) or GEMM + separate mode product."""
         self.check(self.lib.mh_set_fusion(self.h, 1 if on else 0))

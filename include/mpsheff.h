@@ -189,6 +189,12 @@ int mh_set_graphs(mh_handle h, int enable);
  * followed by a separate mode-product launch instead of the fused kernel
  * (default on; environment MH_FUSION=0). */
 int mh_set_fusion(mh_handle h, int enable);
+/* enable = 1 (default; environment MH_KRYLOV_SMALL=0 turns it off): a Krylov
+ * step on vectors of at most 16384 doubles runs its vector work (alpha, the
+ * update, Gram-Schmidt passes, beta, next basis vector) as one launch of one
+ * workgroup instead of the multi-block reduction kernels; the same
+ * recurrence with a different summation order. */
+int mh_set_krylov_small(mh_handle h, int enable);
 /* GEMM configurations (tile shape, K depth, split-K, tile order, waves) the
  * per-shape tuner of each precision chooses from; force_candidate fixes
  * configuration `cand` of the calling precision's list for every GEMM of the

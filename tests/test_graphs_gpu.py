@@ -1,7 +1,8 @@
 """Krylov recurrences captured as hipGraphs give the same numbers as the
 eager launches: dense and block-sparse Lanczos eigensolves and the
 sub-stepped exponentiate (whose step graph is launched once per sub-step),
-each repeated with eager work of the program in between."""
+each repeated with eager work of the program in between, with the
+single-workgroup steps for short vectors and with the multi-block ones."""
 import numpy as np
 import pytest
 import torch
@@ -11,6 +12,14 @@ from finitemptools_amd import models
 from finitemptools_amd.engine import engine, to_device
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=[True, False], ids=["small-steps", "multiblock-steps"])
+def small(request):
+    eng = engine(0)
+    eng.set_krylov_small(request.param)
+    yield request.param
+    eng.set_krylov_small(True)
 
 
 def both(fn):
@@ -29,7 +38,7 @@ def both(fn):
     return out
 
 
-def test_dense_lanczos_graph_matches_eager(gpu):
+def test_dense_lanczos_graph_matches_eager(gpu, small):
     from finitemptools_amd.derivatives import HeffOperator
     from finitemptools_amd.krylov import simple_lanczos_solver
     rng = np.random.default_rng(3)
@@ -45,7 +54,7 @@ def test_dense_lanczos_graph_matches_eager(gpu):
         assert torch.equal(v0, v1) and torch.equal(v0, v2)
 
 
-def test_block_lanczos_graph_matches_eager(gpu):
+def test_block_lanczos_graph_matches_eager(gpu, small):
     from finitemptools_amd import blockmps as bm
     from finitemptools_amd import blocksparse as bs
     phys = bm.hubbard_charges()
@@ -63,7 +72,7 @@ def test_block_lanczos_graph_matches_eager(gpu):
     assert torch.equal(v0, v1) and torch.equal(v0, v2)
 
 
-def test_exponentiate_graph_matches_eager(gpu):
+def test_exponentiate_graph_matches_eager(gpu, small):
     """A tolerance that needs several sub-steps: the step graph is captured
     once and launched for each of them."""
     from finitemptools_amd.derivatives import HeffOperator
@@ -81,7 +90,7 @@ def test_exponentiate_graph_matches_eager(gpu):
     assert torch.equal(y0, y1) and torch.equal(y0, y2)
 
 
-def test_arnoldi_exponentiate_graph_matches_eager(gpu):
+def test_arnoldi_exponentiate_graph_matches_eager(gpu, small):
     """The Arnoldi steps (Hessenberg columns in device memory) captured as a
     graph give bitwise the eager result."""
     from finitemptools_amd.derivatives import HeffOperator
