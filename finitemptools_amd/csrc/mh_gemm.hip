@@ -244,11 +244,23 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[q][i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
 
-  auto compute = [&](int buf) {
+  // Fragments of one 16-deep slice: a[q][i][s], b[q][j][s] (sub-step s).
+  auto frag = [&](int buf, int kb, double (&a)[NP][TM][4], double (&b)[NP][TN][4]) {
+    if constexpr (AMC && !BKC) {
+      // both operands staged [k][r]: sub-step by sub-step, A and B together,
+      // so the first MFMAs wait only for the first reads
 #pragma unroll
-    for (int kb = 0; kb < BK; kb += 16) {
-      if (KG == 2 && ((kb / 16) & 1) != kgrp) continue;
-      double a[NP][TM][4], b[NP][TN][4];
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+            a[q][i][s] = sA[q][buf][(kb + 4 * lk + s) * SA::PITCH + wm * WTM + i * 16 + lr];
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            b[q][j][s] = sB[q][buf][(kb + 4 * lk + s) * SB::PITCH + wn * WTN + j * 16 + lr];
+        }
+    } else {
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
 #pragma unroll
@@ -276,22 +288,40 @@ __global__ void __launch_bounds__(64 * NW)
           }
         }
       }
+    }
+  };
+  auto mma = [&](const double (&a)[NP][TM][4], const double (&b)[NP][TN][4]) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < TN; ++j) {
-            if constexpr (CPLX) {
-              const double nai = -a[1][i][s];
-              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
-              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
-              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[1][j][s], acc[0][i][j], 0, 0, 0);
-              acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][i][s], b[0][j][s], acc[1][i][j], 0, 0, 0);
-            } else {
-              acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
-            }
+        for (int j = 0; j < TN; ++j) {
+          if constexpr (CPLX) {
+            const double nai = -a[1][i][s];
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[1][j][s], acc[0][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][i][s], b[0][j][s], acc[1][i][j], 0, 0, 0);
+          } else {
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
           }
+        }
+  };
+  // One K step from LDS buffer buf.  A wave runs the slices of its K group
+  // (all of them, or every other one for KG = 2), software-pipelined: the
+  // fragments of slice t+1 are read before the MFMAs of slice t are issued,
+  // and the scheduling barrier keeps the compiler from sinking those reads
+  // back to their first use, so LDS latency hides behind the MFMAs.
+  auto compute = [&](int buf) {
+    constexpr int NSL = BK / 16 / KG;  // slices per wave and K step
+    double fa[2][NP][TM][4], fb[2][NP][TN][4];
+    frag(buf, (KG == 2 ? kgrp : 0) * 16, fa[0], fb[0]);
+#pragma unroll
+    for (int t = 0; t < NSL; ++t) {
+      if (t + 1 < NSL) frag(buf, ((t + 1) * KG + (KG == 2 ? kgrp : 0)) * 16, fa[(t + 1) & 1], fb[(t + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(fa[t & 1], fb[t & 1]);
     }
   };
 
