@@ -81,6 +81,7 @@ SIGNATURES = {
                                 c_double, c_void_p, P(c_int)]),
     "mh_set_graphs": (c_int, [c_void_p, c_int]),
     "mh_set_krylov_small": (c_int, [c_void_p, c_int]),
+    "mh_set_bs_plain": (c_int, [c_void_p, c_int]),
     "mh_set_fusion": (c_int, [c_void_p, c_int]),
     "mh_gemm_candidates": (c_int, [c_int, P(c_int)]),
     "mh_gemm_force_candidate": (c_int, [c_void_p, c_int]),

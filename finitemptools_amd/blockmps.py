@@ -522,6 +522,44 @@ def _eig_workers(dev, n):
 
 EIG_THREADS = 4
 
+# Plain per-sector execution (the baseline the batched forms are measured
+# against): one GEMM launch per sector product in every block plan and one
+# eigensolve per sector, in sequence, in the two-site split.
+PLAIN = False
+
+
+class plain_path:
+    """Context manager: run block plans and splits as the plain per-sector
+    loop (FP64, Krylov steps launched eagerly) on the engine of ``device``,
+    restoring the batched forms and step graphs on exit."""
+
+    def __init__(self, device=None):
+        from .engine import engine
+        self.eng = engine(device)
+
+    def __enter__(self):
+        global PLAIN
+        self.eng.set_bs_plain(True)
+        self.eng.set_graphs(False)
+        PLAIN = True
+        return self
+
+    def __exit__(self, *exc):
+        global PLAIN
+        self.eng.set_bs_plain(False)
+        self.eng.set_graphs(True)
+        PLAIN = False
+        return False
+
+
+def _plain_eigh(grams):
+    """One eigh per sector Gram matrix, in sequence: [(n, [charge], lam, V)]."""
+    out = []
+    for c in sorted(grams, key=lambda c: -grams[c].shape[0]):
+        lam, V = torch.linalg.eigh(grams[c])
+        out.append((grams[c].shape[0], [c], lam.unsqueeze(0), V.unsqueeze(0)))
+    return out
+
 
 def _batched_eigh(grams, dev):
     """Eigen-decompositions of the sector Gram matrices: one batched eigh per
@@ -588,7 +626,7 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
         M = cat[tmap.base[c]:tmap.base[c] + R * C].view(R, C)
         mats[c] = M
         grams[c] = M @ M.t() if left else M.t() @ M
-    pending = _batched_eigh(grams, dev)
+    pending = _plain_eigh(grams) if PLAIN else _batched_eigh(grams, dev)
     eig = {}
     lam_host = {}
     lam_all = torch.cat([p[2].reshape(-1) for p in pending]).cpu().numpy() if pending else None

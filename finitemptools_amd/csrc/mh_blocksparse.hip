@@ -628,7 +628,55 @@ static int pick_variant(mh_handle h, int prec, const BsStage &st, const double *
   return st.choice = best;
 }
 
+// The GEMM stage as the per-sector loop of a plain port: one strided FP64
+// GEMM launch per output block and K-segment (segments after the first
+// accumulate with beta = 1; a block without segments is written as zeros by
+// a K = 0 launch).  The configuration comes from the GEMM heuristic: these
+// products are far below the size the tuner times.
+static int run_stage_plain(mh_handle h, int prec, const BsStage &st, void *const *srcs) {
+  if (prec != MH_PREC_F64)
+    return fail(h, MH_ERR_UNSUPPORTED, "the per-sector loop runs FP64 plans only");
+  const double *A = (const double *)srcs[st.a_src];
+  const double *B = (const double *)srcs[st.b_src];
+  double *C = (double *)srcs[st.c_src];
+  int slot;
+  prof_begin(h, MH_K_GEMM, st.flops, &slot);
+  for (const GTask &t : st.h_tasks) {
+    const int nseg = t.nseg > 0 ? t.nseg : 1;
+    for (int sg = 0; sg < nseg; ++sg) {
+      GemmDesc d{};
+      d.M = t.M;
+      d.N = t.N;
+      d.C = C + t.c_off;
+      d.scm = 1;
+      d.scn = t.ldc;
+      d.alpha_re = 1.0;
+      d.beta_re = sg > 0 ? 1.0 : 0.0;
+      if (t.nseg > 0) {
+        const GSeg &g = st.h_segs[t.seg0 + sg];
+        d.K = g.K;
+        d.A = A + g.a_off;
+        d.sam = st.amc ? 1 : g.lda;
+        d.sak = st.amc ? g.lda : 1;
+        d.B = B + g.b_off;
+        d.sbk = st.bkc ? 1 : g.ldb;
+        d.sbn = st.bkc ? g.ldb : 1;
+      } else {
+        d.K = 0;
+        d.A = A;
+        d.B = B;
+        d.sam = d.sbk = 1;
+        d.sak = d.sbn = 1;
+      }
+      MH_TRY(gemm(h, MH_F64, MH_PREC_F64, d));
+    }
+  }
+  prof_end(h, slot);
+  return check_launch(h, "per-sector gemm");
+}
+
 static int run_stage(mh_handle h, int prec, const BsStage &st, void *const *srcs) {
+  if (st.kind == 0 && h->bs_plain) return run_stage_plain(h, prec, st, srcs);
   if (st.kind == 0) {
     if (st.ntile_v[0] == 0) return MH_OK;
     const double *A = (const double *)srcs[st.a_src];

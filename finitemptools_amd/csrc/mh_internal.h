@@ -56,6 +56,9 @@ struct mh_context {
   int fusion = 1;
   // Krylov steps on short vectors as one single-workgroup launch.
   int krylov_small = 1;
+  // Block plans as one GEMM launch per sector product (the plain per-sector
+  // loop) instead of one grouped launch per stage.
+  int bs_plain = 0;
   hipStream_t cap_stream = nullptr;
   int autotune = 1;
   int gemm_force = -1;  // FP64 GEMM configuration forced for every call (-1: tuned)

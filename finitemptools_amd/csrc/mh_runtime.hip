@@ -138,6 +138,8 @@ int mh_create(int device, mh_handle *out) {
   if (const char *s = getenv("MH_FUSION")) h->fusion = atoi(s) != 0;
   // MH_KRYLOV_SMALL=0: multi-block vector kernels for every Krylov step.
   if (const char *s = getenv("MH_KRYLOV_SMALL")) h->krylov_small = atoi(s) != 0;
+  // MH_BS_PLAIN=1: block plans run one GEMM launch per sector product.
+  if (const char *s = getenv("MH_BS_PLAIN")) h->bs_plain = atoi(s) != 0;
   *out = h;
   return MH_OK;
 }
@@ -176,6 +178,12 @@ int mh_set_fusion(mh_handle h, int enable) {
 int mh_set_krylov_small(mh_handle h, int enable) {
   if (!h) return MH_ERR_ARG;
   h->krylov_small = enable ? 1 : 0;
+  return MH_OK;
+}
+
+int mh_set_bs_plain(mh_handle h, int enable) {
+  if (!h) return MH_ERR_ARG;
+  h->bs_plain = enable ? 1 : 0;
   return MH_OK;
 }
 

@@ -119,6 +119,11 @@ class Engine:
         multi-block vector kernels at every size."""
         self.check(self.lib.mh_set_krylov_small(self.h, 1 if on else 0))
 
+    def set_bs_plain(self, on=True):
+        """Block-sparse GEMM stages as one launch per sector product (the
+        plain per-sector loop) instead of the grouped launch (default)."""
+        self.check(self.lib.mh_set_bs_plain(self.h, 1 if on else 0))
+
     def set_fusion(self, on=True):
         """Fused first H_eff step (default) or GEMM + separate mode product."""
         self.check(self.lib.mh_set_fusion(self.h, 1 if on else 0))

@@ -195,6 +195,13 @@ int mh_set_fusion(mh_handle h, int enable);
  * workgroup instead of the multi-block reduction kernels; the same
  * recurrence with a different summation order. */
 int mh_set_krylov_small(mh_handle h, int enable);
+/* enable = 1 (default 0; environment MH_BS_PLAIN=1): the GEMM stages of a
+ * block-sparse plan run as one strided FP64 GEMM launch per sector product
+ * (output block and K-segment, segments accumulated), the per-sector loop of
+ * a plain port of TensorKit's block contraction, instead of one grouped
+ * launch per stage.  FP64 plans only; the baseline the grouped form is
+ * measured against. */
+int mh_set_bs_plain(mh_handle h, int enable);
 /* GEMM configurations (tile shape, K depth, split-K, tile order, waves) the
  * per-shape tuner of each precision chooses from; force_candidate fixes
  * configuration `cand` of the calling precision's list for every GEMM of the
