@@ -14,8 +14,11 @@ SOURCES = [
     "mh_krylov.hip",
     "mh_blocksparse.hip",
     "mh_adjoint.hip",
+    "mh_eigen.hip",
     "mh_bs_planner.cpp",
 ]
+# rocSOLVER's divide-and-conquer tridiagonal eigensolver (mh_stedc_batched)
+LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
@@ -73,7 +76,7 @@ def build(verbose=False, jobs=None):
     if errors:
         raise RuntimeError("HIP compilation failed:\n" + "\n".join(errors))
     tmp = lib_path() + ".tmp"
-    cmd = [hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + LIBS
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)

@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from . import blocksparse as bs
+from . import eigen
 from .blocksparse import (BlockEnv, BlockLayout, BlockTensor, EnvLayout, Space, qadd,
                           qsub, qzero)
 from .truncation import MPSTruncation
@@ -522,6 +523,13 @@ def _eig_workers(dev, n):
 
 EIG_THREADS = 4
 
+# Sector eigensolver of the density-matrix split: "library" (one batched
+# library eigh per size class on side streams, the faster of the two at the
+# sector counts of a D=2048 Hubbard bond) or "sector" (eigen.SectorEigh: one
+# tridiagonalisation launch for all sectors, then a divide-and-conquer call
+# per sector, which the library runs one after another).
+EIG_SOLVER = "library"
+
 # Plain per-sector execution (the baseline the batched forms are measured
 # against): one GEMM launch per sector product in every block plan and one
 # eigensolve per sector, in sequence, in the two-site split.
@@ -612,48 +620,81 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
     """Device split by the sector reduced density matrices (White's original
     DMRG step): for the left sweep rho_c = M_c M_c^T gives u and s^2, and
     s v = u^T M_c is an exact projection; for the right sweep rho_c = M_c^T M_c
-    gives v and u s = M_c v^T.  The eigenproblems of one size class run as one
-    batched eigh; the truncation acts on the merged spectrum s = sqrt(lambda)
-    exactly as for the SVD.  Returns (A1, A2, {c: s_c})."""
+    gives v and u s = M_c v^T.  The eigenpairs of all sectors come from one
+    batched library eigh per size class (EIG_SOLVER = "library") or from
+    `eigen.SectorEigh` (EIG_SOLVER = "sector": one tridiagonalisation launch
+    for every sector, divide and conquer per sector, back-transform of the
+    kept columns only); the truncation acts on the merged spectrum
+    s = sqrt(lambda) exactly as for the SVD.  Returns (A1, A2, {c: s_c})."""
     xl = tmap.xl
     dev = vec.device
     cat = torch.empty_like(vec)
     cat[tmap.dest(dev)] = vec
     left = absorb == "right"
-    mats, grams = {}, {}
-    for c in tmap.charges:
+    charges = list(tmap.charges)
+    mats = {}
+    for c in charges:
         R, C = tmap.shape[c]
-        M = cat[tmap.base[c]:tmap.base[c] + R * C].view(R, C)
-        mats[c] = M
-        grams[c] = M @ M.t() if left else M.t() @ M
-    pending = _plain_eigh(grams) if PLAIN else _batched_eigh(grams, dev)
-    eig = {}
+        mats[c] = cat[tmap.base[c]:tmap.base[c] + R * C].view(R, C)
+    order = {c: (tmap.shape[c][0] if left else tmap.shape[c][1]) for c in charges}
+    use_sector = (not PLAIN and EIG_SOLVER == "sector" and dev.type == "cuda"
+                  and all(n <= eigen.MAX_ORDER for n in order.values()))
     lam_host = {}
-    lam_all = torch.cat([p[2].reshape(-1) for p in pending]).cpu().numpy() if pending else None
-    pos = 0
-    for nb, cs, lam, V in pending:
-        lh = lam_all[pos:pos + lam.numel()].reshape(lam.shape)
-        pos += lam.numel()
-        for j, c in enumerate(cs):
+    if use_sector:
+        se = eigen.SectorEigh([order[c] for c in charges], dev)
+        for j, c in enumerate(charges):
+            M = mats[c]
+            if left:
+                torch.matmul(M, M.t(), out=se.matrix(j))
+            else:
+                torch.matmul(M.t(), M, out=se.matrix(j))
+        lams = se.eigenvalues()
+        lam_all = torch.cat(lams).cpu().numpy()
+        se.check()
+        pos = 0
+        for c in charges:
+            n = order[c]
             R, C = tmap.shape[c]
-            r = min(R, C)
-            # descending; only the top min(R, C) can be nonzero
-            l = np.maximum(lh[j, ::-1][:r], 0.0)
+            l = np.maximum(lam_all[pos:pos + n][::-1][:min(R, C)], 0.0)
             lam_host[c] = np.sqrt(l)
-            eig[c] = (V, j, nb)
+            pos += n
+    else:
+        grams = {c: (mats[c] @ mats[c].t() if left else mats[c].t() @ mats[c]) for c in charges}
+        pending = _plain_eigh(grams) if PLAIN else _batched_eigh(grams, dev)
+        eig = {}
+        lam_all = torch.cat([p[2].reshape(-1) for p in pending]).cpu().numpy() if pending else None
+        pos = 0
+        for nb, cs, lam, V in pending:
+            lh = lam_all[pos:pos + lam.numel()].reshape(lam.shape)
+            pos += lam.numel()
+            for j, c in enumerate(cs):
+                R, C = tmap.shape[c]
+                r = min(R, C)
+                # descending; only the top min(R, C) can be nonzero
+                l = np.maximum(lh[j, ::-1][:r], 0.0)
+                lam_host[c] = np.sqrt(l)
+                eig[c] = (V, j, nb)
     keep = _global_keep_host({c: (None, s) for c, s in lam_host.items()}, trunc)
     keep = {c: k for c, k in keep.items() if lam_host[c][k - 1] > 0.0}
     norm = float(np.sqrt(sum(float((lam_host[c][:k] ** 2).sum()) for c, k in keep.items())))
+    # top-k eigenvectors of each kept sector, descending
+    if use_sector:
+        vk = se.vectors([keep.get(c, 0) for c in charges])
+        top = {c: vk[j].flip(-1) for j, c in enumerate(charges) if c in keep}
+    else:
+        top = {}
+        for c, k in keep.items():
+            V, j, nb = eig[c]
+            top[c] = V[j, :order[c], nb - k:].flip(-1)
     mid = Space(keep.items())
     l1 = BlockLayout(xl.lspace, xl.phys, 1, mid)
     l2 = BlockLayout(mid, xl.phys, 1, xl.rspace)
     svals, us, vs, upieces, vpieces = {}, [], [], [], []
     ubase = vbase = 0
     for c, k in keep.items():
-        V, j, nb = eig[c]
         M = mats[c]
         R, C = tmap.shape[c]
-        Q = V[j, :grams[c].shape[0], nb - k:].flip(-1)   # top-k eigenvectors, descending
+        Q = top[c]
         svals[c] = lam_host[c][:k] / norm
         if left:
             u = Q                                # R x k

@@ -1,0 +1,551 @@
+// Batched FP64 symmetric eigensolver for the sector density matrices of the
+// block-sparse two-site split (the FP64 step that replaces the per-sector
+// truncated SVD of auxiliary/stable_svd.jl:4-18 on graded tensors).
+//
+//   mh_sytrd_batched : Householder tridiagonalisation of every matrix of a
+//                      batch in ONE cooperative launch.  Matrix m gets G_m
+//                      workgroups that own its columns cyclically and meet at
+//                      a group barrier once per column, so a step costs one
+//                      barrier instead of the handful of launches a
+//                      library reduction spends per column.  Output follows
+//                      LAPACK dsytrd (uplo = 'L'): D, E, tau, and reflector
+//                      m's vector v_i (v_i[i+1] = 1 implicit) in A[i+2:, i].
+//   mh_stedc_batched : eigen-decomposition of the tridiagonals by rocSOLVER's
+//                      divide and conquer (dstedc), one call per matrix, the
+//                      calls spread over side streams.
+//   mh_ormtr_batched : selected eigenvector columns back-transformed,
+//                      z <- H_0 H_1 ... H_{n-2} z, one wavefront per column.
+//
+// Algorithm of the tridiagonalisation (Golub & Van Loan, Alg. 8.3.1; the
+// LAPACK dsytd2 recurrence): with the column i of A^(i) split as
+// (alpha, x), H_i = I - tau v v^T maps it to (beta, 0); p = tau A v,
+// w = p - (tau/2)(p.v) v, A^(i+1) = A^(i) - v w^T - w v^T.  Each workgroup
+// applies the rank-2 update of step i-1 to its own columns lazily, in the
+// same pass that forms its entries of p_i = tau_i A^(i) v_i (A symmetric, so
+// row j . v = column j . v); p_i and the next column travel between the
+// workgroups of one matrix through double-buffered global rows.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <thread>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+namespace {
+
+constexpr int SY_THREADS = 256;
+constexpr int SY_WAVES = SY_THREADS / 64;
+constexpr int SY_MAXT = 48;          // matrices per launch (kernel-argument table)
+constexpr int SY_MAXN = 2048;        // largest order (LDS: 3 n doubles per workgroup)
+constexpr int SY_COLS_PER_WG = 16;   // target columns per workgroup
+constexpr int SY_MAX_G = 64;         // workgroups per matrix
+constexpr unsigned SY_SPIN_LIMIT = 1u << 22;
+
+struct SyTask {
+  int64_t a_off;  // element offset of the matrix in A (column-major, ld = n)
+  int64_t v_off;  // element offset of D / E / tau and of the exchange rows
+  int n;          // order
+  int G;          // workgroups of this matrix
+  int g0;         // first workgroup of this matrix in the grid
+  int pad;
+};
+
+struct SyTable {
+  int nt;
+  int pad;
+  int64_t xrow;  // stride between the two halves of the exchange buffers
+  SyTask t[SY_MAXT];
+};
+
+// All lanes of the block get the sum; the order of additions is fixed, so
+// every workgroup of a matrix computes bit-identical values.
+__device__ inline double block_sum(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < SY_WAVES; ++i) s += red[i];
+  return s;
+}
+
+__device__ inline double xload(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void xstore(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier of the G workgroups of one matrix: a monotone counter, target
+// G * (number of barriers passed).  The wait is bounded: past the spin limit
+// the workgroup raises *err and leaves, and once *err is set every later
+// wait returns at once, so the grid always drains.
+__device__ inline void group_barrier(unsigned *cnt, unsigned target, int *err) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(cnt, 1u);
+    unsigned spins = 0;
+    // relaxed polls (no cache invalidation per poll); one acquire below
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (++spins > SY_SPIN_LIMIT) {
+        atomicOr(err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(SY_THREADS) void k_sytrd_group(SyTable tab, double *__restrict__ A,
+                                                            double *__restrict__ D,
+                                                            double *__restrict__ E,
+                                                            double *__restrict__ tau,
+                                                            double *__restrict__ P,
+                                                            double *__restrict__ BC,
+                                                            unsigned *__restrict__ cnt,
+                                                            int *__restrict__ err) {
+  extern __shared__ double sm[];
+  __shared__ double red[SY_WAVES];
+  int t = 0;
+  while (t + 1 < tab.nt && (int)blockIdx.x >= tab.t[t + 1].g0) ++t;
+  const SyTask tk = tab.t[t];
+  const int n = tk.n, G = tk.G, g = (int)blockIdx.x - tk.g0;
+  double *a = A + tk.a_off;
+  double *vp = sm, *wp = sm + n, *vc = sm + 2 * n;
+  double *const P0 = P + tk.v_off, *const P1 = P + tab.xrow + tk.v_off;
+  double *const B0 = BC + tk.v_off, *const B1 = BC + tab.xrow + tk.v_off;
+  unsigned *c = cnt + t;
+  unsigned nbar = 0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+
+  for (int r = threadIdx.x; r < 3 * n; r += SY_THREADS) sm[r] = 0.0;
+  if (g == 0)
+    for (int r = threadIdx.x; r < n; r += SY_THREADS) xstore(B0 + r, a[r]);
+  group_barrier(c, (unsigned)G * ++nbar, err);
+
+  double tau_prev = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double *bc = (i & 1) ? B1 : B0;
+    // Phase A (every workgroup, redundantly): w_{i-1}, column i of A^(i),
+    // and the reflector v_i, tau_i, beta_i.
+    if (i > 0) {
+      const double *pp = ((i - 1) & 1) ? P1 : P0;
+      double s = 0.0;
+      for (int r = i + threadIdx.x; r < n; r += SY_THREADS) s += xload(pp + r) * vp[r];
+      const double h = 0.5 * tau_prev * block_sum(s, red);
+      for (int r = i + threadIdx.x; r < n; r += SY_THREADS) wp[r] = xload(pp + r) - h * vp[r];
+      __syncthreads();
+    }
+    const double vpi = vp[i], wpi = wp[i];
+    double s2 = 0.0;
+    for (int r = i + threadIdx.x; r < n; r += SY_THREADS) {
+      const double x = xload(bc + r) - vp[r] * wpi - wp[r] * vpi;
+      vc[r] = x;
+      if (r >= i + 2) s2 += x * x;
+    }
+    const double xn2 = block_sum(s2, red);  // its barriers publish vc
+    const double di = vc[i];
+    if (i == n - 1) {
+      if (g == 0 && threadIdx.x == 0) {
+        D[tk.v_off + i] = di;
+        E[tk.v_off + i] = 0.0;
+        tau[tk.v_off + i] = 0.0;
+      }
+      if (i % G == g && threadIdx.x == 0) a[i + (int64_t)i * n] = di;
+      break;
+    }
+    const double alpha = vc[i + 1];
+    double beta = alpha, tu = 0.0, scale = 1.0;
+    if (xn2 > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tu = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    for (int r = i + 2 + threadIdx.x; r < n; r += SY_THREADS) vc[r] *= scale;
+    if (threadIdx.x == 0) {
+      vc[i] = 0.0;
+      vc[i + 1] = 1.0;
+    }
+    __syncthreads();
+    if (i % G == g) {  // owner of column i stores the reflector (LAPACK layout)
+      double *col = a + (int64_t)i * n;
+      for (int r = i + 2 + threadIdx.x; r < n; r += SY_THREADS) col[r] = vc[r];
+      if (threadIdx.x == 0) {
+        col[i] = di;
+        col[i + 1] = beta;
+      }
+    }
+    if (g == 0 && threadIdx.x == 0) {
+      D[tk.v_off + i] = di;
+      E[tk.v_off + i] = beta;
+      tau[tk.v_off + i] = tu;
+    }
+    // Phase B: own columns j >= i+1 (j = g mod G), one wavefront per column:
+    // apply step i-1's rank-2 update, form p_i[j]; the owner of column i+1
+    // also publishes that column of A^(i) for step i+1.
+    double *pc = (i & 1) ? P1 : P0;
+    double *bn = ((i + 1) & 1) ? B1 : B0;
+    const int j0 = i + 1 + (((g - (i + 1)) % G) + G) % G;
+    for (int j = j0 + wv * G; j < n; j += SY_WAVES * G) {
+      double *col = a + (int64_t)j * n;
+      const double vpj = vp[j], wpj = wp[j];
+      const bool pub = (j == i + 1);
+      double acc = 0.0;
+      for (int r = i + 1 + lane; r < n; r += 64) {
+        const double x = col[r] - vp[r] * wpj - wp[r] * vpj;
+        col[r] = x;
+        if (pub) xstore(bn + r, x);
+        acc += x * vc[r];
+      }
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+      if (lane == 0) xstore(pc + j, tu * acc);
+    }
+    double *sw = vp;  // v_i becomes the lazy update's vector of the next step
+    vp = vc;
+    vc = sw;
+    tau_prev = tu;
+    group_barrier(c, (unsigned)G * ++nbar, err);
+  }
+}
+
+// z <- Q z for selected columns, Q = H_0 H_1 ... H_{n-2}; one wavefront per
+// column, the column held in LDS, reflectors read from A in reverse order.
+constexpr int OR_THREADS = 256;
+constexpr int OR_WAVES = OR_THREADS / 64;
+
+struct OrTask {
+  int64_t a_off, v_off, z_off;
+  int n, col0, ncol, w0;  // w0: first workgroup of this matrix
+};
+struct OrTable {
+  int nt;
+  int pad;
+  OrTask t[SY_MAXT];
+};
+
+__global__ __launch_bounds__(OR_THREADS) void k_ormtr_cols(OrTable tab, const double *__restrict__ A,
+                                                           const double *__restrict__ tau,
+                                                           double *__restrict__ Z) {
+  extern __shared__ double zs[];
+  int t = 0;
+  while (t + 1 < tab.nt && (int)blockIdx.x >= tab.t[t + 1].w0) ++t;
+  const OrTask tk = tab.t[t];
+  const int n = tk.n, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int jc = ((int)blockIdx.x - tk.w0) * OR_WAVES + wv;  // column within the selection
+  if (jc >= tk.ncol) return;  // whole wavefront leaves; no block barrier below
+  double *z = zs + (int64_t)wv * n;
+  double *zg = Z + tk.z_off + (int64_t)(tk.col0 + jc) * n;
+  for (int r = lane; r < n; r += 64) z[r] = zg[r];
+  const double *a = A + tk.a_off;
+  const double *tv = tau + tk.v_off;
+  for (int i = n - 2; i >= 0; --i) {
+    const double ti = tv[i];
+    if (ti == 0.0) continue;  // uniform across the wavefront
+    const double *v = a + (int64_t)i * n;
+    // v = [0 .. 0, 1 (row i+1), A[i+2:, i]]
+    double s = (lane == 0) ? z[i + 1] : 0.0;
+    for (int r = i + 2 + lane; r < n; r += 64) s += v[r] * z[r];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    const double f = ti * s;
+    if (lane == 0) z[i + 1] -= f;
+    for (int r = i + 2 + lane; r < n; r += 64) z[r] -= f * v[r];
+  }
+  for (int r = lane; r < n; r += 64) zg[r] = z[r];
+}
+
+struct EigState {
+  std::vector<hipStream_t> streams;
+  std::vector<rocblas_handle> rb;
+  std::vector<hipEvent_t> done;
+  hipEvent_t ready = nullptr;
+  int *info = nullptr;
+  int info_len = 0;
+  int coresident = 0;  // sytrd workgroups that fit on the device at once
+  int coresident_lds = -1;
+};
+
+EigState *state(mh_handle h) {
+  if (!h->eig_state) h->eig_state = new EigState();
+  return (EigState *)h->eig_state;
+}
+
+int ensure_eig_scratch(mh_handle h, size_t bytes) {
+  if (h->eig_bytes >= bytes) return MH_OK;
+  MH_HIP(h, hipStreamSynchronize(h->stream));
+  if (h->eig) MH_HIP(h, hipFree(h->eig));
+  h->eig = nullptr;
+  h->eig_err = nullptr;
+  h->eig_bytes = 0;
+  size_t b = (bytes + 255) & ~size_t(255);
+  hipError_t e = hipMalloc(&h->eig, b);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    h->eig = nullptr;
+    return fail(h, MH_ERR_OOM, "eigensolver scratch of %zu bytes: %s", b, hipGetErrorString(e));
+  }
+  h->eig_bytes = b;
+  return MH_OK;
+}
+
+int eig_streams(mh_handle h, int k) {
+  EigState *s = state(h);
+  while ((int)s->streams.size() < k) {
+    hipStream_t st;
+    MH_HIP(h, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    s->streams.push_back(st);
+    rocblas_handle rb;
+    if (rocblas_create_handle(&rb) != rocblas_status_success)
+      return fail(h, MH_ERR_HIP, "rocblas_create_handle failed");
+    if (rocblas_set_stream(rb, st) != rocblas_status_success)
+      return fail(h, MH_ERR_HIP, "rocblas_set_stream failed");
+    s->rb.push_back(rb);
+    hipEvent_t ev;
+    MH_HIP(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    s->done.push_back(ev);
+  }
+  if (!s->ready) MH_HIP(h, hipEventCreateWithFlags(&s->ready, hipEventDisableTiming));
+  return MH_OK;
+}
+
+}  // namespace
+
+void eig_release(mh_handle h) {
+  EigState *s = (EigState *)h->eig_state;
+  if (s) {
+    for (auto st : s->streams) (void)hipStreamSynchronize(st);
+    for (auto rb : s->rb) (void)rocblas_destroy_handle(rb);
+    for (auto st : s->streams) (void)hipStreamDestroy(st);
+    for (auto ev : s->done) (void)hipEventDestroy(ev);
+    if (s->ready) (void)hipEventDestroy(s->ready);
+    if (s->info) (void)hipFree(s->info);
+    delete s;
+  }
+  h->eig_state = nullptr;
+  if (h->eig) (void)hipFree(h->eig);
+  h->eig = nullptr;
+  h->eig_err = nullptr;
+  h->eig_bytes = 0;
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+extern "C" int mh_sytrd_batched(mh_handle h, int nmat, const int *n, const int64_t *a_off,
+                                const int64_t *v_off, double *A, double *D, double *E,
+                                double *tau) {
+  if (!h || nmat < 0 || (nmat > 0 && (!n || !a_off || !v_off || !A || !D || !E || !tau)))
+    return fail(h, MH_ERR_ARG, "mh_sytrd_batched: null argument");
+  if (nmat == 0) return MH_OK;
+  int nmax = 0;
+  int64_t vend = 0;
+  for (int m = 0; m < nmat; ++m) {
+    if (n[m] < 1 || n[m] > SY_MAXN)
+      return fail(h, MH_ERR_SHAPE, "mh_sytrd_batched: order %d of matrix %d outside [1, %d]",
+                  n[m], m, SY_MAXN);
+    if (a_off[m] < 0 || v_off[m] < 0)
+      return fail(h, MH_ERR_SHAPE, "mh_sytrd_batched: negative offset (matrix %d)", m);
+    nmax = std::max(nmax, n[m]);
+    vend = std::max(vend, v_off[m] + n[m]);
+  }
+  MH_HIP(h, hipSetDevice(h->device));
+  EigState *st = state(h);
+  const size_t lds = (size_t)3 * nmax * sizeof(double);
+  if (st->coresident_lds != (int)lds) {
+    int per_cu = 0, ncu = 0;
+    MH_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_sytrd_group,
+                                                            SY_THREADS, lds));
+    MH_HIP(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+    st->coresident = per_cu * ncu;
+    st->coresident_lds = (int)lds;
+  }
+  if (st->coresident < 1)
+    return fail(h, MH_ERR_UNSUPPORTED, "mh_sytrd_batched: kernel does not fit on the device");
+  // scratch: two exchange halves for p and for the broadcast column, the
+  // barrier counters (one per matrix of a launch) and the error flag
+  const size_t xbytes = (size_t)vend * sizeof(double);
+  const size_t need = 4 * xbytes + 256 + (SY_MAXT + 64) * sizeof(unsigned);
+  MH_TRY(ensure_eig_scratch(h, need));
+  double *P = (double *)h->eig;
+  double *BC = P + 2 * vend;
+  unsigned *cnt = (unsigned *)((char *)h->eig + ((4 * xbytes + 255) & ~size_t(255)));
+  int *err = (int *)(cnt + SY_MAXT);
+  h->eig_err = err;
+  MH_HIP(h, hipMemsetAsync(err, 0, sizeof(int), h->stream));
+  // largest matrices first, in launches of at most SY_MAXT matrices whose
+  // workgroups are all resident at once
+  std::vector<int> order(nmat);
+  for (int m = 0; m < nmat; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+  size_t pos = 0;
+  while (pos < order.size()) {
+    SyTable tab{};
+    tab.xrow = vend;
+    int g = 0;
+    while (pos < order.size() && tab.nt < SY_MAXT) {
+      const int m = order[pos];
+      int G = std::min(SY_MAX_G, std::max(1, (n[m] + SY_COLS_PER_WG - 1) / SY_COLS_PER_WG));
+      G = std::min(G, std::max(1, n[m]));
+      if (g + G > st->coresident) {
+        if (tab.nt > 0) break;
+        G = st->coresident;  // a single matrix larger than the device: fewer groups
+      }
+      SyTask &tk = tab.t[tab.nt++];
+      tk.a_off = a_off[m];
+      tk.v_off = v_off[m];
+      tk.n = n[m];
+      tk.G = G;
+      tk.g0 = g;
+      g += G;
+      ++pos;
+    }
+    MH_HIP(h, hipMemsetAsync(cnt, 0, SY_MAXT * sizeof(unsigned), h->stream));
+    void *args[] = {&tab, &A, &D, &E, &tau, &P, &BC, &cnt, &err};
+    MH_HIP(h, hipLaunchCooperativeKernel((const void *)k_sytrd_group, dim3(g), dim3(SY_THREADS),
+                                         args, (unsigned)lds, h->stream));
+  }
+  return MH_OK;
+}
+
+extern "C" int mh_stedc_batched(mh_handle h, int nmat, const int *n, const int64_t *v_off,
+                                const int64_t *z_off, double *D, double *E, double *Z) {
+  if (!h || nmat < 0 || (nmat > 0 && (!n || !v_off || !z_off || !D || !E || !Z)))
+    return fail(h, MH_ERR_ARG, "mh_stedc_batched: null argument");
+  if (nmat == 0) return MH_OK;
+  for (int m = 0; m < nmat; ++m)
+    if (n[m] < 1) return fail(h, MH_ERR_SHAPE, "mh_stedc_batched: order %d", n[m]);
+  MH_HIP(h, hipSetDevice(h->device));
+  int k = 4;
+  if (const char *e = getenv("MH_EIG_STREAMS")) k = std::max(1, std::min(16, atoi(e)));
+  k = std::min(k, nmat);
+  MH_TRY(eig_streams(h, k));
+  EigState *st = state(h);
+  if (st->info_len < nmat) {
+    MH_HIP(h, hipStreamSynchronize(h->stream));
+    if (st->info) MH_HIP(h, hipFree(st->info));
+    st->info = nullptr;
+    st->info_len = 0;
+    MH_HIP(h, hipMalloc(&st->info, (size_t)std::max(nmat, 64) * sizeof(int)));
+    st->info_len = std::max(nmat, 64);
+  }
+  MH_HIP(h, hipMemsetAsync(st->info, 0, (size_t)nmat * sizeof(int), h->stream));
+  MH_HIP(h, hipEventRecord(st->ready, h->stream));
+  for (int q = 0; q < k; ++q) MH_HIP(h, hipStreamWaitEvent(st->streams[q], st->ready, 0));
+  // largest first, each to the stream with the least estimated work; one
+  // host thread per stream issues that stream's calls (a dstedc call can
+  // block its host thread, which would otherwise serialise the streams)
+  std::vector<int> order(nmat);
+  for (int m = 0; m < nmat; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+  std::vector<double> load(k, 0.0);
+  std::vector<std::vector<int>> work(k);
+  for (int m : order) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[q] += 0.5 + 0.003 * n[m];  // ms: a fixed latency plus a size term
+    work[q].push_back(m);
+  }
+  std::vector<int> bad(k, -1), bad_status(k, 0);
+  auto issue = [&](int q) {
+    (void)hipSetDevice(h->device);
+    for (int m : work[q]) {
+      rocblas_status rs = rocsolver_dstedc(st->rb[q], rocblas_evect_tridiagonal, n[m],
+                                           D + v_off[m], E + v_off[m], Z + z_off[m], n[m],
+                                           st->info + m);
+      if (rs != rocblas_status_success) {
+        bad[q] = m;
+        bad_status[q] = (int)rs;
+        return;
+      }
+    }
+  };
+  if (k == 1) {
+    issue(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int q = 0; q < k; ++q) th.emplace_back(issue, q);
+    for (auto &x : th) x.join();
+  }
+  for (int q = 0; q < k; ++q)
+    if (bad[q] >= 0)
+      return fail(h, MH_ERR_HIP, "rocsolver_dstedc (matrix %d, n = %d) failed: status %d", bad[q],
+                  n[bad[q]], bad_status[q]);
+  for (int q = 0; q < k; ++q) {
+    MH_HIP(h, hipEventRecord(st->done[q], st->streams[q]));
+    MH_HIP(h, hipStreamWaitEvent(h->stream, st->done[q], 0));
+  }
+  return MH_OK;
+}
+
+extern "C" int mh_ormtr_batched(mh_handle h, int nmat, const int *n, const int64_t *a_off,
+                                const int64_t *v_off, const double *A, const double *tau,
+                                const int64_t *z_off, const int *col0, const int *ncol, double *Z) {
+  if (!h || nmat < 0 ||
+      (nmat > 0 && (!n || !a_off || !v_off || !A || !tau || !z_off || !col0 || !ncol || !Z)))
+    return fail(h, MH_ERR_ARG, "mh_ormtr_batched: null argument");
+  int nmax = 0;
+  for (int m = 0; m < nmat; ++m) {
+    if (n[m] < 1 || n[m] > SY_MAXN || col0[m] < 0 || ncol[m] < 0 || col0[m] + ncol[m] > n[m])
+      return fail(h, MH_ERR_SHAPE, "mh_ormtr_batched: matrix %d: n %d, columns [%d, %d)", m, n[m],
+                  col0[m], col0[m] + ncol[m]);
+    nmax = std::max(nmax, n[m]);
+  }
+  MH_HIP(h, hipSetDevice(h->device));
+  const size_t lds = (size_t)OR_WAVES * nmax * sizeof(double);
+  int m = 0;
+  while (m < nmat) {
+    OrTable tab{};
+    int w = 0;
+    while (m < nmat && tab.nt < SY_MAXT) {
+      if (ncol[m] > 0) {
+        OrTask &tk = tab.t[tab.nt++];
+        tk.a_off = a_off[m];
+        tk.v_off = v_off[m];
+        tk.z_off = z_off[m];
+        tk.n = n[m];
+        tk.col0 = col0[m];
+        tk.ncol = ncol[m];
+        tk.w0 = w;
+        w += (ncol[m] + OR_WAVES - 1) / OR_WAVES;
+      }
+      ++m;
+    }
+    if (w == 0) continue;
+    hipLaunchKernelGGL(k_ormtr_cols, dim3(w), dim3(OR_THREADS), lds, h->stream, tab, A, tau, Z);
+    MH_TRY(check_launch(h, "k_ormtr_cols"));
+  }
+  return MH_OK;
+}
+
+extern "C" int mh_eig_check(mh_handle h, int nmat) {
+  if (!h || nmat < 0) return fail(h, MH_ERR_ARG, "mh_eig_check: bad argument");
+  MH_HIP(h, hipSetDevice(h->device));
+  MH_HIP(h, hipStreamSynchronize(h->stream));
+  EigState *st = state(h);
+  if (h->eig_err) {
+    int err = 0;
+    MH_HIP(h, hipMemcpy(&err, h->eig_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(h, MH_ERR_HIP, "mh_sytrd_batched: a group barrier timed out");
+  }
+  if (nmat > 0 && st->info) {
+    std::vector<int> info(std::min(nmat, st->info_len));
+    MH_HIP(h, hipMemcpy(info.data(), st->info, info.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t m = 0; m < info.size(); ++m)
+      if (info[m] != 0)
+        return fail(h, MH_ERR_NOCONV, "rocsolver_dstedc: matrix %zu did not converge (info %d)", m,
+                    info[m]);
+  }
+  return MH_OK;
+}

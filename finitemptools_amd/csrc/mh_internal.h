@@ -60,6 +60,12 @@ struct mh_context {
   // loop) instead of one grouped launch per stage.
   int bs_plain = 0;
   hipStream_t cap_stream = nullptr;
+  // Batched eigensolver (mh_eigen.hip): exchange rows, barrier counters and
+  // error flag of the tridiagonalisation; side streams and rocSOLVER handles.
+  void *eig = nullptr;
+  size_t eig_bytes = 0;
+  int *eig_err = nullptr;
+  void *eig_state = nullptr;
   int autotune = 1;
   int gemm_force = -1;  // FP64 GEMM configuration forced for every call (-1: tuned)
   int tune_log = 0;
@@ -155,6 +161,9 @@ int bs_apply_ws(mh_handle h, const mh_bs_plan_s &p, const void *x, const void *p
 int bs_prepare(mh_handle h, mh_bs_plan_s &p);
 // An operator plan (x and y of one size) with the environment buffers it needs.
 int bs_check_square(mh_handle h, const mh_bs_plan_s &p, const void *p1, const void *p2);
+
+// Release the eigensolver's streams, library handles and scratch.
+void eig_release(mh_handle h);
 
 inline size_t dtype_bytes(int dtype) { return dtype == MH_C64 ? 16 : 8; }
 

@@ -1,0 +1,122 @@
+"""Batched FP64 symmetric eigensolver on the device for the sector density
+matrices of the graded two-site split.
+
+The reference truncates every charge sector of the two-site tensor by a
+LAPACK SVD (`auxiliary/stable_svd.jl:4-18`, cut by `MPSTruncation`,
+`auxiliary/truncation.jl:27-47`); the device split takes u and s^2 from the
+eigenpairs of rho_c = M_c M_c^T instead.  `SectorEigh` runs those eigenpairs
+for all sectors of one bond as
+
+1. `mh_sytrd_batched`: Householder tridiagonalisation of every matrix in one
+   cooperative launch (LAPACK dsytrd, uplo = 'L', output format);
+2. `mh_stedc_batched`: rocSOLVER divide and conquer on each tridiagonal, the
+   calls spread over side streams;
+3. `mh_ormtr_batched`: back-transformation of only the eigenvector columns
+   the truncation keeps.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .engine import engine, ptr
+
+MAX_ORDER = 2048   # largest matrix order of mh_sytrd_batched
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _i64(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+class SectorEigh:
+    """Eigen-decomposition of a batch of symmetric FP64 matrices of orders
+    `sizes` on `device`.  Fill `matrix(m)` (an n x n view; symmetric, so row-
+    and column-major agree), call `eigenvalues()` (ascending per matrix, one
+    device tensor per matrix), then `vectors(keep)` for the top `keep[m]`
+    eigenvectors of each matrix (n x keep[m] views, ascending order)."""
+
+    def __init__(self, sizes, device):
+        self.sizes = [int(n) for n in sizes]
+        if any(n < 1 or n > MAX_ORDER for n in self.sizes):
+            raise _lib.DimensionMismatch(
+                2, f"SectorEigh: orders must lie in [1, {MAX_ORDER}]: {self.sizes}")
+        device = torch.device(device)
+        if device.type == "cuda" and device.index is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = device
+        n = np.array(self.sizes, dtype=np.int64)
+        sq = n * n
+        self.a_off = np.concatenate([[0], np.cumsum(sq)[:-1]]).astype(np.int64)
+        self.v_off = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+        tot_a, tot_v = int(sq.sum()), int(n.sum())
+        self.A = torch.empty(max(tot_a, 1), dtype=torch.float64, device=device)
+        self.Z = torch.empty(max(tot_a, 1), dtype=torch.float64, device=device)
+        self.D = torch.empty(max(tot_v, 1), dtype=torch.float64, device=device)
+        self.E = torch.empty(max(tot_v, 1), dtype=torch.float64, device=device)
+        self.tau = torch.empty(max(tot_v, 1), dtype=torch.float64, device=device)
+        self._n = _i32(n)
+        self._a = _i64(self.a_off)
+        self._v = _i64(self.v_off)
+
+    def matrix(self, m):
+        n, o = self.sizes[m], int(self.a_off[m])
+        return self.A[o:o + n * n].view(n, n)
+
+    def eigenvalues(self):
+        """Tridiagonalise and diagonalise every matrix (stream-ordered on the
+        engine's stream); returns [lambda_m ascending] as device views."""
+        e = engine(self.device)
+        h = e.bind_stream()
+        lib = _lib.load()
+        nm = len(self.sizes)
+        _lib.raise_for(lib.mh_sytrd_batched(h, nm, self._n[1], self._a[1], self._v[1],
+                                            ptr(self.A), ptr(self.D), ptr(self.E),
+                                            ptr(self.tau)), h, "mh_sytrd_batched")
+        _lib.raise_for(lib.mh_stedc_batched(h, nm, self._n[1], self._v[1], self._a[1],
+                                            ptr(self.D), ptr(self.E), ptr(self.Z)), h,
+                       "mh_stedc_batched")
+        return [self.D[int(o):int(o) + n] for o, n in zip(self.v_off, self.sizes)]
+
+    def check(self):
+        """Synchronise and raise if a barrier timed out or dstedc failed."""
+        e = engine(self.device)
+        h = e.bind_stream()
+        _lib.raise_for(_lib.load().mh_eig_check(h, len(self.sizes)), h, "mh_eig_check")
+
+    def vectors(self, keep):
+        """Back-transform the top keep[m] eigenvectors of every matrix; returns
+        [n_m x keep_m] device views (columns in ascending eigenvalue order)."""
+        keep = [int(k) for k in keep]
+        col0 = np.array([n - k for n, k in zip(self.sizes, keep)], dtype=np.int32)
+        e = engine(self.device)
+        h = e.bind_stream()
+        c0, k_ = _i32(col0), _i32(keep)
+        _lib.raise_for(_lib.load().mh_ormtr_batched(
+            h, len(self.sizes), self._n[1], self._a[1], self._v[1], ptr(self.A), ptr(self.tau),
+            self._a[1], c0[1], k_[1], ptr(self.Z)), h, "mh_ormtr_batched")
+        out = []
+        for m, (n, k) in enumerate(zip(self.sizes, keep)):
+            o = int(self.a_off[m])
+            Zm = self.Z[o:o + n * n].view(n, n).t()   # column-major n x n
+            out.append(Zm[:, n - k:])
+        return out
+
+
+def sector_eigh(mats, device=None):
+    """Full eigen-decomposition of symmetric FP64 device matrices (for tests):
+    [(lambda ascending, V)] with V column j the eigenvector of lambda[j]."""
+    device = device or mats[0].device
+    se = SectorEigh([m.shape[0] for m in mats], device)
+    for j, M in enumerate(mats):
+        se.matrix(j).copy_(M)
+    lams = se.eigenvalues()
+    V = se.vectors(se.sizes)
+    se.check()
+    return [(l.clone(), v.clone()) for l, v in zip(lams, V)]

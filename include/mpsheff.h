@@ -365,6 +365,35 @@ int mh_bs_lanczos_eig(mh_handle h, mh_bs_plan plan, const void *p1, const void *
                       const void *x0, int kmax, double btol, int reorth, double *eigval,
                       void *xout, int *info, int *niter);
 
+/* ---- batched symmetric eigensolver (block-sparse two-site split) ---------
+ * The FP64 step of the graded two-site split: the truncated SVD of every
+ * charge sector (TensorKit tsvd under auxiliary/stable_svd.jl:4-18,
+ * truncated by auxiliary/truncation.jl:27-47) is taken from the sector
+ * reduced density matrices rho_c = M_c M_c^T, whose eigenpairs give u and
+ * s^2.  nmat symmetric matrices: matrix m of order n[m] (1 <= n[m] <= 2048)
+ * is column-major with ld = n[m] at element offset a_off[m] of A; D, E and tau
+ * hold n[m] entries each from element offset v_off[m]; matrices and their
+ * D/E/tau ranges must not overlap.  Offsets and orders are host arrays.     */
+/* Householder tridiagonalisation, all matrices in one cooperative launch;
+ * output as LAPACK dsytrd with uplo = 'L' (reflector i in A[i+2:, i]). */
+int mh_sytrd_batched(mh_handle h, int nmat, const int *n, const int64_t *a_off,
+                     const int64_t *v_off, double *A, double *D, double *E, double *tau);
+/* Eigenvalues (ascending, into D) and eigenvectors of the tridiagonals
+ * (column-major, ld = n[m], at element offset z_off[m] of Z) by rocSOLVER
+ * dstedc, one call per matrix on side streams; E is overwritten. */
+int mh_stedc_batched(mh_handle h, int nmat, const int *n, const int64_t *v_off,
+                     const int64_t *z_off, double *D, double *E, double *Z);
+/* Columns [col0[m], col0[m] + ncol[m]) of Z_m <- Q_m Z_m, Q_m the product of
+ * the reflectors mh_sytrd_batched left in A and tau: eigenvectors of the
+ * original matrices. */
+int mh_ormtr_batched(mh_handle h, int nmat, const int *n, const int64_t *a_off,
+                     const int64_t *v_off, const double *A, const double *tau,
+                     const int64_t *z_off, const int *col0, const int *ncol, double *Z);
+/* Waits for the handle's stream; MH_ERR_NOCONV if a dstedc call of the last
+ * mh_stedc_batched (nmat matrices) failed to converge, MH_ERR_HIP if a
+ * group barrier of the last tridiagonalisation timed out. */
+int mh_eig_check(mh_handle h, int nmat);
+
 /* ---- measurement --------------------------------------------------------- */
 enum mh_kernel_class { MH_K_GEMM = 0, MH_K_WSTAGE = 1, MH_K_VEC = 2, MH_K_NCLASS = 3 };
 /* enable = 1: bracket every launch with HIP events on the handle's stream */

@@ -1,0 +1,145 @@
+"""Batched device eigensolver (mh_sytrd_batched / mh_stedc_batched /
+mh_ormtr_batched) against LAPACK and the spectral theorem.
+
+Pins: the tridiagonalisation against scipy's LAPACK dsytrd (uplo = 'L': same
+Householder convention, so D, E, tau and the stored reflectors agree entry by
+entry); eigenvalues against numpy's eigvalsh; eigenvectors by residual
+||A v - lambda v|| and orthonormality; the selected-column back-transform
+against the full one."""
+import numpy as np
+import pytest
+import scipy.linalg.lapack as lapack
+
+from finitemptools_amd import eigen
+
+pytestmark = pytest.mark.gpu
+
+
+def density_matrix(rng, n, decay=20.0):
+    """rho = M M^T with a graded spectrum, like a sector reduced density matrix."""
+    M = rng.standard_normal((n, n)) * np.exp(-decay * np.arange(n) / max(n, 1))
+    return M @ M.T
+
+
+def random_symmetric(rng, n):
+    A = rng.standard_normal((n, n))
+    return A + A.T
+
+
+def _mats(rng, sizes):
+    out = []
+    for j, n in enumerate(sizes):
+        out.append(density_matrix(rng, n) if j % 2 == 0 else random_symmetric(rng, n))
+    return out
+
+
+def test_sytrd_matches_lapack(gpu):
+    import torch
+    rng = np.random.default_rng(1)
+    sizes = [1, 2, 3, 5, 17, 64, 100, 257, 600, 33]
+    mats = _mats(rng, sizes)
+    se = eigen.SectorEigh(sizes, "cuda")
+    for j, M in enumerate(mats):
+        se.matrix(j).copy_(torch.from_numpy(M))
+    from finitemptools_amd import _lib
+    from finitemptools_amd.engine import ptr
+    h = gpu.bind_stream()
+    lib = _lib.load()
+    st = lib.mh_sytrd_batched(h, len(sizes), se._n[1], se._a[1], se._v[1], ptr(se.A), ptr(se.D),
+                              ptr(se.E), ptr(se.tau))
+    _lib.raise_for(st, h)
+    _lib.raise_for(lib.mh_eig_check(h, 0), h)
+    A = se.A.cpu().numpy()
+    D, E, tau = se.D.cpu().numpy(), se.E.cpu().numpy(), se.tau.cpu().numpy()
+    for j, (n, M) in enumerate(zip(sizes, mats)):
+        a_ref, d_ref, e_ref, tau_ref, info = lapack.dsytrd(M, lower=1)
+        assert info == 0
+        scale = max(np.abs(M).max(), 1e-300)
+        vo, ao = int(se.v_off[j]), int(se.a_off[j])
+        # backward-stable reductions agree to O(n eps ||A||)
+        tol = 1e-13 * max(n, 10) * scale
+        np.testing.assert_allclose(D[vo:vo + n], d_ref, atol=tol, rtol=0)
+        np.testing.assert_allclose(E[vo:vo + n - 1], e_ref, atol=tol, rtol=0)
+        Am = A[ao:ao + n * n].reshape(n, n).T   # column-major storage
+        # a reflector is determined by its column only where that column lies
+        # above the rounding level of the trailing block (graded spectra end
+        # in columns of pure rounding noise, whose reflectors are arbitrary)
+        for i in range(max(n - 2, 0)):
+            if abs(e_ref[i]) >= 1e-3 * scale:
+                assert abs(tau[vo + i] - tau_ref[i]) <= 1e-8
+                np.testing.assert_allclose(Am[i + 2:, i], a_ref[i + 2:, i], atol=1e-8, rtol=0)
+
+
+@pytest.mark.parametrize("sizes", [[1, 2, 7, 64, 300, 801], list(range(1, 71))])
+def test_eigenpairs(gpu, sizes):
+    """Eigenvalues vs eigvalsh, residuals and orthonormality; the second case
+    spans more matrices than one launch's table holds."""
+    import torch
+    rng = np.random.default_rng(len(sizes))
+    mats = _mats(rng, sizes)
+    res = eigen.sector_eigh([torch.from_numpy(M).cuda() for M in mats])
+    for M, (lam, V) in zip(mats, res):
+        n = M.shape[0]
+        lam, V = lam.cpu().numpy(), V.cpu().numpy()
+        scale = max(np.abs(np.linalg.eigvalsh(M)).max(), 1e-300)
+        np.testing.assert_allclose(lam, np.linalg.eigvalsh(M), atol=1e-12 * scale, rtol=0)
+        assert np.abs(M @ V - V * lam).max() <= 1e-12 * scale * max(1.0, np.sqrt(n))
+        assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-12 * n
+
+
+def test_selected_columns_and_many_workgroups(gpu):
+    """Top-k back-transform equals the same columns of the full one; 40
+    matrices of order 1000 need more workgroups than are resident at once,
+    so the tridiagonalisation runs as several launches."""
+    import torch
+    rng = np.random.default_rng(7)
+    sizes = [1000] * 40 + [3, 1]
+    mats = [density_matrix(rng, n, decay=8.0) for n in sizes]
+    full = eigen.sector_eigh([torch.from_numpy(M).cuda() for M in mats])
+    se = eigen.SectorEigh(sizes, "cuda")
+    for j, M in enumerate(mats):
+        se.matrix(j).copy_(torch.from_numpy(M))
+    se.eigenvalues()
+    keep = [min(n, 250) for n in sizes]
+    V = se.vectors(keep)
+    se.check()
+    for (lam, Vf), Vk, k, M in zip(full, V, keep, mats):
+        torch.testing.assert_close(Vk, Vf[:, Vf.shape[1] - k:], atol=0, rtol=0)
+        scale = float(lam.abs().max())
+        Vk = Vk.cpu().numpy()
+        lk = lam.cpu().numpy()[len(lam) - k:]
+        assert np.abs(M @ Vk - Vk * lk).max() <= 1e-11 * scale
+
+
+@pytest.mark.parametrize("absorb", ["left", "right"])
+@pytest.mark.parametrize("solver", ["sector", "library"])
+def test_device_split_matches_svd_split(gpu, absorb, solver, monkeypatch):
+    """The device density-matrix split (either sector eigensolver) keeps the
+    singular values of the host sector SVDs and reproduces the same truncated
+    two-site tensor."""
+    import torch
+    from finitemptools_amd import blockmps as bm
+    from finitemptools_amd import blocksparse as bs
+    from finitemptools_amd.truncation import MPSTruncation
+    from plan_exec import execute
+    from test_blocksparse import rand_space, reachable
+    monkeypatch.setattr(bm, "EIG_SOLVER", solver)
+    rng = np.random.default_rng(31)
+    phys = bm.hubbard_charges()
+    Sl = rand_space(rng, [(a, b) for a in range(1, 4) for b in range(1, 4)], 4, 30)
+    Sr = rand_space(rng, reachable(Sl, phys, 2), 4, 30)
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    x = rng.standard_normal(xl.size)
+    tmap = bm.TwoSiteMap(xl)
+    tr = MPSTruncation(D=150, eps=1e-10)
+    a1, a2, s_ref = bm.split_two_site(tmap, x, tr, absorb, "cpu")
+    b1, b2, s_dev = bm.split_two_site_rho(tmap, torch.as_tensor(x).cuda(), tr, absorb)
+    assert sorted(s_ref) == sorted(s_dev)
+    for c in s_ref:
+        np.testing.assert_allclose(s_dev[c], s_ref[c], atol=1e-12, rtol=0)
+    rec = []
+    for A1, A2 in ((a1, a2), (b1, b2)):
+        tab = bs.plan_two_site(A1.layout, A2.layout, xl)
+        rec.append(execute(tab, [A1.data.cpu().numpy(), np.zeros(xl.size), A2.data.cpu().numpy(),
+                                 None, None, None])[1])
+    np.testing.assert_allclose(rec[1], rec[0], atol=1e-11, rtol=0)

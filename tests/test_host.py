@@ -75,3 +75,13 @@ def test_density_matrix_split_equals_truncated_svd(cplx, absorb):
     assert np.allclose(rec, ref, atol=1e-12)
     iso = as_matrix(A1, 2) if absorb == "right" else as_matrix(A2, 1).conj().t()
     assert np.allclose((iso.conj().t() @ iso).numpy(), np.eye(iso.shape[1]), atol=1e-12)
+
+
+def test_sector_eigh_rejects_orders_beyond_the_kernel():
+    """Orders outside [1, 2048] are refused on the host, before any device work."""
+    import pytest
+    from finitemptools_amd import _lib, eigen
+    with pytest.raises(_lib.DimensionMismatch):
+        eigen.SectorEigh([4, 4096], "cpu")
+    with pytest.raises(_lib.DimensionMismatch):
+        eigen.SectorEigh([0], "cpu")
