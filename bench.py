@@ -208,7 +208,7 @@ def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
     # (b) one sweep per precision from the same state
     base_sites = [bm.BlockTensor(t.layout, t.data.clone()) for t in psi.sites]
     base_env = [bm.BlockEnv(q.layout, q.data.clone()) for q in env.hstorage]
-    sweeps = {}
+    sweeps, schedule = {}, {}
     for prec in ("f64", "f32", "bf16x3"):
         psi.sites = [bm.BlockTensor(t.layout, t.data.clone()) for t in base_sites]
         env.hstorage = [bm.BlockEnv(q.layout, q.data.clone()) for q in base_env]
@@ -223,6 +223,15 @@ def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
                         "lanczos_s": round(st["lanczos_s"], 3), "split_s": round(st.get("split_s", 0), 3),
                         "plan_s": round(st.get("plan_s", 0), 3), "env_s": round(st["env_s"], 3),
                         "lanczos_tflops": round(st["matvec_flops"] / max(st["lanczos_s"], 1e-9) / 1e12, 3)}
+        # schedule: this sweep, then one FP64 polish sweep from its state
+        env.prec = "f64"
+        alg = DMRG2(trunc=MPSTruncation(D=ramp[-1], eps=1e-10), prec="f64")
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        e2, _ = bm.sweep(env, alg)
+        torch.cuda.synchronize()
+        schedule[f"{prec}+f64"] = {"s": round(sweeps[prec]["s"] + time.perf_counter() - ts, 3),
+                                   "E_per_site": e2[-1] / L}
     # the same sweep as the plain per-sector FP64 loop: one GEMM launch per
     # sector product in every plan, one eigensolve per sector in sequence
     psi.sites = [bm.BlockTensor(t.layout, t.data.clone()) for t in base_sites]
@@ -246,9 +255,17 @@ def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
         sweeps[prec]["speedup_vs_f64"] = round(sweeps["f64"]["s"] / sweeps[prec]["s"], 3)
     for prec in ("f64", "f32", "bf16x3"):
         sweeps[prec]["speedup_vs_plain_f64"] = round(sweeps["f64_plain_per_sector"]["s"] / sweeps[prec]["s"], 3)
+    # mixed sweep + FP64 polish against two FP64 sweeps from the same state:
+    # the energy criterion is |dE/L| <= 1e-9
+    ref = schedule["f64+f64"]
+    for prec in ("f32", "bf16x3"):
+        ent = schedule[f"{prec}+f64"]
+        ent["dE_per_site_vs_f64+f64"] = ent["E_per_site"] - ref["E_per_site"]
+        ent["meets_1e-9"] = bool(abs(ent["dE_per_site_vs_f64+f64"]) <= 1e-9)
+        ent["speedup_vs_f64+f64"] = round(ref["s"] / ent["s"], 3)
     return {"workload": f"Hubbard L={L} t=1 U=4 half filling U1xU1, DMRG2 ramp {list(ramp)}",
             "ramp_s": round(ramp_s, 2), "ramp": ramp_log, "matvec_middle_bond": mv,
-            "sweep_D2048": sweeps}
+            "sweep_D2048": sweeps, "schedule_mixed_then_f64": schedule}
 
 
 def xxz_dmrg_L20_D64(cpu=True):
