@@ -95,6 +95,58 @@ def gpu_run(kind, D, steps, warmup, prec="f64", profile=False):
             "y0": yd[0].cpu().numpy(), "inputs": (W, L, R, xs), "prof": prof}
 
 
+def complex_tfim_matvec(D=512, nvec=8, steps=5, precs=("f64", "f32", "bf16x3")):
+    """The unit of work of the TDVP quench at its bond-dimension cap: the
+    two-site H_eff of the TFIM chain (w = 3, d = 2) on ComplexF64 data at
+    D = 512, 4 x F2 = 26.1 GFLOP of real arithmetic per matvec; seeded complex
+    Gaussian environments (std 1/sqrt(D w)) and unit vectors; one vector
+    checked against the oracle."""
+    import torch
+    from finitemptools_amd import models
+    from finitemptools_amd.derivatives import HeffOperator
+    from finitemptools_amd.engine import colmajor_empty, to_device
+    from oracle import contractions as C
+    W = models.tfim_mpo(4, 1.0, 1.0, dtype=np.complex128)[1]
+    w, d = W.shape[0], W.shape[1]
+    rng = np.random.default_rng(3)
+    std = 1.0 / np.sqrt(2 * D * w)
+    L = (rng.standard_normal((D, w, D)) + 1j * rng.standard_normal((D, w, D))) * std
+    R = (rng.standard_normal((D, w, D)) + 1j * rng.standard_normal((D, w, D))) * std
+    xs = []
+    for i in range(nvec):
+        r = np.random.default_rng(2000 + i)
+        x = r.standard_normal((D, d, d, D)) + 1j * r.standard_normal((D, d, d, D))
+        xs.append(x / np.linalg.norm(x))
+    F = 4.0 * flops(2, D, w=w, d=d)
+    Wd, Ld, Rd = to_device(W), to_device(L), to_device(R)
+    xd = [to_device(x) for x in xs]
+    out = {"workload": f"ac2_prime two-site H_eff, TFIM w={w}, d={d}, D={D}, ComplexF64 storage",
+           "gflop_per_matvec": round(F / 1e9, 2)}
+    yref = None
+    for prec in precs:
+        op = HeffOperator(2, Ld, Rd, Wd, Wd, prec=prec)
+        yd = [colmajor_empty(op.shape, dtype=torch.complex128) for _ in xs]
+        for x, y in zip(xd, yd):
+            op.apply(x, out=y)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for x, y in zip(xd, yd):
+                op.apply(x, out=y)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / (steps * nvec)
+        ent = {"ms": round(sec * 1e3, 4), "tflops": round(F / sec / 1e12, 3)}
+        y0 = yd[0].cpu().numpy()
+        if yref is None:
+            yref = C.ac2_prime(xs[0], W, W, L, R)
+            ent["rel_err_vs_oracle"] = float(np.linalg.norm(y0 - yref) / np.linalg.norm(yref))
+        else:
+            ent["rel_err_vs_oracle"] = float(np.linalg.norm(y0 - yref) / np.linalg.norm(yref))
+            ent["speedup_vs_f64"] = round(out["f64"]["ms"] / ent["ms"], 3)
+        out[prec] = ent
+    return out
+
+
 def cpu_baseline(kind, D, budget_s=12.0):
     """The oracle (numpy + host BLAS, same pairwise order as the reference)
     on the host cores: as many of the same matvecs as fit in the budget."""
@@ -432,8 +484,8 @@ def main():
     ap.add_argument("--no-blocksparse", action="store_true")
     ap.add_argument("--no-gradients", action="store_true")
     ap.add_argument("--no-tdvp", action="store_true")
-    ap.add_argument("--only", choices=("all", "blocksparse"), default="all",
-                    help="blocksparse: run only the block-sparse section (profiling)")
+    ap.add_argument("--only", choices=("all", "blocksparse", "complex"), default="all",
+                    help="run only the block-sparse section or the complex matvec (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="first copy MiB of device memory three times (counter calibration)")
     args = ap.parse_args()
@@ -459,6 +511,9 @@ def main():
         del a, b
     if args.only == "blocksparse":
         print(json.dumps({"blocksparse_hubbard_L64_D2048": blocksparse_hubbard()}))
+        return
+    if args.only == "complex":
+        print(json.dumps({"ac2_prime_D512_c64_tfim": complex_tfim_matvec()}))
         return
 
     D2, D1 = 512, 1024
@@ -534,6 +589,8 @@ def main():
     if not args.no_tdvp:
         print("tdvp TFIM L=100 ...", file=sys.stderr, flush=True)
         out["tdvp_tfim_L100_quench"] = tdvp_tfim_quench()
+        print("complex TFIM matvec D=512 ...", file=sys.stderr, flush=True)
+        out["ac2_prime_D512_c64_tfim"] = complex_tfim_matvec()
     print("xxz L=20 D=64 ...", file=sys.stderr, flush=True)
     out["dmrg_xxz_L20_D64"] = xxz_dmrg_L20_D64(cpu=not args.no_cpu)
     if not args.no_cpu:
