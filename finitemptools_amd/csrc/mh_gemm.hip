@@ -71,10 +71,15 @@ struct Stage {
 // staged step, so each wave keeps a 2x-larger tile (more MFMAs per LDS
 // fragment) while two waves still share each SIMD; the second grid's tile is
 // added to the first's through LDS after the K loop (a fixed order).
-template <bool CPLX, int BM, int BN, int BK, bool AMC, bool BKC, bool VEC, int NW = 4, int KG = 1>
+// PF = 2: the global -> register prefetch runs two K steps ahead (two
+// register sets, the K loop unrolled by two), so a tile's loads have two
+// compute phases to land instead of one.
+template <bool CPLX, int BM, int BN, int BK, bool AMC, bool BKC, bool VEC, int NW = 4, int KG = 1,
+          int PF = 1>
 __global__ void __launch_bounds__(64 * NW)
     k_gemm(const GemmP p) {
   static_assert(KG == 1 || (KG == 2 && NW == 8 && BK % 32 == 0), "K-grouped waves");
+  static_assert(PF == 1 || PF == 2, "prefetch depth");
   constexpr int WGM = 2, WGN = NW / (2 * KG);
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
@@ -132,12 +137,13 @@ __global__ void __launch_bounds__(64 * NW)
     }
   }
 
-  double ra[NP][SA::NPAIR][2], rb[NP][SB::NPAIR][2];
+  double ra0[NP][SA::NPAIR][2], rb0[NP][SB::NPAIR][2];
+  double ra1[NP][SA::NPAIR][2], rb1[NP][SB::NPAIR][2];
 
   // Global -> registers, branch-free: k is clamped into [0, K) (a pair along
   // k into [0, K-1)), rows were clamped once above, so every address is
   // inside the matrix; out-of-range values are zeroed at the LDS store.
-  auto load = [&](int k0) {
+  auto load = [&](double (&ra)[NP][SA::NPAIR][2], double (&rb)[NP][SB::NPAIR][2], int k0) {
 #pragma unroll
     for (int i = 0; i < SA::NPAIR; ++i) {
       int r, k;
@@ -202,7 +208,8 @@ __global__ void __launch_bounds__(64 * NW)
 
   // Registers -> LDS (masked: rows outside the matrix and k outside the
   // split's range become zero).
-  auto store = [&](int buf, int k0) {
+  auto store = [&](const double (&ra)[NP][SA::NPAIR][2], const double (&rb)[NP][SB::NPAIR][2],
+                   int buf, int k0) {
 #pragma unroll
     for (int i = 0; i < SA::NPAIR; ++i) {
       int r, k;
@@ -326,21 +333,46 @@ __global__ void __launch_bounds__(64 * NW)
   };
 
   const int nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-  if (nk > 0) {
-    load(kbeg);
-    store(0, kbeg);
+  if (nk > 0 && PF == 1) {
+    load(ra0, rb0, kbeg);
+    store(ra0, rb0, 0, kbeg);
     __syncthreads();
     // Steady state: prefetch step kt+1 into registers, run step kt from LDS,
     // then write the prefetched tile to the other LDS buffer.  The last
     // compute is peeled so the loop body has no branches.
     for (int kt = 0; kt + 1 < nk; ++kt) {
       const int k1 = kbeg + (kt + 1) * BK;
-      load(k1);
+      load(ra0, rb0, k1);
       compute(kt & 1);
-      store((kt + 1) & 1, k1);
+      store(ra0, rb0, (kt + 1) & 1, k1);
       __syncthreads();
     }
     compute((nk - 1) & 1);
+  } else if (nk > 0) {
+    // Two register sets: at the top of each pair of steps LDS buffer 0 holds
+    // step kt and set 1 holds step kt+1 (in flight).  Loads of steps past the
+    // end read clamped addresses and are never stored.
+    load(ra0, rb0, kbeg);
+    store(ra0, rb0, 0, kbeg);
+    load(ra1, rb1, kbeg + BK);
+    __syncthreads();
+    int kt = 0;
+    for (; kt + 2 < nk; kt += 2) {
+      load(ra0, rb0, kbeg + (kt + 2) * BK);
+      compute(0);
+      store(ra1, rb1, 1, kbeg + (kt + 1) * BK);
+      __syncthreads();
+      load(ra1, rb1, kbeg + (kt + 3) * BK);
+      compute(1);
+      store(ra0, rb0, 0, kbeg + (kt + 2) * BK);
+      __syncthreads();
+    }
+    compute(0);
+    if (kt + 1 < nk) {
+      store(ra1, rb1, 1, kbeg + (kt + 1) * BK);
+      __syncthreads();
+      compute(1);
+    }
   }
 
   if constexpr (KG == 2) {
@@ -432,6 +464,9 @@ static const Cand kCands[] = {
     // eight waves as two K-slice groups of a 2x2 wave grid
     {64, 64, 32, 1, 0, 8, 2}, {64, 64, 32, 1, 8, 8, 2}, {128, 64, 32, 1, 0, 8, 2},
     {64, 64, 64, 1, 0, 8, 2},
+    // the same eight-wave forms with the two-step register prefetch (real)
+    {64, 64, 32, 1, 0, 8, 2, 2}, {64, 64, 64, 1, 0, 8, 2, 2}, {128, 64, 32, 1, 0, 8, 2, 2},
+    {64, 64, 32, 1, 8, 8, 2, 2}, {64, 64, 32, 1, 0, 8, 0, 2}, {64, 64, 16, 1, 0, 8, 0, 2},
 };
 constexpr int kNCand = sizeof(kCands) / sizeof(kCands[0]);
 
@@ -444,7 +479,22 @@ static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
   hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC, 8>), grid, dim3(512), 0, s, p)
 #define MH_L8K(BM, BN, BK) \
   hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC, 8, 2>), grid, dim3(512), 0, s, p)
-  if (c.kgroups == 2) {
+#define MH_L8KP(BM, BN, BK) \
+  hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC, 8, 2, 2>), grid, dim3(512), 0, s, p)
+#define MH_L8P(BM, BN, BK) \
+  hipLaunchKernelGGL((k_gemm<CPLX, BM, BN, BK, AMC, BKC, VEC, 8, 1, 2>), grid, dim3(512), 0, s, p)
+  if (c.prefetch == 2) {
+    if constexpr (!CPLX) {  // real only (registers)
+      if (c.kgroups == 2) {
+        if (c.bm == 128) MH_L8KP(128, 64, 32);
+        else if (c.bk == 64) MH_L8KP(64, 64, 64);
+        else MH_L8KP(64, 64, 32);
+      } else {
+        if (c.bk == 32) MH_L8P(64, 64, 32);
+        else MH_L8P(64, 64, 16);
+      }
+    }
+  } else if (c.kgroups == 2) {
     if constexpr (!CPLX) {  // real only (LDS and registers)
       if (c.bm == 128) MH_L8K(128, 64, 32);
       else if (c.bk == 64) MH_L8K(64, 64, 64);
@@ -463,6 +513,8 @@ static void launch_cfg(const Cand &c, GemmP p, hipStream_t s) {
 #undef MH_L
 #undef MH_L8
 #undef MH_L8K
+#undef MH_L8KP
+#undef MH_L8P
 }
 
 template <bool CPLX, bool VEC>
@@ -501,6 +553,7 @@ static bool cand_ok_f64(const Cand &c, const GemmP &p, bool cplx) {
   if (c.split > 1 && p.K < 256 * c.split) return false;
   if (cplx && c.bk == 32) return false;              // LDS: keep <= 80 KB
   if (cplx && c.kgroups == 2) return false;
+  if (cplx && c.prefetch == 2) return false;
   if (cplx && c.bm * c.bn > 64 * 128) return false;  // register pressure
   return true;
 }
@@ -860,9 +913,10 @@ static int tune(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx, b
     MH_HIP(h, hipEventElapsedTime(&ms, e0, e1));
     ms /= 3.f;
     if (h->tune_log)
-      fprintf(stderr, "[mh tune] %s%s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d waves=%d kgroups=%d: %.1f us (%.2f TFLOP/s)\n",
+      fprintf(stderr, "[mh tune] %s%s M=%d N=%d K=%d amc=%d bkc=%d tile=%dx%dx%d split=%d group=%d waves=%d kgroups=%d prefetch=%d: %.1f us (%.2f TFLOP/s)\n",
               fam.name, cplx ? "/c" : "", p.M, p.N, p.K, (int)amc, (int)bkc, c.bm, c.bn, c.bk,
-              c.split, c.group, c.waves ? c.waves : 4, c.kgroups ? c.kgroups : 1, ms * 1e3,
+              c.split, c.group, c.waves ? c.waves : 4, c.kgroups ? c.kgroups : 1,
+              c.prefetch == 2 ? 2 : 1, ms * 1e3,
               2.0 * p.M * (double)p.N * p.K * (cplx ? 4 : 1) / (ms * 1e-3) / 1e12);
     if (ms < best) {
       best = ms;

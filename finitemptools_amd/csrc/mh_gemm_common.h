@@ -40,6 +40,7 @@ struct Cand {
   int waves;  // waves per block; 0 means the family default (4)
   int kgroups;  // 2: the 8 waves form two 2x2 wave grids that take alternate
                 // 16-deep K slices and add their tiles at the end (0/1: off)
+  int prefetch; // 2: global loads run two K steps ahead (0/1: one step)
 };
 
 struct GemmFamily {
