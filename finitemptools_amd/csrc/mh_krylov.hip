@@ -962,7 +962,11 @@ static int run_steps(mh_handle h, Krylov &K, int kmax, double btol, int reorth, 
   };
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(h->stream, &cs);
-  if (!h->graphs || h->profiling || cs != hipStreamCaptureStatusNone) return eager();
+  // short-vector steps are a few launches each: capture and instantiation
+  // cost more than the graph saves unless capture is forced (graphs == 2)
+  if (!h->graphs || (h->graphs == 1 && K.small()) || h->profiling ||
+      cs != hipStreamCaptureStatusNone)
+    return eager();
   if (!h->cap_stream &&
       hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking) != hipSuccess) {
     (void)hipGetLastError();

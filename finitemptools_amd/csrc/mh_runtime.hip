@@ -1,4 +1,5 @@
 // Handle lifecycle, error reporting, workspace and launch profiling.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -132,8 +133,9 @@ int mh_create(int device, mh_handle *out) {
   // MH_AUTOTUNE=0 disables per-shape GEMM tuning; MH_TUNE_LOG=1 prints it.
   if (const char *s = getenv("MH_AUTOTUNE")) h->autotune = atoi(s) != 0;
   if (const char *s = getenv("MH_TUNE_LOG")) h->tune_log = atoi(s) != 0;
-  // MH_GRAPHS=0: run the Krylov recurrences eagerly instead of as a hipGraph.
-  if (const char *s = getenv("MH_GRAPHS")) h->graphs = atoi(s) != 0;
+  // MH_GRAPHS=0: run the Krylov recurrences eagerly instead of as a hipGraph;
+  // MH_GRAPHS=2: capture the short-vector steps too.
+  if (const char *s = getenv("MH_GRAPHS")) h->graphs = std::min(2, std::max(0, atoi(s)));
   // MH_FUSION=0: run the first H_eff step as GEMM + separate mode product.
   if (const char *s = getenv("MH_FUSION")) h->fusion = atoi(s) != 0;
   // MH_KRYLOV_SMALL=0: multi-block vector kernels for every Krylov step.
@@ -166,7 +168,7 @@ int mh_destroy(mh_handle h) {
 
 int mh_set_graphs(mh_handle h, int enable) {
   if (!h) return MH_ERR_ARG;
-  h->graphs = enable ? 1 : 0;
+  h->graphs = enable < 0 ? 0 : (enable > 2 ? 2 : enable);
   return MH_OK;
 }
 

@@ -110,9 +110,11 @@ class Engine:
     def profile_reset(self):
         self.check(self.lib.mh_profile_reset(self.h))
 
-    def set_graphs(self, on=True):
-        """Krylov recurrences as hipGraphs (default) or eager launches."""
-        self.check(self.lib.mh_set_graphs(self.h, 1 if on else 0))
+    def set_graphs(self, mode=1):
+        """Krylov recurrences as hipGraphs: 1 (default, True) captures the
+        multi-block steps and runs the short-vector steps eagerly, 2 captures
+        both, 0 (False) captures nothing."""
+        self.check(self.lib.mh_set_graphs(self.h, int(mode)))
 
     def set_krylov_small(self, on=True):
         """Single-workgroup Krylov steps for short vectors (default) or the

@@ -179,10 +179,13 @@ int mh_arnoldi_expm(mh_handle h, const mh_heff_op *op, const void *x0,
                     double dt_re, double dt_im, int kdim, double tol,
                     void *xout, int *info);
 
-/* enable = 1 (default; environment MH_GRAPHS=0 turns it off): the Lanczos
+/* enable = 1 (default; environment MH_GRAPHS sets 0/1/2): the Lanczos
  * steps after the first of mh_lanczos_eig / mh_bs_lanczos_eig /
  * mh_lanczos_expm are captured into one hipGraph and launched as a whole
- * (exponentiate re-launches it for every sub-step).  Results are identical
+ * (exponentiate re-launches it for every sub-step), except steps on short
+ * vectors (the single-workgroup step kernel, a few launches per step),
+ * which run eagerly because capturing them costs more than it saves;
+ * enable = 2 captures those too; 0 captures nothing.  Results are identical
  * to the eager launches. */
 int mh_set_graphs(mh_handle h, int enable);
 /* enable = 0: run the first step of the FP64 one-/two-site H_eff as a GEMM

@@ -23,11 +23,12 @@ def small(request):
 
 
 def both(fn):
-    """fn() with graphs off, then on (twice), with unrelated eager work between."""
+    """fn() with graphs off, then on (twice), with unrelated eager work between;
+    capture is forced (mode 2) so the short-vector steps are captured too."""
     eng = engine(0)
     out = []
     try:
-        for on in (False, True, True):
+        for on in (0, 2, 2):
             eng.set_graphs(on)
             a = torch.randn(4096, dtype=torch.float64, device="cuda")
             (a @ a).item()
