@@ -27,6 +27,8 @@
 #include <cstring>
 #include <vector>
 
+#include <map>
+
 #include "mh_gemm_common.h"
 
 namespace mh {
@@ -44,7 +46,18 @@ struct GSeg {       // one K-segment of a task
   int K, lda, ldb, pad;
 };
 struct GTile {      // task < 0: padding entry of the XCD-ordered grid
-  int task, m0, n0, pad;
+  int task, m0, n0, pad;   // task: a stack; m0: first row within the stack
+};
+// Tasks that share N and the B side of every K-segment (offset, K, ldb) are
+// stacked along M: one GEMM of sum(M) rows whose row r comes from task
+// rows[row0 + r].task, local row rows[row0 + r].row.  A tile then covers the
+// rows of several small blocks instead of padding each to the tile height.
+struct GStack {
+  int M, N, nseg, task0;   // task0: the task whose segments give the B side
+  int row0, pad;
+};
+struct GRow {
+  int task, row;
 };
 struct MChunk {     // up to 1024 consecutive elements of one destination block
   int64_t dst_off;
@@ -83,12 +96,16 @@ struct BsStage {
   double flops;
   // Device copies of the tables (set when the plan is uploaded).
   GTile *tiles_v[NTS];
+  GStack *stacks;
+  GRow *rows;
   GTask *tasks;
   GSeg *segs;
   MChunk *chunks;
   MContrib *contribs;
   // Host tables: the plan is built on the host and uploaded on first use.
   std::vector<GTile> h_tiles[NTS];
+  std::vector<GStack> h_stacks;
+  std::vector<GRow> h_rows;
   std::vector<GTask> h_tasks;
   std::vector<GSeg> h_segs;
   std::vector<MChunk> h_chunks;
@@ -137,7 +154,8 @@ struct GStage {
 
 template <int BM, int BN, int BK, bool AMC, bool BKC>
 __global__ void __launch_bounds__(256)
-    k_ggemm(const GTile *__restrict__ tiles, const GTask *__restrict__ tasks,
+    k_ggemm(const GTile *__restrict__ tiles, const GStack *__restrict__ stacks,
+            const GRow *__restrict__ rows, const GTask *__restrict__ tasks,
             const GSeg *__restrict__ segs, const double *__restrict__ Abase,
             const double *__restrict__ Bbase, double *__restrict__ Cbase) {
   constexpr int WTM = BM / 2, WTN = BN / 2;
@@ -148,11 +166,26 @@ __global__ void __launch_bounds__(256)
   __shared__ __align__(16) double sB[2][SB::SIZE];
   const GTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;
-  const GTask t = tasks[tl.task];
+  const GStack S = stacks[tl.task];
+  const GTask t = tasks[S.task0];  // B side and K of every segment
   const int m0 = tl.m0, n0 = tl.n0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int lr = lane & 15, lk = lane >> 4;
+  // the task (its first segment) and local row of each staged A row,
+  // clamped into the stack (values of rows past it are masked at the store)
+  int aseg[GStage<BM, BK, AMC>::NPAIR][2], alrow[GStage<BM, BK, AMC>::NPAIR][2];
+#pragma unroll
+  for (int i = 0; i < GStage<BM, BK, AMC>::NPAIR; ++i) {
+    int r, k;
+    GStage<BM, BK, AMC>::coord(tid, i, r, k);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const GRow rw = rows[S.row0 + min(m0 + r + (AMC ? e : 0), S.M - 1)];
+      aseg[i][e] = tasks[rw.task].seg0;
+      alrow[i][e] = rw.row;
+    }
+  }
 
   d4_t acc[TM][TN];
 #pragma unroll
@@ -160,15 +193,15 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
 
-  for (int sg = 0; sg < t.nseg; ++sg) {
+  for (int sg = 0; sg < S.nseg; ++sg) {
     const GSeg g = segs[t.seg0 + sg];
-    const double *A = Abase + g.a_off;
     const double *B = Bbase + g.b_off;
-    const int64_t a_sr = AMC ? 1 : g.lda, a_sk = AMC ? g.lda : 1;
     const int64_t b_sr = BKC ? g.ldb : 1, b_sk = BKC ? 1 : g.ldb;
     const int K = g.K;
-    // rows clamped into the block (their values are masked at the LDS store)
-    int64_t aoff[SA::NPAIR][2], boff[SB::NPAIR][2];
+    // each staged A row reads its own task's segment (offset, lda); rows
+    // clamped into the stack and into the block are masked at the LDS store
+    const double *aptr[SA::NPAIR][2];
+    int64_t aks[SA::NPAIR][2], boff[SB::NPAIR][2];
     bool aok[SA::NPAIR][2], bok[SB::NPAIR][2];
 #pragma unroll
     for (int i = 0; i < SA::NPAIR; ++i) {
@@ -176,9 +209,10 @@ __global__ void __launch_bounds__(256)
       SA::coord(tid, i, r, k);
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        const int gr = m0 + r + (AMC ? e : 0);
-        aok[i][e] = gr < t.M;
-        aoff[i][e] = (int64_t)min(gr, t.M - 1) * a_sr;
+        const GSeg ga = segs[aseg[i][e] + sg];
+        aok[i][e] = m0 + r + (AMC ? e : 0) < S.M;
+        aptr[i][e] = Abase + ga.a_off + (int64_t)alrow[i][e] * (AMC ? 1 : ga.lda);
+        aks[i][e] = AMC ? ga.lda : 1;
       }
     }
 #pragma unroll
@@ -188,8 +222,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
         const int gr = n0 + r + (BKC ? 0 : e);
-        bok[i][e] = gr < t.N;
-        boff[i][e] = (int64_t)min(gr, t.N - 1) * b_sr;
+        bok[i][e] = gr < S.N;
+        boff[i][e] = (int64_t)min(gr, S.N - 1) * b_sr;
       }
     }
     double ra[SA::NPAIR][2], rb[SB::NPAIR][2];
@@ -201,7 +235,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const int kc = min(k0 + k + (AMC ? 0 : e), K - 1);
-          ra[i][e] = A[aoff[i][e] + (int64_t)kc * a_sk];
+          ra[i][e] = aptr[i][e][(int64_t)kc * aks[i][e]];
         }
       }
 #pragma unroll
@@ -292,18 +326,21 @@ __global__ void __launch_bounds__(256)
     compute((nk - 1) & 1);
   }
 
-  // FP64 accumulator register r of lane l holds C[l/16 + 4r][l%16].
-  double *C = Cbase + t.c_off;
+  // FP64 accumulator register r of lane l holds C[l/16 + 4r][l%16]; stack
+  // row m goes to its task's block.
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int n = n0 + wn * WTN + j * 16 + lr;
-      if (n >= t.N) continue;
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + wm * WTM + i * 16 + lk + 4 * r;
+      if (m >= S.M) continue;
+      const GRow rw = rows[S.row0 + m];
+      const GTask tj = tasks[rw.task];
+      double *C = Cbase + tj.c_off + rw.row;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * WTM + i * 16 + lk + 4 * r;
-        if (m < t.M) C[m + (int64_t)t.ldc * n] = acc[i][j][r];
+      for (int j = 0; j < TN; ++j) {
+        const int n = n0 + wn * WTN + j * 16 + lr;
+        if (n < S.N) C[(int64_t)tj.ldc * n] = acc[i][j][r];
       }
     }
 }
@@ -341,7 +378,8 @@ struct XStage {
 
 template <int PM, int BM, int BN, bool AMC, bool BKC>
 __global__ void __launch_bounds__(256)
-    k_ggemm_mx(const GTile *__restrict__ tiles, const GTask *__restrict__ tasks,
+    k_ggemm_mx(const GTile *__restrict__ tiles, const GStack *__restrict__ stacks,
+               const GRow *__restrict__ rows, const GTask *__restrict__ tasks,
                const GSeg *__restrict__ segs, const double *__restrict__ Abase,
                const double *__restrict__ Bbase, double *__restrict__ Cbase) {
   constexpr int BK = 32;
@@ -356,11 +394,21 @@ __global__ void __launch_bounds__(256)
   __shared__ __align__(16) E sB[NS][2][BN * PITCH];
   const GTile tl = tiles[blockIdx.x];
   if (tl.task < 0) return;
-  const GTask t = tasks[tl.task];
+  const GStack S = stacks[tl.task];
+  const GTask t = tasks[S.task0];  // B side and K of every segment
   const int m0 = tl.m0, n0 = tl.n0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int lr = lane & 15, lk = lane >> 4;
+  int aseg[SA::NPAIR], alrow[SA::NPAIR];
+#pragma unroll
+  for (int i = 0; i < SA::NPAIR; ++i) {
+    int r, k;
+    SA::coord(tid, i, r, k);
+    const GRow rw = rows[S.row0 + min(m0 + r, S.M - 1)];
+    aseg[i] = tasks[rw.task].seg0;
+    alrow[i] = rw.row;
+  }
 
   f4_t acc[TM][TN];
 #pragma unroll
@@ -368,28 +416,29 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
-  for (int sg = 0; sg < t.nseg; ++sg) {
+  for (int sg = 0; sg < S.nseg; ++sg) {
     const GSeg g = segs[t.seg0 + sg];
-    const double *A = Abase + g.a_off;
     const double *B = Bbase + g.b_off;
-    const int64_t a_sr = AMC ? 1 : g.lda, a_sk = AMC ? g.lda : 1;
     const int64_t b_sr = BKC ? g.ldb : 1, b_sk = BKC ? 1 : g.ldb;
     const int K = g.K;
-    int64_t aoff[SA::NPAIR], boff[SB::NPAIR];
+    const double *aptr[SA::NPAIR];
+    int64_t aks[SA::NPAIR], boff[SB::NPAIR];
     bool aok[SA::NPAIR], bok[SB::NPAIR];
 #pragma unroll
     for (int i = 0; i < SA::NPAIR; ++i) {
       int r, k;
       SA::coord(tid, i, r, k);
-      aok[i] = m0 + r < t.M;
-      aoff[i] = (int64_t)min(m0 + r, t.M - 1) * a_sr;
+      const GSeg ga = segs[aseg[i] + sg];
+      aok[i] = m0 + r < S.M;
+      aptr[i] = Abase + ga.a_off + (int64_t)alrow[i] * (AMC ? 1 : ga.lda);
+      aks[i] = AMC ? ga.lda : 1;
     }
 #pragma unroll
     for (int i = 0; i < SB::NPAIR; ++i) {
       int r, k;
       SB::coord(tid, i, r, k);
-      bok[i] = n0 + r < t.N;
-      boff[i] = (int64_t)min(n0 + r, t.N - 1) * b_sr;
+      bok[i] = n0 + r < S.N;
+      boff[i] = (int64_t)min(n0 + r, S.N - 1) * b_sr;
     }
     double ra[SA::NPAIR][2], rb[SB::NPAIR][2];
     auto load = [&](int k0) {
@@ -398,7 +447,7 @@ __global__ void __launch_bounds__(256)
         int r, k;
         SA::coord(tid, i, r, k);
 #pragma unroll
-        for (int e = 0; e < 2; ++e) ra[i][e] = A[aoff[i] + (int64_t)min(k0 + k + e, K - 1) * a_sk];
+        for (int e = 0; e < 2; ++e) ra[i][e] = aptr[i][(int64_t)min(k0 + k + e, K - 1) * aks[i]];
       }
 #pragma unroll
       for (int i = 0; i < SB::NPAIR; ++i) {
@@ -493,18 +542,21 @@ __global__ void __launch_bounds__(256)
     compute((nk - 1) & 1);
   }
 
-  // FP32 accumulator register r of lane l holds C[4*(l/16) + r][l%16].
-  double *C = Cbase + t.c_off;
+  // FP32 accumulator register r of lane l holds C[4*(l/16) + r][l%16];
+  // stack row m goes to its task's block.
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int n = n0 + wn * WTN + j * 16 + lr;
-      if (n >= t.N) continue;
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + wm * WTM + i * 16 + 4 * lk + r;
+      if (m >= S.M) continue;
+      const GRow rw = rows[S.row0 + m];
+      const GTask tj = tasks[rw.task];
+      double *C = Cbase + tj.c_off + rw.row;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * WTM + i * 16 + 4 * lk + r;
-        if (m < t.M) C[m + (int64_t)t.ldc * n] = (double)acc[i][j][r];
+      for (int j = 0; j < TN; ++j) {
+        const int n = n0 + wn * WTN + j * 16 + lr;
+        if (n < S.N) C[(int64_t)tj.ldc * n] = (double)acc[i][j][r];
       }
     }
 }
@@ -537,11 +589,11 @@ static void launch_gg(const BsStage &st, int v, const double *A, const double *B
   const dim3 grid(st.ntile_v[g.list]);
   const GTile *tl = st.tiles_v[g.list];
 #define MH_GG(BM, BN, BK)                                                                    \
-  hipLaunchKernelGGL((k_ggemm<BM, BN, BK, AMC, BKC>), grid, dim3(256), 0, s, tl, st.tasks, \
-                     st.segs, A, B, C)
+  hipLaunchKernelGGL((k_ggemm<BM, BN, BK, AMC, BKC>), grid, dim3(256), 0, s, tl, st.stacks, \
+                     st.rows, st.tasks, st.segs, A, B, C)
 #define MH_GX(BM, BN)                                                                         \
-  hipLaunchKernelGGL((k_ggemm_mx<PM, BM, BN, AMC, BKC>), grid, dim3(256), 0, s, tl, st.tasks, \
-                     st.segs, A, B, C)
+  hipLaunchKernelGGL((k_ggemm_mx<PM, BM, BN, AMC, BKC>), grid, dim3(256), 0, s, tl, \
+                     st.stacks, st.rows, st.tasks, st.segs, A, B, C)
   if constexpr (PM == 0) {
     if constexpr (AMC || BKC) {
       if (v == 1) MH_GG(64, 64, 32);
@@ -759,6 +811,8 @@ int bs_prepare(mh_handle h, mh_bs_plan_s &p) {
   std::vector<size_t> offs;
   for (const auto &st : p.stages) {
     for (int sh = 0; sh < NTS; ++sh) lay_table(st.h_tiles[sh], total, offs);
+    lay_table(st.h_stacks, total, offs);
+    lay_table(st.h_rows, total, offs);
     lay_table(st.h_tasks, total, offs);
     lay_table(st.h_segs, total, offs);
     lay_table(st.h_chunks, total, offs);
@@ -777,6 +831,8 @@ int bs_prepare(mh_handle h, mh_bs_plan_s &p) {
   };
   for (const auto &st : p.stages) {
     for (int sh = 0; sh < NTS; ++sh) put(st.h_tiles[sh]);
+    put(st.h_stacks);
+    put(st.h_rows);
     put(st.h_tasks);
     put(st.h_segs);
     put(st.h_chunks);
@@ -804,6 +860,8 @@ int bs_prepare(mh_handle h, mh_bs_plan_s &p) {
   };
   for (auto &st : p.stages) {
     for (int sh = 0; sh < NTS; ++sh) st.tiles_v[sh] = (GTile *)at(st.h_tiles[sh]);
+    st.stacks = (GStack *)at(st.h_stacks);
+    st.rows = (GRow *)at(st.h_rows);
     st.tasks = (GTask *)at(st.h_tasks);
     st.segs = (GSeg *)at(st.h_segs);
     st.chunks = (MChunk *)at(st.h_chunks);
@@ -890,13 +948,44 @@ int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int 
   st.bkc = bkc ? 1 : 0;
   st.flops = flops;
   st.choice = -1;
+  // Stacks: tasks with the same N and the same B side of every K-segment
+  // (b_off, K, ldb) in the same order, in order of first appearance.
+  std::vector<GStack> stacks;
+  std::vector<GRow> rowmap;
+  {
+    std::map<std::vector<int64_t>, int> key2stack;
+    std::vector<std::vector<int>> members;
+    for (int i = 0; i < ntask; ++i) {
+      std::vector<int64_t> key{tasks[i].N, tasks[i].nseg};
+      for (int s = tasks[i].seg0; s < tasks[i].seg0 + tasks[i].nseg; ++s) {
+        key.push_back(segs[s].b_off);
+        key.push_back(segs[s].K);
+        key.push_back(segs[s].ldb);
+      }
+      auto it = key2stack.find(key);
+      if (it == key2stack.end()) {
+        key2stack.emplace(std::move(key), (int)members.size());
+        members.push_back({i});
+      } else {
+        members[it->second].push_back(i);
+      }
+    }
+    for (const auto &mem : members) {
+      GStack sk{0, tasks[mem[0]].N, tasks[mem[0]].nseg, mem[0], (int)rowmap.size(), 0};
+      for (int i : mem) {
+        for (int r = 0; r < tasks[i].M; ++r) rowmap.push_back(GRow{i, r});
+        sk.M += tasks[i].M;
+      }
+      stacks.push_back(sk);
+    }
+  }
   std::vector<std::vector<GTile>> lists(NTS);
   for (int sh = 0; sh < NTS; ++sh) {
     const int bm = kTileBM[sh], bn = kTileBN[sh];
     logical.clear();
-    for (int i = 0; i < ntask; ++i)
-      for (int m0 = 0; m0 < tasks[i].M; m0 += bm)
-        for (int n0 = 0; n0 < tasks[i].N; n0 += bn) logical.push_back(GTile{i, m0, n0, 0});
+    for (int q = 0; q < (int)stacks.size(); ++q)
+      for (int m0 = 0; m0 < stacks[q].M; m0 += bm)
+        for (int n0 = 0; n0 < stacks[q].N; n0 += bn) logical.push_back(GTile{q, m0, n0, 0});
     // Blocks are dispatched round-robin over the 8 XCDs: block b runs on XCD
     // b % 8.  XCD x gets the contiguous logical range [x*chunk, (x+1)*chunk).
     const int nl = (int)logical.size();
@@ -924,6 +1013,8 @@ int mh_bs_plan_add_gemm(mh_handle h, mh_bs_plan plan, int a_src, int b_src, int 
   }
   st.h_tasks = std::move(tasks);
   st.h_segs = std::move(segs);
+  st.h_stacks = std::move(stacks);
+  st.h_rows = std::move(rowmap);
   for (int sh = 0; sh < NTS; ++sh)
     if (st.ntile_v[sh] > 0) st.h_tiles[sh] = std::move(lists[sh]);
   plan->stages.push_back(std::move(st));
