@@ -80,6 +80,13 @@ class Block:
         self.off, self.rows, self.cols, self.colstart = off, rows, cols, colstart
 
 
+# Block structures by (left sectors, physical charges, nphys, right
+# sectors): a sweep rebuilds the same layouts several times per bond, and a
+# layout is never modified after construction, so they are shared.
+_LAYOUTS = {}
+_LAYOUTS_MAX = 4096
+
+
 class BlockLayout:
     """Block structure of an MPS-type tensor with ``nphys`` physical legs,
     each carrying the charges ``phys[s]``."""
@@ -89,31 +96,43 @@ class BlockLayout:
         self.phys = [tuple(q) for q in phys]
         self.nphys = nphys
         self.d = len(phys)
-        self.blocks = []
-        self.index = {}      # (qa, p) -> Block
-        self.group = {}      # qa -> (offset, ncols)
-        self.by_right = {}   # qb -> [Block] (in storage order)
-        self.by_left = {}    # qa -> [Block] (in storage order)
+        key = (tuple(lspace.sectors), tuple(self.phys), nphys, tuple(rspace.sectors))
+        tables = _LAYOUTS.get(key)
+        if tables is None:
+            tables = self._build(lspace, rspace)
+            if len(_LAYOUTS) >= _LAYOUTS_MAX:
+                _LAYOUTS.clear()
+            _LAYOUTS[key] = tables
+        # blocks; (qa, p) -> Block; qa -> (offset, ncols); qb -> [Block] and
+        # qa -> [Block] in storage order; total size
+        self.blocks, self.index, self.group, self.by_right, self.by_left, self.size = tables
+
+    def _build(self, lspace, rspace):
+        blocks, index, group, by_right, by_left = [], {}, {}, {}, {}
+        combos = []
+        for p in itertools.product(range(self.d), repeat=self.nphys):
+            qs = None   # total charge of the physical indices p
+            for s in p:
+                qs = self.phys[s] if qs is None else qadd(qs, self.phys[s])
+            combos.append((p, qs))
         off = 0
         for qa, Da in lspace.sectors:
             g0, ncols = off, 0
-            for p in itertools.product(range(self.d), repeat=nphys):
-                qb = qa
-                for s in p:
-                    qb = qadd(qb, self.phys[s])
+            for p, qs in combos:
+                qb = qa if qs is None else qadd(qa, qs)
                 if qb not in rspace:
                     continue
                 Db = rspace.dim[qb]
                 b = Block(qa, p, qb, off, Da, Db, ncols)
-                self.blocks.append(b)
-                self.index[(qa, p)] = b
-                self.by_right.setdefault(qb, []).append(b)
-                self.by_left.setdefault(qa, []).append(b)
+                blocks.append(b)
+                index[(qa, p)] = b
+                by_right.setdefault(qb, []).append(b)
+                by_left.setdefault(qa, []).append(b)
                 off += Da * Db
                 ncols += Db
             if ncols:
-                self.group[qa] = (g0, ncols)
-        self.size = off
+                group[qa] = (g0, ncols)
+        return blocks, index, group, by_right, by_left, off
 
     def dense_shape(self):
         return (self.lspace.total,) + (self.d,) * self.nphys + (self.rspace.total,)
