@@ -522,6 +522,9 @@ def _eig_workers(dev, n):
 
 
 EIG_THREADS = 4
+# A new size class of the batched eigensolves starts below this fraction of
+# the current class's largest order.
+EIG_CLASS_RATIO = 0.5
 
 # Sector eigensolver of the density-matrix split: "library" (one batched
 # library eigh per size class on side streams, the faster of the two at the
@@ -580,7 +583,7 @@ def _batched_eigh(grams, dev):
     cur = None
     for c in sorted(grams, key=lambda c: -grams[c].shape[0]):
         n = grams[c].shape[0]
-        if cur is None or n < 0.5 * cur:
+        if cur is None or n < EIG_CLASS_RATIO * cur:
             cur = -(-n // 32) * 32
         classes.setdefault(cur, []).append(c)
 
