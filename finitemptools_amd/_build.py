@@ -16,6 +16,7 @@ SOURCES = [
     "mh_adjoint.hip",
     "mh_eigen.hip",
     "mh_bs_planner.cpp",
+    "mh_schur.cpp",
 ]
 # rocSOLVER's divide-and-conquer tridiagonal eigensolver (mh_stedc_batched)
 LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]

@@ -69,6 +69,10 @@ SIGNATURES = {
                                 c_void_p]),
     "mh_ac_prime_open": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
+    "mh_schur_fold": (c_int, [c_int, c_int, c_int, c_int, c_int, P(c_int), P(c_int), P(c_int),
+                              P(c_double), c_void_p, c_void_p]),
+    "mh_env_gather": (c_int, [c_void_p, c_int, c_int, c_int, c_int, P(c_void_p), c_void_p]),
+    "mh_env_scatter": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, P(c_void_p)]),
     "mh_heff_size": (c_int, [P(HeffOp), P(c_size_t)]),
     "mh_heff_apply": (c_int, [c_void_p, P(HeffOp), c_void_p, c_void_p]),
     "mh_lanczos_eig": (c_int, [c_void_p, P(HeffOp), c_void_p, c_int, c_double, c_int,

@@ -210,3 +210,37 @@ def local_expectation(mps, op, site):
     num = expectation(mpo_to_device(site_operator_mpo(L, site, op, dt), mps[0].dtype, mps[0].device), mps)
     den = expectation(mpo_to_device(site_operator_mpo(L, site, np.eye(d), dt), mps[0].dtype, mps[0].device), mps)
     return num / den
+
+
+def env_gather(parts):
+    """The reference's Vector environment (one (D1, D2) device matrix per MPO
+    channel) as the engine's E[a, w, a'] (mh_env_gather)."""
+    import ctypes
+    from . import _lib
+    from .engine import as_colmajor, colmajor_empty, dtype_code, engine, ptr
+    cm = [as_colmajor(p) for p in parts]   # column-major storage
+    D1, D2 = cm[0].shape
+    E = colmajor_empty((D1, len(cm), D2), cm[0].dtype, cm[0].device)
+    e = engine(cm[0].device)
+    h = e.bind_stream()
+    arr = (ctypes.c_void_p * len(cm))(*[ptr(p).value for p in cm])
+    _lib.raise_for(_lib.load().mh_env_gather(h, dtype_code(cm[0]), D1, D2, len(cm), arr,
+                                             ptr(E)), h, "mh_env_gather")
+    return E
+
+
+def env_scatter(E):
+    """E[a, w, a'] as the Vector of per-channel (D1, D2) matrices
+    (mh_env_scatter)."""
+    import ctypes
+    from . import _lib
+    from .engine import as_colmajor, colmajor_empty, dtype_code, engine, ptr
+    E = as_colmajor(E)
+    D1, nw, D2 = E.shape
+    parts = [colmajor_empty((D1, D2), E.dtype, E.device) for _ in range(nw)]
+    e = engine(E.device)
+    h = e.bind_stream()
+    arr = (ctypes.c_void_p * nw)(*[ptr(p).value for p in parts])
+    _lib.raise_for(_lib.load().mh_env_scatter(h, dtype_code(E), D1, D2, nw, ptr(E), arr), h,
+                   "mh_env_scatter")
+    return parts

@@ -153,3 +153,33 @@ def site_operator_mpo(L, i, op, dtype=np.float64):
         m = op if k == i else np.eye(d)
         mpo.append(np.asarray(m, dtype=dtype).reshape(1, d, 1, d))
     return mpo
+
+
+def schur_fold(terms, wl, wr, d, dtype=np.float64):
+    """Dense site tensor W[wl, s_out, wr, s_in] from the entries of a
+    Schur-form MPO site tensor, through mh_schur_fold (host only): terms are
+    (i, j, coef, O) with O a (d x d) matrix O[s_out, s_in], or None for a
+    scalar entry (coef times the identity) -- the opkeys and scalkeys of the
+    reference's AbstractSiteMPOTensor (algorithms/derivatives.jl:7-17)."""
+    import ctypes
+    from . import _lib
+    cplx = np.dtype(dtype) == np.complex128
+    n = len(terms)
+    ti = (ctypes.c_int * max(n, 1))(*[int(t[0]) for t in terms])
+    tj = (ctypes.c_int * max(n, 1))(*[int(t[1]) for t in terms])
+    kind = (ctypes.c_int * max(n, 1))(*[0 if t[3] is None else 1 for t in terms])
+    coef = np.zeros(2 * max(n, 1))
+    for k, t in enumerate(terms):
+        coef[2 * k], coef[2 * k + 1] = complex(t[2]).real, complex(t[2]).imag
+    mats = [np.asarray(t[3], dtype=np.complex128 if cplx else np.float64) for t in terms
+            if t[3] is not None]
+    # each O column-major (O[s_out, s_in] at s_out + d s_in): C order of O^T
+    ops = np.ascontiguousarray(np.stack([m.T for m in mats])) if mats else np.zeros(1)
+    ops = ops.view(np.float64) if cplx else ops
+    W = np.zeros(wl * d * wr * d, dtype=np.complex128 if cplx else np.float64)
+    st = _lib.load().mh_schur_fold(_lib.MH_C64 if cplx else _lib.MH_F64, wl, wr, d, n, ti, tj,
+                                   kind, coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   ops.ctypes.data_as(ctypes.c_void_p),
+                                   W.ctypes.data_as(ctypes.c_void_p))
+    _lib.raise_for(st, None, "mh_schur_fold")
+    return W.reshape((wl, d, wr, d), order="F")

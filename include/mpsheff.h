@@ -121,6 +121,27 @@ int mh_updateright(mh_handle h, int dtype, int prec, int Dl, int d, int Dr,
                    int wl, int wr, const void *hold, const void *psiA,
                    const void *W, const void *psiB, void *hnew);
 
+/* ---- Schur-form MPO tensors and Vector environments ---------------------- */
+
+/* W[wl, s_out, wr, s_in] (host arrays, column-major; dtype MH_F64 or MH_C64)
+ * from the entries of a Schur-form site tensor (the reference's
+ * MPOHamiltonian: ac_prime(x, ham::AbstractSiteMPOTensor, hleft::Vector,
+ * hright::Vector), algorithms/derivatives.jl:7-17,24-49): term t adds
+ * coef[t] * O_t at channel pair (ti[t], tj[t]), where O_t is the next d x d
+ * matrix of ops (O[s_out, s_in], column-major) when kind[t] = 1, or the
+ * identity when kind[t] = 0 (the scalar entries, scalkeys).  coef holds
+ * (re, im) pairs; a real W needs im = 0.  Host only: no handle, no device. */
+int mh_schur_fold(int dtype, int wl, int wr, int d, int nterms, const int *ti,
+                  const int *tj, const int *kind, const double *coef, const void *ops,
+                  void *W);
+/* The reference's Vector environments (one D1 x D2 device matrix per MPO
+ * channel, hleft[w][a, a']) <-> the engine's E[a, w, a'] (extents D1, nw,
+ * D2), as strided device copies on the handle's stream. */
+int mh_env_gather(mh_handle h, int dtype, int D1, int D2, int nw, const void *const *parts,
+                  void *E);
+int mh_env_scatter(mh_handle h, int dtype, int D1, int D2, int nw, const void *E,
+                   void *const *parts);
+
 /* ---- operator descriptor for the device-resident Krylov drivers ---------- */
 enum mh_heff_kind { MH_HEFF0 = 0, MH_HEFF1 = 1, MH_HEFF2 = 2 };
 
