@@ -66,3 +66,38 @@ def test_vector_form_ac_prime_matches_folded_dense(gpu, cplx):
     back = env_scatter(L)
     for k in range(wl):
         assert torch.equal(back[k].cpu(), torch.as_tensor(Lv[k]))
+
+
+@pytest.mark.parametrize("cplx", [False, True], ids=["real", "complex"])
+def test_vector_form_env_update_matches_folded_dense(gpu, cplx):
+    """mpohamiltonian/transfer.jl:48-75: hnew[j] += updateleft(hold[i], A,
+    W_ij, B) over the site tensor's entries (identity entries contract only
+    the MPS legs) equals the engine's updateleft on the folded site tensor and
+    gathered environment, scattered back to one matrix per channel."""
+    from finitemptools_amd.envs import env_gather, env_scatter
+    from finitemptools_amd.transfer import updateleft
+    rng = np.random.default_rng(29 + cplx)
+    cell = (models.tfim_mpo(4, 1.0, 0.6, dtype=np.complex128) if cplx
+            else models.heisenberg_xxz_mpo(4, 1.0, 0.7, 0.3))[1]
+    wl, d, wr, _ = cell.shape
+    terms = _terms(cell)
+    Dl, Dr = 6, 8
+    dt = np.complex128 if cplx else np.float64
+
+    def rnd(*shape):
+        a = rng.standard_normal(shape)
+        return (a + 1j * rng.standard_normal(shape)) if cplx else a
+
+    hold = [rnd(Dl, Dl) for _ in range(wl)]
+    A, B = rnd(Dl, d, Dr), rnd(Dl, d, Dr)
+    ref = [np.zeros((Dr, Dr), dtype=dt) for _ in range(wr)]
+    for i, j, c, O in terms:
+        Wt = (c * (np.eye(d) if O is None else O)).reshape(1, d, 1, d).astype(dt)
+        ref[j] += OC.updateleft(hold[i].reshape(Dl, 1, Dl), A, Wt, B).reshape(Dr, Dr)
+    W = models.schur_fold(terms, wl, wr, d, dtype=dt)
+    E = updateleft(env_gather([to_device(h) for h in hold]), to_device(A), to_device(W),
+                   to_device(B))
+    got = [m.cpu().numpy() for m in env_scatter(E)]
+    scale = max(np.linalg.norm(r) for r in ref)
+    for g, r in zip(got, ref):
+        assert np.linalg.norm(g - r) <= 1e-12 * scale
