@@ -405,3 +405,29 @@ def test_every_gemm_configuration(gpu, cplx):
                 assert relerr(C2, A @ Bt.T) < 1e-13, (c, M, N, K)
     finally:
         eng.check(eng.lib.mh_gemm_force_candidate(eng.h, -1))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_engine_two_site_reduces_to_one_site_with_absorbed_env(gpu, cplx):
+    """On the device: ac2_prime(A1.A2) contracted with conj(A2) equals
+    ac_prime(A1) with site 2 absorbed into the right environment by
+    updateright (ragged bond dimensions, real and complex)."""
+    import torch
+    from finitemptools_amd import models
+    from finitemptools_amd.derivatives import ac2_prime, ac_prime
+    from finitemptools_amd.engine import to_device
+    from finitemptools_amd.transfer import updateright
+    rng = np.random.default_rng(31 + cplx)
+    W = (models.tfim_mpo(4, 1.0, 0.8, dtype=np.complex128) if cplx
+         else models.heisenberg_xxz_mpo(4))[1]
+    w, d = W.shape[0], W.shape[1]
+    Dl, Dm, Dr = 40, 56, 48
+    A1, A2 = rand(rng, (Dl, d, Dm), cplx), rand(rng, (Dm, d, Dr), cplx)
+    L, R = rand(rng, (Dl, w, Dl), cplx), rand(rng, (Dr, w, Dr), cplx)
+    x = np.einsum("asm,mtb->astb", A1, A2)
+    Wd, A2d = to_device(W), to_device(A2)
+    y2 = ac2_prime(to_device(x), Wd, Wd, to_device(L), to_device(R)).cpu().numpy()
+    lhs = np.einsum("astb,mtb->asm", y2, A2.conj())
+    Rp = updateright(to_device(R), A2d, Wd, A2d)
+    rhs = ac_prime(to_device(A1), Wd, to_device(L), Rp).cpu().numpy()
+    assert np.linalg.norm(lhs - rhs) <= 1e-12 * np.linalg.norm(rhs)

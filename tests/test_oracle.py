@@ -274,3 +274,27 @@ def test_ch_expansion_terms_pinned_by_ac_prime(cplx):
     ref = alpha * C.ac_prime(A, W, Lfull, R)
     closed = np.einsum("af,fsb->asb", Lfull.reshape(wl * Dl, wl * Dl, order="F"), Q)
     assert np.allclose(closed, ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_two_site_heff_reduces_to_one_site_with_absorbed_env(cplx):
+    """ac2_prime(A1.A2, W1, W2, L, R) contracted with conj(A2) over (s2', b)
+    is ac_prime(A1, W1, L, R') with R' = updateright(R, A2, W2, A2): the
+    two-site, one-site and environment-update restatements agree (any A2)."""
+    rng = np.random.default_rng(23 + cplx)
+    W = (models.tfim_mpo(4, 1.0, 0.8, dtype=np.complex128) if cplx
+         else models.heisenberg_xxz_mpo(4))[1]
+    w, d = W.shape[0], W.shape[1]
+    Dl, Dm, Dr = 3, 5, 4
+
+    def rnd(*shape):
+        a = rng.standard_normal(shape)
+        return (a + 1j * rng.standard_normal(shape)) if cplx else a
+
+    A1, A2 = rnd(Dl, d, Dm), rnd(Dm, d, Dr)
+    L, R = rnd(Dl, w, Dl), rnd(Dr, w, Dr)
+    x = np.einsum("asm,mtb->astb", A1, A2)
+    y2 = C.ac2_prime(x, W, W, L, R)
+    lhs = np.einsum("astb,mtb->asm", y2, A2.conj())
+    rhs = C.ac_prime(A1, W, L, C.updateright(R, A2, W, A2))
+    assert np.allclose(lhs, rhs, rtol=1e-12, atol=1e-12 * np.abs(rhs).max())
