@@ -125,3 +125,16 @@ def test_schur_fold_rebuilds_the_model_site_tensors():
     ref[0, :, 1, :] = 0.5 * O + np.eye(2)
     ref[1, :, 1, :] = -2.0 * np.eye(2)
     assert np.array_equal(W, ref)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says
+    why, before touching a device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
