@@ -134,6 +134,9 @@ SIGNATURES = {
     "mh_ormtr_batched": (c_int, [c_void_p, c_int, P(c_int), P(c_int64), P(c_int64), c_void_p,
                                  c_void_p, P(c_int64), P(c_int), P(c_int), c_void_p]),
     "mh_eig_check": (c_int, [c_void_p, c_int]),
+    "mh_heev_small": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                              c_void_p]),
+    "mh_heev_small_max": (c_int, [c_void_p, c_int, P(c_int)]),
     "mh_profile_enable": (c_int, [c_void_p, c_int]),
     "mh_profile_read": (c_int, [c_void_p, c_int, P(c_double), P(c_int64), P(c_double)]),
     "mh_profile_reset": (c_int, [c_void_p]),

@@ -13,6 +13,9 @@ for all sectors of one bond as
    calls spread over side streams;
 3. `mh_ormtr_batched`: back-transformation of only the eigenvector columns
    the truncation keeps.
+
+`small_eigh` is the one-launch solver for the small matrices of the dense
+split (`mh_heev_small`: one workgroup per matrix, the matrix in LDS).
 """
 import ctypes
 
@@ -120,3 +123,47 @@ def sector_eigh(mats, device=None):
     V = se.vectors(se.sizes)
     se.check()
     return [(l.clone(), v.clone()) for l, v in zip(lams, V)]
+
+
+_SMALL_MAX = {}
+
+
+def small_eigh_max(dtype, device):
+    """Largest order `small_eigh` takes for this dtype on this device."""
+    device = torch.device(device)
+    if device.type != "cuda":
+        return 0
+    e = engine(device)
+    key = (e.device, dtype)
+    if key not in _SMALL_MAX:
+        h = e.bind_stream()
+        out = ctypes.c_int(0)
+        code = _lib.MH_C64 if dtype == torch.complex128 else _lib.MH_F64
+        _lib.raise_for(_lib.load().mh_heev_small_max(h, code, ctypes.byref(out)), h,
+                       "mh_heev_small_max")
+        _SMALL_MAX[key] = out.value
+    return _SMALL_MAX[key]
+
+
+def small_eigh(Gt, col0=0):
+    """Eigenpairs of a Hermitian (complex128) or symmetric (float64) n x n
+    device matrix G given as Gt = G^T (row-major), i.e. G in column-major
+    storage.  Returns (lam ascending, V, info): V[:, j] is the eigenvector of
+    lam[j] for j >= col0 (the columns below col0 are unfinished); info is a
+    device int32 flag, 1 when the QL iteration did not converge (lam is then
+    NaN).  One launch, stream-ordered on the engine's stream."""
+    n = Gt.shape[0]
+    if Gt.dim() != 2 or Gt.shape[1] != n:
+        raise _lib.DimensionMismatch(2, f"small_eigh: square matrix expected, got {tuple(Gt.shape)}")
+    if Gt.dtype not in (torch.float64, torch.complex128):
+        raise _lib.MHError(3, f"small_eigh: float64 or complex128, got {Gt.dtype}")
+    Gt = Gt.contiguous()
+    e = engine(Gt.device)
+    h = e.bind_stream()
+    lam = torch.empty(n, dtype=torch.float64, device=Gt.device)
+    Zt = torch.empty((n, n), dtype=Gt.dtype, device=Gt.device)
+    info = torch.empty(1, dtype=torch.int32, device=Gt.device)
+    code = _lib.MH_C64 if Gt.dtype == torch.complex128 else _lib.MH_F64
+    _lib.raise_for(_lib.load().mh_heev_small(h, code, n, 1, ptr(Gt), ptr(lam), ptr(Zt),
+                                             int(col0), ptr(info)), h, "mh_heev_small")
+    return lam, Zt.t(), info

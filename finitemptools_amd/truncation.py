@@ -8,10 +8,19 @@ two-site tensor split as (a, s1) | (s2, b).
 import numpy as np
 import torch
 
+from . import eigen
+from ._lib import MHError
 from .linalg import as_matrix, from_matrix
 
 DEFAULT_SVD_CUTOFF = 1.0e-8   # truncation.jl:49
 DEFAULT_NONSYMMETRIC_D = 100  # truncation.jl:51
+# Dense splits whose density matrix has order <= SMALL_EIGH_ORDER use the
+# one-launch device solver (mh_heev_small); larger ones the library eigh.
+# Measured on MI355X (eigenvalues back on the host included): n = 16 complex
+# 0.29 ms against 0.51 ms, even at n = 32, and slower above (n = 64: 3.4
+# against 1.8 ms), where its one-thread QL recurrence dominates.
+SMALL_EIGH = True
+SMALL_EIGH_ORDER = 24
 
 
 class MPSTruncation:
@@ -49,15 +58,26 @@ def split_rho(x, trunc, absorb, normalize=True):
     Dl, d1, d2, Dr = x.shape
     M = as_matrix(x, 2)
     left = absorb == "right"
-    G = M @ M.conj().t() if left else M.conj().t() @ M
-    lam, V = torch.linalg.eigh(G)
     r = min(M.shape)
-    lam_h = np.maximum(lam.detach().cpu().numpy()[::-1][:r], 0.0)
+    n = M.shape[0] if left else M.shape[1]
+    if (SMALL_EIGH and x.is_cuda and n <= SMALL_EIGH_ORDER
+            and n <= eigen.small_eigh_max(M.dtype, x.device)):
+        # one-launch solver; it takes G column-major: G^T = conj(M) M^T (left)
+        # or M^T conj(M) (right); only the top r columns are finished
+        Gt = M.conj() @ M.t() if left else M.t() @ M.conj()
+        lam, V, _ = eigen.small_eigh(Gt, col0=n - r)
+    else:
+        G = M @ M.conj().t() if left else M.conj().t() @ M
+        lam, V = torch.linalg.eigh(G)
+    lam_h = lam.detach().cpu().numpy()[::-1][:r]
+    if not np.isfinite(lam_h).all():
+        raise MHError(6, "split_rho: the eigensolver did not converge")
+    lam_h = np.maximum(lam_h, 0.0)
     s_h = np.sqrt(lam_h)
     k, _ = trunc.truncate(s_h)
     k = max(1, min(k, int(np.count_nonzero(s_h > 0)) or 1))
     norm = float(np.linalg.norm(s_h[:k])) if normalize else 1.0
-    Q = V[:, G.shape[0] - k:].flip(-1)
+    Q = V[:, n - k:].flip(-1)
     s = torch.as_tensor(s_h[:k] / norm, device=x.device)  # normalize=False: the raw values
     if left:
         A1 = from_matrix(Q, (Dl, d1, k))

@@ -418,6 +418,21 @@ int mh_ormtr_batched(mh_handle h, int nmat, const int *n, const int64_t *a_off,
  * group barrier of the last tridiagonalisation timed out. */
 int mh_eig_check(mh_handle h, int nmat);
 
+/* Eigen-decomposition of `batch` Hermitian (MH_C64) or symmetric (MH_F64)
+ * n x n matrices, one workgroup per matrix with the matrix in LDS: the small
+ * reduced density matrices of the dense two-site split (the FP64 step behind
+ * auxiliary/stable_svd.jl:4-18 and truncation.jl:27-47 at small bonds), where
+ * a library syevd is launch-latency bound.  A: batch matrices, column-major,
+ * contiguous (read only).  W: batch x n eigenvalues, ascending.  Z: batch x
+ * n x n, column-major; columns col0 .. n-1 hold the eigenvectors of W[col0..]
+ * (the columns below col0 are left unfinished).  info (device, nullable):
+ * per matrix 0, or 1 when the QL iteration did not converge (W is then NaN).
+ * n above mh_heev_small_max -> MH_ERR_SHAPE. */
+int mh_heev_small(mh_handle h, int dtype, int n, int batch, const void *A, double *W, void *Z,
+                  int col0, int *info);
+/* largest order mh_heev_small takes on the handle's device for dtype */
+int mh_heev_small_max(mh_handle h, int dtype, int *nmax);
+
 /* ---- measurement --------------------------------------------------------- */
 enum mh_kernel_class { MH_K_GEMM = 0, MH_K_WSTAGE = 1, MH_K_VEC = 2, MH_K_NCLASS = 3 };
 /* enable = 1: bracket every launch with HIP events on the handle's stream */
