@@ -18,7 +18,7 @@ DEFAULT_NONSYMMETRIC_D = 100  # truncation.jl:51
 # one-launch device solver (mh_heev_small); larger ones the library eigh.
 # Measured on MI355X (eigenvalues back on the host included): n = 16 complex
 # 0.29 ms against 0.51 ms, even at n = 32, and slower above (n = 64: 3.4
-# against 1.8 ms), where its one-thread QL recurrence dominates.
+# against 1.8 ms).
 SMALL_EIGH = True
 SMALL_EIGH_ORDER = 24
 
