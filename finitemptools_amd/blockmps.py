@@ -521,6 +521,15 @@ def _eig_workers(dev, n):
     return _EIG_POOL[key]
 
 
+def shutdown_workers():
+    """Join the eigensolve worker threads and drop their side streams."""
+    for pool, streams in list(_EIG_POOL.values()):
+        pool.shutdown(wait=True)
+        for s in streams:
+            s.synchronize()
+    _EIG_POOL.clear()
+
+
 EIG_THREADS = 4
 # A new size class of the batched eigensolves starts below this fraction of
 # the current class's largest order.

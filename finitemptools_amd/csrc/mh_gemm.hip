@@ -937,9 +937,7 @@ int gemm_dispatch(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx,
   const bool amc = (p.sam == 1) || (p.M == 1);
   const bool bkc = (p.sbk == 1) || (p.K == 1);
   const double flops = 2.0 * p.M * (double)p.N * p.K * (cplx ? 4.0 : 1.0);
-  char key[192];
-  snprintf(key, sizeof(key), "%s:%d:%d:%d:%d:%d%d:%d:%d", tag, cplx ? 1 : 0, p.M, p.N, p.K,
-           (int)amc, (int)bkc, (p.beta_re != 0.0 || p.beta_im != 0.0) ? 1 : 0, p.c_f32);
+  const bool has_beta = p.beta_re != 0.0 || p.beta_im != 0.0;
   int choice = -1;
   // a forced configuration (tests), if this family has it and admits it
   if (h->gemm_force >= 0 && h->gemm_force < fam.n && fam.ok(fam.cands[h->gemm_force], p, cplx)) {
@@ -949,19 +947,36 @@ int gemm_dispatch(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx,
     prof_end(h, slot);
     return st;
   }
+  auto make_key = [&](int beta_flag) {
+    char key[192];
+    snprintf(key, sizeof(key), "%s:%d:%d:%d:%d:%d%d:%d:%d", tag, cplx ? 1 : 0, p.M, p.N, p.K,
+             (int)amc, (int)bkc, beta_flag, p.c_f32);
+    return std::string(key);
+  };
+  const std::string key = make_key(has_beta ? 1 : 0);
   auto it = h->tune.find(key);
   if (it != h->tune.end()) {
     choice = it->second;
   } else {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(h->stream, &cs);
-    // Tune shapes worth more than ~50 us of work; small ones use the heuristic.
-    if (h->autotune && cs == hipStreamCaptureStatusNone && flops > 2e9 && !h->profiling) {
+    // Tune shapes worth more than ~50 us of work; small ones use the
+    // heuristic.  Tuning runs every candidate on the caller's C, which is
+    // only valid when C is pure output (beta = 0): with beta != 0 each run
+    // would fold the previous run's result back in.  Such shapes take the
+    // configuration tuned for the same shape with beta = 0 when there is
+    // one that admits beta, else the heuristic.
+    if (!has_beta && h->autotune && cs == hipStreamCaptureStatusNone && flops > 2e9 &&
+        !h->profiling) {
       MH_TRY(tune(h, fam, p, cplx, amc, bkc, &choice));
       h->tune[key] = choice;
       return MH_OK;
     }
-    choice = heuristic(fam, p, cplx);
+    if (has_beta) {
+      auto it0 = h->tune.find(make_key(0));
+      if (it0 != h->tune.end() && fam.ok(fam.cands[it0->second], p, cplx)) choice = it0->second;
+    }
+    if (choice < 0) choice = heuristic(fam, p, cplx);
     h->tune[key] = choice;
   }
   int slot;

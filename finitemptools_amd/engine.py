@@ -5,6 +5,7 @@ is column-major in the index order of the reference (Julia/TensorKit): the
 first index is fastest.  ``colmajor_empty((Dl, d, Dr))`` allocates such a
 tensor; indexing ``A[a, s, b]`` then means what it means in the reference.
 """
+import atexit
 import ctypes
 
 import torch
@@ -151,6 +152,28 @@ class Engine:
 
 
 _ENGINES = {}
+_ATEXIT = False
+
+
+def close_all():
+    """Tear the engines down while the HIP runtime and every library are
+    still alive: first the block-split worker pools (their threads launch on
+    side streams), then every handle (mh_destroy waits for its streams and
+    destroys the eigensolver's side streams and library handles).  Runs from
+    ``atexit`` (registered with the first engine, so before torch's own exit
+    hooks) instead of from ``__del__`` during interpreter teardown, when the
+    order against runtime and profiler shutdown is not defined."""
+    import sys
+    bm = sys.modules.get(__package__ + ".blockmps")
+    if bm is not None:
+        bm.shutdown_workers()
+    for dev in list(_ENGINES):
+        e = _ENGINES.pop(dev)
+        try:
+            torch.cuda.synchronize(dev)
+        except Exception:
+            pass
+        e.close()
 
 
 def engine(device=None):
@@ -159,8 +182,12 @@ def engine(device=None):
         device = torch.cuda.current_device()
     if isinstance(device, torch.device):
         device = device.index if device.index is not None else torch.cuda.current_device()
+    global _ATEXIT
     e = _ENGINES.get(device)
     if e is None:
         e = Engine(device)
         _ENGINES[device] = e
+        if not _ATEXIT:
+            atexit.register(close_all)
+            _ATEXIT = True
     return e

@@ -97,5 +97,5 @@ def test_xxz_L20_D64_four_sweeps(gpu):
     four sweeps from a seeded random right-canonical MPS."""
     energies, delta, psi, alg = run_gpu_dmrg(models.heisenberg_xxz_mpo(20), 20, 2, 64, nsweeps=4,
                                              eps=1e-8)
-    assert abs(energies[-1] - ED["xxz_L20_J1_D1"]) < 1e-6
+    assert abs(energies[-1] - ED["xxz_L20_J1_D1"]) < 1e-8
     assert psi.bond_dimension() <= 64

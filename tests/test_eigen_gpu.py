@@ -168,3 +168,53 @@ def test_block_dmrg_with_sector_eigensolver_matches_ed(gpu, monkeypatch):
         out[solver] = energies[-1]
     assert abs(out["sector"] - ed["hubbard_L8_t1_U4_N4_4"]) < 1e-8
     assert abs(out["sector"] - out["library"]) < 1e-10
+
+
+CHILD = r"""
+import sys
+sys.path.insert(0, {root!r})
+import ctypes
+import numpy as np
+import scipy.linalg as sl
+import torch
+from finitemptools_amd import _lib
+from finitemptools_amd.engine import engine, ptr
+rng = np.random.default_rng(3)
+sizes = [40, 96, 17]
+vo = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+zo = np.concatenate([[0], np.cumsum(np.square(sizes))[:-1]]).astype(np.int64)
+d = rng.standard_normal(sum(sizes))
+e = rng.standard_normal(sum(sizes))
+D, E = torch.as_tensor(d, device="cuda"), torch.as_tensor(e, device="cuda")
+Z = torch.empty(int(np.square(sizes).sum()), dtype=torch.float64, device="cuda")
+eng = engine(0)
+h = eng.bind_stream()
+n32 = np.array(sizes, dtype=np.int32)
+P32, P64 = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)
+eng.check(eng.lib.mh_stedc_batched(h, 3, n32.ctypes.data_as(P32), vo.ctypes.data_as(P64),
+                                   zo.ctypes.data_as(P64), ptr(D), ptr(E), ptr(Z)))
+eng.check(eng.lib.mh_eig_check(h, 3))
+lam = D.cpu().numpy()
+for n, o in zip(sizes, vo):
+    ref = sl.eigh_tridiagonal(d[o:o + n], e[o:o + n - 1], eigvals_only=True)
+    assert np.abs(lam[o:o + n] - ref).max() < 1e-12 * np.abs(ref).max()
+print("child done", flush=True)
+"""
+
+
+def test_exit_after_sector_eigensolver_is_clean(gpu):
+    """A process that ran the batched tridiagonal eigensolver (the side
+    streams and rocBLAS handles of mh_stedc_batched) exits with status 0:
+    the engines are destroyed from atexit, before the runtime tears down.
+    The child runs no cooperative launch (the tridiagonalisation), which the
+    device may refuse to a second process while this one holds its
+    cooperative-launch resources."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("child: batched tridiagonal eigensolver, then interpreter exit", flush=True)
+    r = subprocess.run([sys.executable, "-c", CHILD.format(root=root)], capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    assert "child done" in r.stdout

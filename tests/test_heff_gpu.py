@@ -76,6 +76,39 @@ def test_gemm_strided_layouts(gpu, cplx, layout):
         assert relerr(host(Cd), ref) < TOL
 
 
+def test_gemm_beta_on_a_tuned_shape(gpu):
+    """C = alpha A B + beta C at a size the tuner times (1024^3 > 2e9 flops)
+    on a fresh handle, so the first call meets an uncached shape: the tuning
+    runs must not fold their results into C."""
+    import torch
+    from finitemptools_amd.engine import Engine
+    rng = np.random.default_rng(21)
+    n = 1024
+    A, B, C0 = (rng.standard_normal((n, n)) for _ in range(3))
+    e = Engine(0)
+    try:
+        h = e.bind_stream()
+        Ad, Bd = dev(A), dev(B)
+        for call in range(2):   # first call tunes (beta = 0 shape unknown), second is cached
+            Cd = dev(C0)
+            print(f"gemm beta call {call}", flush=True)
+            st = e.lib.mh_gemm(h, 0, 0, n, n, n, ctypes.c_void_p(Ad.data_ptr()), 1, n, 0,
+                               ctypes.c_void_p(Bd.data_ptr()), 1, n, ctypes.c_void_p(Cd.data_ptr()),
+                               1, n, 0.7, 0.0, 0.3, 0.0)
+            e.check(st)
+            torch.cuda.synchronize()
+            assert relerr(host(Cd), 0.7 * A @ B + 0.3 * C0) < TOL
+        # a beta = 0 call tunes the shape; a later beta call reuses that choice
+        Cd = dev(C0)
+        e.check(e.lib.mh_gemm(h, 0, 0, n, n, n, ctypes.c_void_p(Ad.data_ptr()), 1, n, 0,
+                              ctypes.c_void_p(Bd.data_ptr()), 1, n, ctypes.c_void_p(Cd.data_ptr()),
+                              1, n, 1.0, 0.0, 0.0, 0.0))
+        torch.cuda.synchronize()
+        assert relerr(host(Cd), A @ B) < TOL
+    finally:
+        e.close()
+
+
 SHAPES1 = [  # Dl, d, Dr, W
     (1, 2, 2, "xxz"), (2, 2, 4, "xxz"), (16, 2, 16, "xxz"), (37, 2, 29, "xxz"),
     (64, 2, 64, "xxz"), (24, 4, 20, "hubbard"), (33, 2, 47, "tfim"),
