@@ -59,6 +59,9 @@ SIGNATURES = {
     "mh_ac2_prime": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                              c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p]),
+    "mh_ac2_prime_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                     c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
     "mh_c_prime": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
                            c_void_p, c_void_p, c_void_p]),
     "mh_updateleft": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,

@@ -947,6 +947,9 @@ int gemm_dispatch(mh_handle h, const GemmFamily &fam, const GemmP &p, bool cplx,
     prof_end(h, slot);
     return st;
   }
+  // Allocation-free callers: the heuristic configuration, no table lookup
+  // (the key string would be a host allocation) and no tuning runs.
+  if (h->noalloc) return run_cand(h, fam, fam.cands[heuristic(fam, p, cplx)], p, cplx, amc, bkc);
   auto make_key = [&](int beta_flag) {
     char key[192];
     snprintf(key, sizeof(key), "%s:%d:%d:%d:%d:%d%d:%d:%d", tag, cplx ? 1 : 0, p.M, p.N, p.K,

@@ -12,9 +12,42 @@
 // over the shared MPO bond).  The MPO matrices are sparse; rows of Mt that are
 // zero for a block's column chunk are skipped (the test is uniform across a
 // wave, so it costs no divergence).
+#include <mutex>
+
 #include "mh_internal.h"
 
 namespace mh {
+
+// ------------------------------------------- allocation-free entry point
+// Scratch of mh_ac2_prime_noalloc: the intermediates and the folded MPO
+// matrix of one two-site H_eff live in this file-scope device array instead
+// of the handle's grow-only workspace.  Every device the library runs on
+// holds its own copy, at its own address, until the process ends.  Host
+// code never passes the array's name to a kernel (on the host it is only a
+// stand-in symbol); it passes the address hipGetSymbolAddress returned for
+// the device, kept in g_scratch_ptr.
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;  // 64 MiB per device
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+static void *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+int noalloc_setup(mh_handle h) {
+  if (h->device < 0 || h->device >= kMaxDevices)
+    return fail(h, MH_ERR_ARG, "device %d beyond the scratch table (%d devices)", h->device,
+                kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[h->device]) return MH_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return fail(h, MH_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", h->device,
+                hipGetErrorString(e));
+  }
+  g_scratch_ptr[h->device] = ptr;
+  return MH_OK;
+}
 
 __device__ __forceinline__ double cmul_re(double2 a, double2 b) { return a.x * b.x - a.y * b.y; }
 __device__ __forceinline__ double cmul_im(double2 a, double2 b) { return a.x * b.y + a.y * b.x; }
@@ -483,6 +516,33 @@ int mh_ac2_prime(mh_handle h, int dtype, int prec, int Dl, int d1, int d2,
                  const void *W2, const void *L, const void *R, void *y) {
   mh_heff_op op{MH_HEFF2, dtype, prec, Dl, d1, d2, Dr, wl, wm, wr, W1, W2, L, R};
   return mh_heff_apply(h, &op, x, y);
+}
+
+int mh_ac2_prime_noalloc(mh_handle h, int dtype, int prec, int Dl, int d1, int d2, int Dr,
+                         int wl, int wm, int wr, const void *x, const void *W1, const void *W2,
+                         const void *L, const void *R, void *y) {
+  if (!h) return MH_ERR_ARG;
+  mh_heff_op op{MH_HEFF2, dtype, prec, Dl, d1, d2, Dr, wl, wm, wr, W1, W2, L, R};
+  MH_TRY(heff_check(h, op));
+  if (!x || !y) return fail(h, MH_ERR_ARG, "null vector");
+  if (x == y) return fail(h, MH_ERR_ARG, "x and y must not alias");
+  if (prec != MH_PREC_F64)
+    return fail(h, MH_ERR_UNSUPPORTED, "mh_ac2_prime_noalloc computes in FP64 only (prec %d)",
+                prec);
+  const size_t need = heff_ws_bytes(op);
+  if (need > kNoallocScratchBytes)
+    return fail(h, MH_ERR_SHAPE,
+                "mh_ac2_prime_noalloc needs %zu bytes of scratch; its static scratch holds %zu",
+                need, kNoallocScratchBytes);
+  void *ws = (h->device >= 0 && h->device < kMaxDevices) ? g_scratch_ptr[h->device] : nullptr;
+  if (!ws) return fail(h, MH_ERR_HIP, "no scratch address for device %d", h->device);
+  // Nothing here needs to start at zero: the folded MPO matrix and the
+  // intermediates are written in full before they are read (a split-K GEMM
+  // clears its output with a kernel).
+  h->noalloc = 1;
+  const int st = heff_apply_ws(h, op, x, y, ws, nullptr, false);
+  h->noalloc = 0;
+  return st;
 }
 
 int mh_mode_product(mh_handle h, int dtype, int P, int ni, int nj, int Q, const void *in,

@@ -67,6 +67,9 @@ struct mh_context {
   int *eig_err = nullptr;
   void *eig_state = nullptr;
   int autotune = 1;
+  // Set for the duration of an allocation-free entry point: no tuning runs,
+  // no tuner-table insertions and no profiling events (host or device).
+  int noalloc = 0;
   int gemm_force = -1;  // FP64 GEMM configuration forced for every call (-1: tuned)
   int tune_log = 0;
 };
@@ -164,6 +167,11 @@ int bs_check_square(mh_handle h, const mh_bs_plan_s &p, const void *p1, const vo
 
 // Release the eigensolver's streams, library handles and scratch.
 void eig_release(mh_handle h);
+
+// Device address of the allocation-free entry points' static scratch array
+// on h's device (looked up once per device when the first handle of that
+// device is created).
+int noalloc_setup(mh_handle h);
 
 inline size_t dtype_bytes(int dtype) { return dtype == MH_C64 ? 16 : 8; }
 

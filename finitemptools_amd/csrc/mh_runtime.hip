@@ -69,7 +69,7 @@ static hipEvent_t take_event(mh_handle h) {
 
 void prof_begin(mh_handle h, int kclass, double flops, int *slot) {
   *slot = -1;
-  if (!h->profiling) return;
+  if (!h->profiling || h->noalloc) return;
   ProfRec r;
   r.start = take_event(h);
   r.stop = take_event(h);
@@ -142,6 +142,14 @@ int mh_create(int device, mh_handle *out) {
   if (const char *s = getenv("MH_KRYLOV_SMALL")) h->krylov_small = atoi(s) != 0;
   // MH_BS_PLAIN=1: block plans run one GEMM launch per sector product.
   if (const char *s = getenv("MH_BS_PLAIN")) h->bs_plain = atoi(s) != 0;
+  // The allocation-free entry points find their static scratch array's
+  // address on this device now, with the device current.
+  const int st = mh::noalloc_setup(h);
+  if (st != MH_OK) {
+    fprintf(stderr, "mh_create: %s\n", h->last_error.c_str());
+    delete h;
+    return st;
+  }
   *out = h;
   return MH_OK;
 }

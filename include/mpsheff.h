@@ -84,6 +84,22 @@ int mh_ac2_prime(mh_handle h, int dtype, int prec, int Dl, int d1, int d2,
                  int Dr, int wl, int wm, int wr, const void *x, const void *W1,
                  const void *W2, const void *L, const void *R, void *y);
 
+/* mh_ac2_prime without any allocation: no reference counterpart.  For
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, hosts that forbid hipMalloc).  Same arguments and
+ * result as mh_ac2_prime; prec must be MH_PREC_F64 (MH_ERR_UNSUPPORTED
+ * otherwise).  The intermediates live in a static 64 MiB device array that
+ * each GPU holds until the process ends (the handle's workspace is not
+ * touched): (wl + wr)*Dl*d1*d2*Dr elements (8 bytes real, 16 complex) plus
+ * the folded MPO matrix.  An operator that needs more returns MH_ERR_SHAPE
+ * (at w = 5, d = 2: D up to 458 real, 323 complex).  GEMM tiles come
+ * from the heuristic (no tuning runs), so results agree with mh_ac2_prime to
+ * rounding, not bit for bit.  All calls on one GPU share its array: calls
+ * that overlap on one GPU (two streams, two host threads) race. */
+int mh_ac2_prime_noalloc(mh_handle h, int dtype, int prec, int Dl, int d1, int d2, int Dr,
+                         int wl, int wm, int wr, const void *x, const void *W1,
+                         const void *W2, const void *L, const void *R, void *y);
+
 /* out[p, j, q] = sum_i in[p, i, q] Mt[i, j]: a small matrix on the middle
  * index of a 3-index tensor (in: P x ni x Q, out: P x nj x Q, Mt: ni x nj,
  * all column-major on the device; ni, nj <= 4096).  With Mt = G^T it applies
