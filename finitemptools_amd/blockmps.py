@@ -23,6 +23,7 @@ from . import blocksparse as bs
 from . import eigen
 from .blocksparse import (BlockEnv, BlockLayout, BlockTensor, EnvLayout, Space, qadd,
                           qsub, qzero)
+from . import truncation as tr
 from .truncation import MPSTruncation
 
 
@@ -601,6 +602,11 @@ def _batched_eigh(grams, dev):
         for j, c in enumerate(cs):
             n = grams[c].shape[0]
             Gb[j, :n, :n] = grams[c]
+            if n < nb:
+                # padding eigenvalues below the (PSD) spectrum: they come first
+                # and the true eigenvectors stay on the first n coordinates
+                pad = torch.arange(n, nb, device=dev)
+                Gb[j, pad, pad] = -(1.0 + grams[c].diagonal().abs().sum())
         lam, V = torch.linalg.eigh(Gb)
         return (nb, cs, lam, V)
 
@@ -632,7 +638,10 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
     """Device split by the sector reduced density matrices (White's original
     DMRG step): for the left sweep rho_c = M_c M_c^T gives u and s^2, and
     s v = u^T M_c is an exact projection; for the right sweep rho_c = M_c^T M_c
-    gives v and u s = M_c v^T.  The eigenpairs of all sectors come from one
+    gives v and u s = M_c v^T.  Where the truncation's relative cutoff falls
+    into the eigenvalues' noise (``truncation.rho_cut_resolved``) the bond is
+    split by the reference's sector SVDs instead (``split_two_site``); the
+    kept singular values are the row (column) norms of s v (u s).  The eigenpairs of all sectors come from one
     batched library eigh per size class (EIG_SOLVER = "library") or from
     `eigen.SectorEigh` (EIG_SOLVER = "sector": one tridiagonalisation launch
     for every sector, divide and conquer per sector, back-transform of the
@@ -686,11 +695,30 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
                 l = np.maximum(lh[j, ::-1][:r], 0.0)
                 lam_host[c] = np.sqrt(l)
                 eig[c] = (V, j, nb)
+    # the merged spectrum and the noise of each value: a sector's eigenvalues
+    # carry about n_c eps64 lambda_max,c of absolute error (truncation.py)
+    keys = list(lam_host)
+    s2 = np.concatenate([lam_host[c] ** 2 for c in keys])
+    noise = np.concatenate([np.full(len(lam_host[c]), tr.RHO_NOISE * order[c] * tr.EPS64 *
+                                    float(lam_host[c][0] ** 2 if len(lam_host[c]) else 0.0))
+                            for c in keys])
+    srt = np.argsort(-s2, kind="stable")
+    vecs = None
+    if not tr.rho_cut_resolved(s2[srt], noise[srt], trunc):
+        # the cut falls where sqrt(lambda) is noise
+        refined = None
+        if not use_sector and not PLAIN:
+            refined = _refine_tail(mats, left, order, lam_host, eig, dev, trunc)
+        if refined is None:
+            # the reference's sector SVDs
+            return split_two_site(tmap, vec.detach().cpu().numpy(), trunc, absorb, dev)
+        lam_host, vecs = refined
     keep = _global_keep_host({c: (None, s) for c, s in lam_host.items()}, trunc)
     keep = {c: k for c, k in keep.items() if lam_host[c][k - 1] > 0.0}
-    norm = float(np.sqrt(sum(float((lam_host[c][:k] ** 2).sum()) for c, k in keep.items())))
     # top-k eigenvectors of each kept sector, descending
-    if use_sector:
+    if vecs is not None:
+        top = {c: vecs[c][:, :k] for c, k in keep.items()}
+    elif use_sector:
         vk = se.vectors([keep.get(c, 0) for c in charges])
         top = {c: vk[j].flip(-1) for j, c in enumerate(charges) if c in keep}
     else:
@@ -701,19 +729,29 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
     mid = Space(keep.items())
     l1 = BlockLayout(xl.lspace, xl.phys, 1, mid)
     l2 = BlockLayout(mid, xl.phys, 1, xl.rspace)
+    # s v = Q^T M (left) or u s = M Q (right); the norms of its rows
+    # (columns) are the singular values, accurate where sqrt(lambda) is not
+    proj, s_dev = {}, []
+    for c in keep:
+        B = top[c].t() @ mats[c] if left else mats[c] @ top[c]
+        proj[c] = B
+        s_dev.append(torch.linalg.vector_norm(B, dim=1 if left else 0))
+    s_all = torch.cat(s_dev)
+    norm = torch.linalg.vector_norm(s_all)
+    s_host = (s_all / norm).cpu().numpy()
     svals, us, vs, upieces, vpieces = {}, [], [], [], []
-    ubase = vbase = 0
+    ubase = vbase = spos = 0
     for c, k in keep.items():
-        M = mats[c]
         R, C = tmap.shape[c]
         Q = top[c]
-        svals[c] = lam_host[c][:k] / norm
+        svals[c] = s_host[spos:spos + k]
+        spos += k
         if left:
             u = Q                                # R x k
-            v = (Q.t() @ M) / norm               # s v, normalised s
+            v = proj[c] / norm                   # s v, normalised s
         else:
             v = Q.t()                            # k x C
-            u = (M @ Q) / norm                   # u s
+            u = proj[c] / norm                   # u s
         us.append(u.reshape(-1))
         vs.append(v.reshape(-1))
         ro = 0
@@ -733,6 +771,100 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
     blk, mat = piece_index(np.array(vpieces, dtype=np.int64).reshape(-1, 7), dev)
     d2[blk] = torch.cat(vs)[mat]
     return BlockTensor(l1, d1), BlockTensor(l2, d2), svals
+
+
+# Level-1 eigenpairs at least SIGNAL_MARGIN times above their noise bound
+# are kept as they are; the rest of each sector is diagonalised again.
+SIGNAL_MARGIN = 1.0e3
+
+
+def _refine_tail(mats, left, order, lam_host, eig, dev, trunc):
+    """Second level of the density-matrix split, for a bond whose relative
+    cutoff falls below the resolution of sqrt(lambda).
+
+    Per sector, with X = M_c (left sweep) or M_c^T (right sweep) and the
+    level-1 eigenvectors Q = [Q_s | Q_n] of X X^T (Q_s: eigenvalues above
+    SIGNAL_MARGIN times the noise bound):
+      - s_i = ||q_i^T X|| for the signal rows (accurate to ~eps64 s_max);
+      - V_s = the rows of Q_s^T X normalised and orthonormalised (two
+        Newton-Schulz steps); the rows of Q_n^T X are projected off span(V_s)
+        (twice), which removes the signal admixture that Q_n's rounding
+        carries, so the block's Gram matrix G = P P^T has eigenvalues at its
+        own scale (its noise is eps64 times the tail's largest value, not the
+        sector's);
+      - the tail's values and vectors come from G's eigenpairs, W, with the
+        tail vectors Q_n W exactly orthogonal to Q_s.
+    Returns ({c: descending values}, {c: n_c x r_c vectors}) or None when the
+    cut is still not resolved (the caller then takes the SVD split)."""
+    tails, parts = {}, {}
+    vals, noise2 = {}, {}
+    for c, (V, j, nb) in eig.items():
+        n = order[c]
+        R, C = mats[c].shape
+        r = min(R, C)
+        X = mats[c] if left else mats[c].t()
+        lam = lam_host[c] ** 2
+        lam_max = float(lam[0]) if len(lam) else 0.0
+        bound = tr.RHO_NOISE * n * tr.EPS64 * lam_max
+        k_s = int(np.count_nonzero(lam > SIGNAL_MARGIN * bound))
+        Q = V[j, :n, nb - n:].flip(-1)                      # n x n, descending
+        Qs, Qn = Q[:, :k_s], Q[:, k_s:]
+        Bs = Qs.t() @ X
+        ss = torch.linalg.vector_norm(Bs, dim=1)
+        parts[c] = (Qs, Qn, ss, r, k_s)
+        # the level-2 values' noise: the tail Gram's own rounding plus the
+        # projection's, eps64 s_max on every entry
+        noise2[c] = (tr.RHO_NOISE * n * tr.EPS64) ** 2 * lam_max
+        if k_s >= r:
+            continue
+        Bn = Qn.t() @ X
+        if k_s > 0:
+            Vs = Bs / ss[:, None]
+            for _ in range(2):
+                Vs = 1.5 * Vs - 0.5 * (Vs @ Vs.t()) @ Vs
+            for _ in range(2):
+                Bn = Bn - (Bn @ Vs.t()) @ Vs
+        tails[c] = Bn @ Bn.t()
+    pending = _batched_eigh(tails, dev) if tails else []
+    # one transfer for the tail eigenvalues and the signal row norms
+    keys = list(parts)
+    flat = [p[2].reshape(-1) for p in pending] + [parts[c][2] for c in keys]
+    host = torch.cat(flat).cpu().numpy()
+    ntail = sum(p[2].numel() for p in pending)
+    lam_all = host[:ntail]
+    ss_host, pos = {}, ntail
+    for c in keys:
+        k_s = parts[c][4]
+        ss_host[c] = host[pos:pos + k_s]
+        pos += k_s
+    tail_host, tail_vec = {}, {}
+    pos = 0
+    for nb, cs, lam, W in pending:
+        lh = lam_all[pos:pos + lam.numel()].reshape(lam.shape)
+        pos += lam.numel()
+        for j, c in enumerate(cs):
+            Qs, Qn, ss, r, k_s = parts[c]
+            m = Qn.shape[1]
+            t = min(r - k_s, m)
+            mu = np.maximum(lh[j, ::-1][:t], 0.0)
+            tail_host[c] = mu
+            tail_vec[c] = Qn @ W[j, :m, nb - t:].flip(-1)
+    out_vals, out_vecs = {}, {}
+    s2_all, nz_all = [], []
+    for c, (Qs, Qn, ss, r, k_s) in parts.items():
+        sv = np.concatenate([ss_host[c] ** 2, tail_host.get(c, np.zeros(0))])
+        vv = torch.cat([Qs, tail_vec[c]], dim=1) if c in tail_vec else Qs
+        o = np.argsort(-sv, kind="stable")
+        out_vals[c] = np.sqrt(sv[o])
+        out_vecs[c] = vv[:, torch.as_tensor(o.copy(), device=vv.device)]
+        s2_all.append(sv)
+        nz_all.append(np.full(len(sv), noise2[c]))
+    s2 = np.concatenate(s2_all)
+    nz = np.concatenate(nz_all)
+    srt = np.argsort(-s2, kind="stable")
+    if not tr.rho_cut_resolved(s2[srt], nz[srt], trunc):
+        return None
+    return out_vals, out_vecs
 
 
 def _global_keep_host(svd, trunc):

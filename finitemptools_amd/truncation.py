@@ -21,6 +21,51 @@ DEFAULT_NONSYMMETRIC_D = 100  # truncation.jl:51
 # against 1.8 ms).
 SMALL_EIGH = True
 SMALL_EIGH_ORDER = 24
+# SVDs of two-site matrices up to this size (larger extent) run in host
+# LAPACK, which the reference calls too: at these orders a device SVD is
+# launch-latency bound.
+HOST_SVD_MAX = 256
+
+# A density-matrix eigenvalue of an order-n matrix carries an absolute error
+# of about n * eps64 * lambda_max (eps64 = 2^-52); singular values taken as
+# sqrt(lambda) are therefore only known to sqrt of that.  RHO_NOISE is the
+# safety factor on the bound.
+EPS64 = float(np.finfo(np.float64).eps)
+RHO_NOISE = 10.0
+
+
+def rho_cut_resolved(s2, noise, trunc):
+    """Whether MPSTruncation's decision on s = sqrt(s2) holds for every
+    spectrum within ``noise`` of s2.
+
+    s2: squared singular values from density-matrix eigenvalues, descending;
+    noise: absolute error bound of each (squared units).  The relative cutoff
+    compares every value with t = eps * ||s|| (truncation.jl:32): the cut is
+    resolved when no compared value lies within twice its noise of t^2.
+    When D binds instead (more than D values above t), only the D-th value
+    meets the cutoff.  Ties between values around a D cut are not tested: an
+    SVD breaks those arbitrarily too."""
+    s2 = np.asarray(s2, dtype=np.float64)
+    noise = np.broadcast_to(np.asarray(noise, dtype=np.float64), s2.shape)
+    if s2.size == 0:
+        return True
+    t2 = trunc.eps ** 2 * float(s2.sum())
+    k_eps = int(np.count_nonzero(s2 > t2))
+    if k_eps <= trunc.D:
+        return not bool(np.any(np.abs(s2 - t2) <= 2.0 * noise))
+    i = trunc.D - 1
+    return not (abs(float(s2[i]) - t2) <= 2.0 * float(noise[i]))
+
+
+def split_svd(x, trunc, absorb, normalize=True):
+    """The reference's split: stable_tsvd, s normalised (dmrg.jl:105) unless
+    normalize=False, absorbed into A2 ("right": u | s v) or A1 ("left")."""
+    u, s, v, _ = stable_tsvd(x, trunc)
+    if normalize:
+        s = s / torch.linalg.vector_norm(s)
+    if absorb == "right":
+        return u, v * s.to(v.dtype).reshape(-1, 1, 1), s
+    return u * s.to(u.dtype).reshape(1, 1, -1), v, s
 
 
 class MPSTruncation:
@@ -49,12 +94,17 @@ def split_rho(x, trunc, absorb, normalize=True):
     """Truncated split of a two-site tensor x[a,s1,s2,b] through the reduced
     density matrix (White's formulation of the DMRG step), FP64 on the
     device: the left sweep diagonalises rho = M M^dag (M = x as (a,s1)|(s2,b))
-    for u and s^2 and forms s v = u^dag M exactly; the right sweep
-    diagonalises M^dag M for v and forms u s = M v^dag.  The kept values and
-    the truncated tensor u s v are those of the SVD (truncation.jl:27-47 on
-    s = sqrt(lambda)), with s normalised as dmrg.jl:105 does (normalize=False
-    keeps the raw values, as the TDVP sweeps need).  Returns (A1, A2, s)
-    with s absorbed into A2 (absorb="right") or A1 ("left")."""
+    for u and forms s v = u^dag M exactly; the right sweep diagonalises
+    M^dag M for v and forms u s = M v^dag.  The cut of truncation.jl:27-47 is
+    taken on s = sqrt(lambda) when ``rho_cut_resolved`` shows it cannot move
+    within the eigenvalues' noise; otherwise (a relative cutoff that falls
+    below about sqrt(n eps64) s_max, where sqrt(lambda) is noise) the split
+    is the SVD split (``split_svd``).  The returned singular values are the
+    norms of the rows of s v (columns of u s), which are accurate to eps64
+    s_max where sqrt(lambda) is not; s is normalised as dmrg.jl:105 does
+    (normalize=False keeps the raw values, as the TDVP sweeps need).
+    Returns (A1, A2, s) with s absorbed into A2 (absorb="right") or A1
+    ("left")."""
     Dl, d1, d2, Dr = x.shape
     M = as_matrix(x, 2)
     left = absorb == "right"
@@ -72,30 +122,46 @@ def split_rho(x, trunc, absorb, normalize=True):
     lam_h = lam.detach().cpu().numpy()[::-1][:r]
     if not np.isfinite(lam_h).all():
         raise MHError(6, "split_rho: the eigensolver did not converge")
+    lam_max = max(float(lam_h[0]), 0.0)
     lam_h = np.maximum(lam_h, 0.0)
-    s_h = np.sqrt(lam_h)
-    k, _ = trunc.truncate(s_h)
-    k = max(1, min(k, int(np.count_nonzero(s_h > 0)) or 1))
-    norm = float(np.linalg.norm(s_h[:k])) if normalize else 1.0
+    if not rho_cut_resolved(lam_h, RHO_NOISE * n * EPS64 * lam_max, trunc):
+        return split_svd(x, trunc, absorb, normalize)
+    k, _ = trunc.truncate(np.sqrt(lam_h))
+    k = max(1, min(k, int(np.count_nonzero(lam_h > 0)) or 1))
     Q = V[:, n - k:].flip(-1)
-    s = torch.as_tensor(s_h[:k] / norm, device=x.device)  # normalize=False: the raw values
     if left:
-        A1 = from_matrix(Q, (Dl, d1, k))
-        A2 = from_matrix((Q.conj().t() @ M) / norm, (k, d2, Dr))
+        B = Q.conj().t() @ M                      # s v, rows of norm s
+        s = torch.linalg.vector_norm(B, dim=1)
     else:
-        A2 = from_matrix(Q.conj().t(), (k, d2, Dr))
-        A1 = from_matrix((M @ Q) / norm, (Dl, d1, k))
-    return A1, A2, s
+        B = M @ Q                                 # u s, columns of norm s
+        s = torch.linalg.vector_norm(B, dim=0)
+    if normalize:
+        norm = torch.linalg.vector_norm(s)
+        B = B / norm
+        s = s / norm
+    if left:
+        return from_matrix(Q, (Dl, d1, k)), from_matrix(B, (k, d2, Dr)), s
+    return from_matrix(B, (Dl, d1, k)), from_matrix(Q.conj().t(), (k, d2, Dr)), s
 
 
 def stable_tsvd(x, trunc):
     """x[a, s1, s2, b] -> u[a, s1, k], s[k] (host + device), v[k, s2, b], err."""
     Dl, d1, d2, Dr = x.shape
     M = as_matrix(x, 2)
-    try:
-        U, S, Vh = torch.linalg.svd(M, full_matrices=False)
-    except RuntimeError:
-        U, S, Vh = torch.linalg.svd(M, full_matrices=False, driver="gesvd")
+    if max(M.shape) <= HOST_SVD_MAX:
+        # LAPACK gesdd, gesvd if it fails (tsvd(alg=SDD()) and its fallback)
+        import scipy.linalg as sla
+        Mh = M.detach().cpu().numpy()
+        try:
+            Uh, Sh, Vhh = sla.svd(Mh, full_matrices=False, lapack_driver="gesdd", check_finite=False)
+        except (np.linalg.LinAlgError, ValueError):
+            Uh, Sh, Vhh = sla.svd(Mh, full_matrices=False, lapack_driver="gesvd", check_finite=False)
+        U, S, Vh = (torch.as_tensor(a, device=x.device) for a in (Uh, Sh, Vhh))
+    else:
+        try:
+            U, S, Vh = torch.linalg.svd(M, full_matrices=False)
+        except RuntimeError:
+            U, S, Vh = torch.linalg.svd(M, full_matrices=False, driver="gesvd")
     s_host = S.detach().cpu().numpy()
     k, err = trunc.truncate(s_host)
     k = max(k, 1)

@@ -369,3 +369,99 @@ def test_native_planner_rejects_out_of_range_charges():
     Lbad = bs.EnvLayout(xl.lspace, [(q[0] + big,) for q in Lay.qw])
     with pytest.raises(_lib.MHError):
         bs.host_native_tables(kind, xl, Lbad, Ray, W1, W2, prec)
+
+
+@pytest.mark.parametrize("absorb", ["left", "right"])
+@pytest.mark.parametrize("eps", [1e-8, 1e-10])
+def test_density_matrix_split_at_a_cutoff_inside_its_noise(absorb, eps, monkeypatch):
+    """Sector matrices whose merged spectrum falls from 1 to 1e-13, so the
+    relative cutoff lies where sqrt(lambda) of the sector density matrices is
+    noise: the block split must keep what the sector SVDs keep
+    (truncation.jl:27-47 on the merged spectrum), with the same values."""
+    from finitemptools_amd import truncation as tr_mod
+    from finitemptools_amd.truncation import MPSTruncation
+    rng = np.random.default_rng(13)
+    phys = bm.hubbard_charges()
+    Sl = rand_space(rng, [(a, b) for a in range(1, 3) for b in range(1, 3)], 3, 6)
+    Sr = rand_space(rng, reachable(Sl, phys, 2), 3, 6)
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    tmap = bm.TwoSiteMap(xl)
+    cat = np.zeros(xl.size)
+    for c in tmap.charges:
+        R, C = tmap.shape[c]
+        r = min(R, C)
+        top = rng.uniform(-1.0, 0.0)
+        s = np.logspace(top, top - 13, r)
+        U = np.linalg.qr(rng.standard_normal((R, r)))[0]
+        V = np.linalg.qr(rng.standard_normal((C, r)))[0]
+        cat[tmap.base[c]:tmap.base[c] + R * C] = ((U * s) @ V.T).ravel()
+    x = np.empty(xl.size)
+    x[tmap.perm] = cat
+    tr = MPSTruncation(D=10 ** 6, eps=eps)
+    a1, a2, s_ref = bm.split_two_site(tmap, x, tr, absorb, "cpu")
+    # the level-1 values cannot decide this cut; the second level must (no
+    # SVD fallback), and it must land on the SVD's kept set and values
+    xt = torch.as_tensor(x)
+    cat = torch.empty_like(xt)
+    cat[tmap.dest("cpu")] = xt
+    lam = []
+    for c in tmap.charges:
+        R, C = tmap.shape[c]
+        Mc = cat[tmap.base[c]:tmap.base[c] + R * C].view(R, C).numpy()
+        G = Mc @ Mc.T if absorb == "right" else Mc.T @ Mc
+        lam.append((np.linalg.eigvalsh(G)[::-1][:min(R, C)], G.shape[0]))
+    s2 = np.concatenate([np.maximum(l, 0) for l, _ in lam])
+    nz = np.concatenate([np.full(len(l), tr_mod.RHO_NOISE * n * tr_mod.EPS64 * max(l[0], 0))
+                         for l, n in lam])
+    o = np.argsort(-s2)
+    assert not tr_mod.rho_cut_resolved(s2[o], nz[o], tr)
+
+    def no_svd(*a, **k):
+        raise AssertionError("the two-level split fell back to the SVD")
+    monkeypatch.setattr(bm, "split_two_site", no_svd)
+    b1, b2, s_rho = bm.split_two_site_rho(tmap, xt, tr, absorb)
+    assert sorted(s_ref) == sorted(s_rho)
+    for c in s_ref:
+        assert len(s_ref[c]) == len(s_rho[c])
+        assert np.allclose(s_ref[c], s_rho[c], rtol=1e-10, atol=1e-15)
+    # the truncated tensors agree too
+    rec = []
+    for A1, A2 in ((a1, a2), (b1, b2)):
+        tab = bs.plan_two_site(A1.layout, A2.layout, xl)
+        rec.append(execute(tab, [A1.data.numpy(), np.zeros(xl.size), A2.data.numpy(),
+                                 None, None, None])[1])
+    assert np.allclose(rec[0], rec[1], atol=1e-11)
+
+
+@pytest.mark.parametrize("absorb", ["left", "right"])
+def test_density_matrix_split_where_D_binds_reads_values_off_the_projection(absorb, monkeypatch):
+    """D binds far above the noise: the density-matrix path runs, and the kept
+    singular values (read off s v or u s) match the sector SVDs to 1e-12
+    relative, also values three decades below the largest."""
+    from finitemptools_amd.truncation import MPSTruncation
+    rng = np.random.default_rng(14)
+    phys = bm.hubbard_charges()
+    Sl = rand_space(rng, [(a, b) for a in range(1, 3) for b in range(1, 3)], 4, 7)
+    Sr = rand_space(rng, reachable(Sl, phys, 2), 4, 7)
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    tmap = bm.TwoSiteMap(xl)
+    cat = np.zeros(xl.size)
+    for c in tmap.charges:
+        R, C = tmap.shape[c]
+        r = min(R, C)
+        s = np.logspace(0, -12, r) * rng.uniform(0.5, 1.0)
+        U = np.linalg.qr(rng.standard_normal((R, r)))[0]
+        V = np.linalg.qr(rng.standard_normal((C, r)))[0]
+        cat[tmap.base[c]:tmap.base[c] + R * C] = ((U * s) @ V.T).ravel()
+    x = np.empty(xl.size)
+    x[tmap.perm] = cat
+    tr = MPSTruncation(D=30, eps=1e-8)
+    _, _, s_ref = bm.split_two_site(tmap, x, tr, absorb, "cpu")
+
+    def no_svd(*a, **k):
+        raise AssertionError("the density-matrix path fell back to the SVD")
+    monkeypatch.setattr(bm, "split_two_site", no_svd)
+    _, _, s_rho = bm.split_two_site_rho(tmap, torch.as_tensor(x), tr, absorb)
+    assert sorted(s_ref) == sorted(s_rho)
+    for c in s_ref:
+        assert np.allclose(s_ref[c], s_rho[c], rtol=1e-12, atol=0)

@@ -343,3 +343,36 @@ def test_plain_per_sector_dmrg_matches_batched(gpu):
     with bm.plain_path():
         energies, _, _, _ = run_block_dmrg(models.hubbard_mpo(6, 1.0, 4.0), phys, psi, 64, 6)
     assert abs(energies[-1] - ED["hubbard_L6_t1_U4_N3_3"]) < 1e-8
+
+
+# per-site energy bounds of a plain mixed-precision DMRG2 run (the GEMM
+# operands rounded to FP32, or split into BF16 pairs: ~1e-6 / ~1e-5 relative
+# per matvec), and the criterion after one FP64 polish sweep
+MIXED_DE_PER_SITE = {"f32": 1e-5, "bf16x3": 5e-4}
+POLISH_DE_PER_SITE = 1e-9
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+def test_block_dmrg2_hubbard_L8_mixed_then_f64_polish(gpu, prec):
+    """Hubbard L=8 at half filling (U1 x U1, charge-density-wave start): six
+    mixed-precision DMRG2 sweeps land within MIXED_DE_PER_SITE of the sector
+    ED energy per site, and one FP64 sweep from that state reaches
+    |dE|/L <= 1e-9 (the mixed-then-FP64 schedule of the D=2048 benchmark)."""
+    from finitemptools_amd.dmrg import DMRG2
+    from finitemptools_amd.truncation import MPSTruncation
+    L = 8
+    phys = bm.hubbard_charges()
+    empty, double = phys.index((0, 0)), phys.index((1, 1))
+    psi = bm.prodmps(phys, [empty, double] * (L // 2))
+    mpo = bm.BlockMPO(models.hubbard_mpo(L, 1.0, 4.0), phys)
+    e_ed = ED["hubbard_L8_t1_U4_N4_4"]
+    print(f"mixed block DMRG2 {prec}", flush=True)
+    alg = DMRG2(trunc=MPSTruncation(D=256, eps=1e-10), prec=prec)
+    e_mixed, _ = bm.ground_state(psi, mpo, alg, nsweeps=6)
+    de_mixed = abs(e_mixed[-1] - e_ed) / L
+    assert de_mixed <= MIXED_DE_PER_SITE[prec], de_mixed
+    print(f"  mixed dE/L {de_mixed:.2e}; FP64 polish", flush=True)
+    alg = DMRG2(trunc=MPSTruncation(D=256, eps=1e-10), prec="f64")
+    e_pol, _ = bm.ground_state(psi, mpo, alg, nsweeps=1)
+    de_pol = abs(e_pol[-1] - e_ed) / L
+    assert de_pol <= POLISH_DE_PER_SITE, de_pol

@@ -138,3 +138,86 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120, cwd=root)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def _graded_matrix(rng, R, C, s, cplx=False):
+    """R x C matrix with singular values s (random orthonormal factors)."""
+    def orth(n, k):
+        A = rng.standard_normal((n, k))
+        if cplx:
+            A = A + 1j * rng.standard_normal((n, k))
+        return np.linalg.qr(A)[0]
+    k = len(s)
+    return (orth(R, k) * s) @ orth(C, k).conj().T
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("absorb", ["left", "right"])
+@pytest.mark.parametrize("eps", [1e-8, 1e-10])
+def test_density_matrix_split_at_a_cutoff_inside_its_noise(cplx, absorb, eps):
+    """A 24 x 24 two-site matrix whose spectrum runs from 1 down to 1e-12
+    straddles the relative cutoff: sqrt(lambda) from rho resolves s only down
+    to about sqrt(n eps64) ~ 7e-8, so a kept count taken from the density
+    matrix depends on rounding for this spectrum.  split_rho must return
+    the SVD split's kept count, singular values and truncation error
+    (truncation.jl:27-47 on stable_tsvd)."""
+    import torch
+    from finitemptools_amd.linalg import as_matrix
+    from finitemptools_amd.truncation import MPSTruncation, split_rho, stable_tsvd
+    rng = np.random.default_rng(31)
+    Dl, d, Dr = 12, 2, 12
+    s_true = np.logspace(0, -12, 24)
+    M = _graded_matrix(rng, Dl * d, d * Dr, s_true, cplx)
+    xt = torch.as_tensor(M.reshape(Dl, d, d, Dr, order="F"))
+    tr = MPSTruncation(D=100, eps=eps)
+    # what the check guards against: around the cut sqrt(eig(M M^dag)) is
+    # noise, so a count taken from it is a matter of rounding
+    lam = np.linalg.eigvalsh(M @ M.conj().T)[::-1]
+    s_rho = np.sqrt(np.maximum(lam, 0))
+    k_svd, err_svd = tr.truncate(np.linalg.svd(M, compute_uv=False))
+    near = np.abs(np.log10(s_true) - np.log10(eps)) < 1.0
+    assert np.max(np.abs(s_rho[near] - s_true[near]) / s_true[near]) > 0.1
+    A1, A2, s = split_rho(xt, tr, absorb)
+    u, s_ref, v, err = stable_tsvd(xt, tr)
+    assert len(s) == len(s_ref) == k_svd
+    assert np.isclose(err, err_svd, rtol=1e-6, atol=1e-15)
+    s_ref = s_ref / torch.linalg.vector_norm(s_ref)
+    assert np.allclose(s.numpy(), s_ref.numpy(), rtol=1e-10, atol=1e-15)
+    rec = (as_matrix(A1, 2) @ as_matrix(A2, 1)).numpy()
+    ref = (as_matrix(u, 2) @ torch.diag(s_ref.to(u.dtype)) @ as_matrix(v, 1)).numpy()
+    assert np.allclose(rec, ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("absorb", ["left", "right"])
+def test_density_matrix_split_where_D_binds_keeps_accurate_values(absorb):
+    """D binds well above the noise (the middle-bond case of a sweep): the
+    density-matrix path is taken, and its singular values, read off the
+    projected factor, match the SVD's to 1e-13 relative even four decades
+    below the largest (sqrt(lambda) alone is off by ~1e-8 there)."""
+    import torch
+    from finitemptools_amd import truncation as T
+    rng = np.random.default_rng(32)
+    Dl, d, Dr = 20, 2, 20
+    s_true = np.logspace(0, -12, 40)
+    M = _graded_matrix(rng, Dl * d, d * Dr, s_true)
+    xt = torch.as_tensor(M.reshape(Dl, d, d, Dr, order="F"))
+    tr = T.MPSTruncation(D=14, eps=1e-8)
+    lam = np.maximum(np.linalg.eigvalsh(M @ M.T)[::-1], 0)
+    assert T.rho_cut_resolved(lam, T.RHO_NOISE * 40 * T.EPS64 * lam[0], tr)
+    A1, A2, s = T.split_rho(xt, tr, absorb)
+    s_svd = np.linalg.svd(M, compute_uv=False)[:14]
+    assert np.allclose(s.numpy(), s_svd / np.linalg.norm(s_svd), rtol=1e-13, atol=0)
+
+
+def test_rho_cut_resolved_rules():
+    """The cutoff band: a relative cut inside the noise is unresolved, one far
+    above it resolved; with D binding only the D-th value meets the cutoff."""
+    from finitemptools_amd.truncation import MPSTruncation, rho_cut_resolved
+    s2 = np.array([1.0, 1e-2, 1e-4, 1e-6])
+    assert rho_cut_resolved(s2, 1e-15, MPSTruncation(D=10, eps=1e-8))
+    # t^2 = 1e-6 * 1.0101 lies 1.01e-8 above the last value: resolved at a
+    # noise of 1e-15, not at 1e-8
+    assert rho_cut_resolved(s2, 1e-15, MPSTruncation(D=10, eps=1e-3))
+    assert not rho_cut_resolved(s2, 1e-8, MPSTruncation(D=10, eps=1e-3))
+    assert rho_cut_resolved(np.append(s2, 1e-20), 1e-15, MPSTruncation(D=2, eps=1e-8))
+    assert not rho_cut_resolved(np.append(s2, 1e-20), 1e-15, MPSTruncation(D=10, eps=1e-8))

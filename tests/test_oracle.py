@@ -298,3 +298,138 @@ def test_two_site_heff_reduces_to_one_site_with_absorbed_env(cplx):
     lhs = np.einsum("astb,mtb->asm", y2, A2.conj())
     rhs = C.ac_prime(A1, W, L, C.updateright(R, A2, W, A2))
     assert np.allclose(lhs, rhs, rtol=1e-12, atol=1e-12 * np.abs(rhs).max())
+
+
+# ---------------------------------------------------------- Schur (Vector) forms
+
+def schur_entries(cell, rng=None):
+    """Schur site tensor of a dense cell: identity blocks as ("scal", c),
+    other nonzero blocks as ("op", O[1, d, 1, d]); rng rescales every entry
+    (so two sites differ and no coefficient is 1)."""
+    wl, d, wr, _ = cell.shape
+    out = {}
+    for i in range(wl):
+        for j in range(wr):
+            B = cell[i, :, j, :]
+            if not np.any(B):
+                continue
+            f = rng.uniform(0.5, 1.5) if rng is not None else 1.0
+            c = B[0, 0]
+            if np.array_equal(B, c * np.eye(d)):
+                out[(i, j)] = ("scal", f * c)
+            else:
+                out[(i, j)] = ("op", (f * B).reshape(1, d, 1, d))
+    return out
+
+
+SCHUR_CELLS = {
+    "xxz_field": lambda: OD_models().heisenberg_xxz_mpo(4, 1.0, 0.7, 0.3)[1],
+    "hubbard": lambda: OD_models().hubbard_mpo(4, 1.0, 4.0)[1],
+    "tfim_c64": lambda: OD_models().tfim_mpo(4, 1.0, 0.6, dtype=np.complex128)[1],
+}
+
+
+def OD_models():
+    from finitemptools_amd import models
+    return models
+
+
+@pytest.mark.parametrize("name", sorted(SCHUR_CELLS))
+def test_schur_ac2_prime_four_branches_match_kronecker(name):
+    """oracle.schur.ac2_prime (derivatives.jl:24-49, the four branches) equals
+    the Kronecker-product H_eff of the folded site tensors (independent of
+    the index strings); every branch is exercised."""
+    from oracle import schur as OS
+    rng = np.random.default_rng(41)
+    cell = SCHUR_CELLS[name]()
+    cplx = np.iscomplexobj(cell)
+    wl, d, wr, _ = cell.shape
+    h1, h2 = schur_entries(cell, rng), schur_entries(cell, rng)
+    branches = set()
+    for (i, j), e1 in h1.items():
+        for (j2, k), e2 in h2.items():
+            if j2 == j:
+                branches.add((e1[0], e2[0]))
+    assert branches == {("scal", "scal"), ("scal", "op"), ("op", "scal"), ("op", "op")}
+    Dl, Dr = 3, 4
+    hl = [rand(rng, (Dl, 1, Dl), cplx) for _ in range(wl)]
+    hr = [rand(rng, (Dr, 1, Dr), cplx) for _ in range(wr)]
+    x = rand(rng, (Dl, d, d, Dr), cplx)
+    y = OS.ac2_prime(x, h1, h2, hl, hr)
+    W1, W2 = OS.fold(h1, wl, wr, d), OS.fold(h2, wl, wr, d)
+    L = np.concatenate(hl, axis=1)
+    R = np.concatenate(hr, axis=1)
+    H = C.dense_heff2(W1, W2, L, R)
+    assert np.allclose(C.vec(y), H @ C.vec(x), rtol=0, atol=1e-12 * np.abs(H).max() * x.size)
+    # and the one-site Vector form (derivatives.jl:7-17) likewise
+    x1 = rand(rng, (Dl, d, Dr), cplx)
+    y1 = OS.ac_prime(x1, h1, hl, hr)
+    H1 = C.dense_heff1(W1, L, R)
+    assert np.allclose(C.vec(y1), H1 @ C.vec(x1), rtol=0, atol=1e-12 * np.abs(H1).max() * x1.size)
+
+
+def test_schur_ac2_prime_detects_a_dropped_branch():
+    """A restatement missing one branch fails the Kronecker check above:
+    each branch carries weight in this case."""
+    from oracle import schur as OS
+    rng = np.random.default_rng(42)
+    cell = SCHUR_CELLS["xxz_field"]()
+    wl, d, wr, _ = cell.shape
+    h1, h2 = schur_entries(cell, rng), schur_entries(cell, rng)
+    hl = [rand(rng, (3, 1, 3)) for _ in range(wl)]
+    hr = [rand(rng, (4, 1, 4)) for _ in range(wr)]
+    x = rand(rng, (3, d, d, 4))
+    full = OS.ac2_prime(x, h1, h2, hl, hr)
+    for kind in ("scal", "op"):
+        h1_cut = {k: v for k, v in h1.items() if v[0] != kind}
+        assert np.linalg.norm(OS.ac2_prime(x, h1_cut, h2, hl, hr) - full) > 1e-3 * np.linalg.norm(full)
+
+
+@pytest.mark.parametrize("name", sorted(SCHUR_CELLS))
+def test_schur_env_updates_match_folded_dense(name):
+    """mpohamiltonian/transfer.jl:48-75 term by term equals the dense
+    updateleft / updateright (mpo/transfer.jl:26-34, pinned three ways
+    above) on the folded site tensor and the gathered environment."""
+    from oracle import schur as OS
+    rng = np.random.default_rng(43)
+    cell = SCHUR_CELLS[name]()
+    cplx = np.iscomplexobj(cell)
+    wl, d, wr, _ = cell.shape
+    h = schur_entries(cell, rng)
+    W = OS.fold(h, wl, wr, d)
+    Dl, Dr = 4, 5
+    A, B = rand(rng, (Dl, d, Dr), cplx), rand(rng, (Dl, d, Dr), cplx)
+    hold_l = [rand(rng, (Dl, 1, Dl), cplx) for _ in range(wl)]
+    got = OS.updateleft(hold_l, A, h, B, wr)
+    ref = C.updateleft(np.concatenate(hold_l, axis=1), A, W, B)
+    for j in range(wr):
+        assert np.allclose(got[j][:, 0, :], ref[:, j, :], atol=1e-12 * np.abs(ref).max())
+    hold_r = [rand(rng, (Dr, 1, Dr), cplx) for _ in range(wr)]
+    got = OS.updateright(hold_r, A, h, B, wl)
+    ref = C.updateright(np.concatenate(hold_r, axis=1), A, W, B)
+    for i in range(wl):
+        assert np.allclose(got[i][:, 0, :], ref[:, i, :], atol=1e-12 * np.abs(ref).max())
+
+
+def test_schur_updateright_chain_gives_the_energy():
+    """<psi|H|psi> of an XXZ chain (field included) by Schur updateright from
+    the right boundary (ones on the last channel, mpohamiltonian/transfer.jl:3-24)
+    equals the exact-diagonalisation Hamiltonian's expectation."""
+    from finitemptools_amd import models
+    from oracle import schur as OS
+    L = 5
+    mpo = models.heisenberg_xxz_mpo(L, 1.0, 0.7, 0.3)
+    psi = OD.randommps(L, 2, 4, seed=5)
+    hold = None
+    for i in range(L - 1, -1, -1):
+        cell = mpo[i]
+        wl, d, wr, _ = cell.shape
+        h = schur_entries(cell)
+        if hold is None:
+            hold = [np.ones((1, 1, 1)) if j == wr - 1 else np.zeros((1, 1, 1)) for j in range(wr)]
+        hold = OS.updateright(hold, psi[i], h, psi[i], wl)
+        hold = [np.zeros_like(hold[0]) if t is None else t for t in hold]
+    E = hold[0].reshape(-1)[0]
+    v = OD.to_dense(psi)
+    H = ed.xxz_hamiltonian(L, 1.0, 0.7, 0.3)
+    assert abs(E - v @ (H @ v)) < 1e-12 * abs(v @ v) * np.abs(H).max() * 4
