@@ -1,19 +1,21 @@
 #!/usr/bin/env python3
-"""H_eff matvec benchmark on one MI355X.
+"""H_eff benchmark on one MI355X.
 
-Workload (the H_eff microbenchmark configuration): XXZ MPO tensor (w = 5,
-d = 2), environments L, R with iid N(0, 1/sqrt(D w)) entries, 32 random
-unit-norm input vectors.  One step = 32 two-site H_eff matvecs at D = 512
-(ac2_prime), each on its own vector, inputs resident in HBM.  The headline
-value is FP64 TFLOP/s with the algorithmic flop count
-F2 = 4 D^3 w d^2 + 4 D^2 w^2 d^3 per matvec.  The one-site matvec at
-D = 1024 (F1 = 4 D^3 w d + 2 D^2 w^2 d^2) is reported beside it.
-
-Beside the headline the same JSON line carries: the block-sparse Hubbard
-DMRG2 (L = 64, D ramp to 2048; matvec per precision, one sweep per
-precision, and both again as the plain per-sector FP64 loop), J1-J2 gradients (L = 40, D = 256), the TFIM TDVP2 quench
-(L = 100, 40 steps, vs the free-fermion closed form and FP64), the XXZ
-L = 20 D = 64 DMRG2 against exact diagonalisation, and the CPU baseline.
+Headline (the metric's configuration): the two-site H_eff matvec on the
+middle bond of the Hubbard chain at D = 2048 (L = 64, t = 1, U = 4, half
+filling, U1 x U1 charges, d = 4, w = 6): the bond's charge sectors come from
+a DMRG ramp (tests/golden/hubbard_L64_D2048_bonds.json), the environment
+blocks and the 32 unit input vectors are seeded Gaussians.  One step = 32
+grouped block-sparse FP64 matvecs, inputs resident in HBM; the value is
+TFLOP/s with the planner's flop count (sum of 2 m n k over the sector
+products, plus the block mode stage).  Beside it: the roofline of the
+grouped-GEMM kernel, the mixed precisions and the plain per-sector loop on
+the same inputs, the CPU per-sector restatement (baseline and full-size
+parity), the D = 2048 DMRG sweep seconds per precision (ramp from the
+charge-density-wave state, mixed sweep + FP64 polish schedule, plain
+per-sector sweep), the dense D = 512 microbenchmark, J1-J2 gradients
+(L = 40, D = 256), the TFIM TDVP2 quench (L = 100, 40 steps) and the XXZ
+L = 20 D = 64 DMRG2 against exact diagonalisation.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -95,6 +97,169 @@ def gpu_run(kind, D, steps, warmup, prec="f64", profile=False):
             "y0": yd[0].cpu().numpy(), "inputs": (W, L, R, xs), "prof": prof}
 
 
+BOND_FIXTURE = os.path.join(ROOT, "tests", "golden", "hubbard_L64_D2048_bonds.json")
+HUBBARD_L, MID_SITE = 64, 31
+
+
+def middle_bond_case(seed=0):
+    """The two-site H_eff of the Hubbard chain's middle bond at D = 2048
+    (L = 64, t = 1, U = 4, half filling, U1 x U1): charge sectors from the
+    committed fixture (a DMRG ramp to D = 2048), seeded iid N(0, 1/(D w))
+    environment blocks and NVEC seeded unit vectors."""
+    from finitemptools_amd import blockmps as bm
+    from finitemptools_amd import blocksparse as bs
+    from finitemptools_amd import models
+    with open(BOND_FIXTURE) as f:
+        b = json.load(f)["bonds"][str(MID_SITE)]
+
+    def sp(lst):
+        return bs.Space([(tuple(q), int(n)) for q, n in lst])
+
+    phys = bm.hubbard_charges()
+    xl = bs.BlockLayout(sp(b["x_l"]), phys, 2, sp(b["x_r"]))
+    Lay = bs.EnvLayout(sp(b["L_space"]), [tuple(q) for q in b["L_qw"]])
+    Ray = bs.EnvLayout(sp(b["R_space"]), [tuple(q) for q in b["R_qw"]])
+    mpo = models.hubbard_mpo(HUBBARD_L, 1.0, 4.0)
+    W1, W2 = mpo[MID_SITE], mpo[MID_SITE + 1]
+    rng = np.random.default_rng(seed)
+    std = 1.0 / np.sqrt(2048 * W1.shape[0])
+    Le = rng.standard_normal(Lay.size) * std
+    Re = rng.standard_normal(Ray.size) * std
+    xs = []
+    for i in range(NVEC):
+        x = np.random.default_rng(1000 + i).standard_normal(xl.size)
+        xs.append(x / np.linalg.norm(x))
+    return phys, xl, Lay, Ray, W1, W2, Le, Re, xs
+
+
+def _oracle_blocks(phys, xl, Lay, Ray, W1, W2, Le, Re, x):
+    """the CPU per-sector restatement on the same blocks"""
+    from finitemptools_amd import blocksparse as bs
+    from oracle import blocksparse as OB
+
+    def env_entries(lay):
+        return [((q, w), off, lay.space.dim[q], lay.space.dim[bs.qsub(q, lay.qw[w])])
+                for (q, w), off in lay.off.items()]
+
+    xe = [((b.qa, b.p[0], b.p[1], b.qb), b.off, b.rows, b.cols) for b in xl.blocks]
+    args = (OB.blocks_from_flat(x, xe), W1, W2, OB.blocks_from_flat(Le, env_entries(Lay)),
+            OB.blocks_from_flat(Re, env_entries(Ray)), phys, Lay.qw, Ray.qw,
+            dict(xl.lspace.sectors), dict(xl.rspace.sectors))
+    return (lambda: OB.flat_from_blocks(OB.ac2_prime(*args), xe, xl.size))
+
+
+def blocksparse_headline(steps, warmup, cpu=True, extras=True):
+    """One step = NVEC FP64 two-site H_eff matvecs on the D = 2048 middle
+    bond, each a grouped plan (two grouped-GEMM launches and one block mode
+    product), inputs resident in HBM.  Flops: the planner's sum of 2 m n k
+    over the sector products plus the mode stage's."""
+    import torch
+    from finitemptools_amd import _lib
+    from finitemptools_amd import blockmps as bm
+    from finitemptools_amd import blocksparse as bs
+    from finitemptools_amd.engine import engine
+    phys, xl, Lay, Ray, W1, W2, Le, Re, xs = middle_bond_case()
+    Ld = torch.as_tensor(Le, device="cuda")
+    Rd = torch.as_tensor(Re, device="cuda")
+    xd = [torch.as_tensor(x, device="cuda") for x in xs]
+    yd = [torch.empty_like(x) for x in xd]
+    plan = bs.make_plan("heff2", xl, Lay, Ray, W1, W2, "f64")
+    F = plan.flops()
+
+    def step(p):
+        for x, y in zip(xd, yd):
+            p.apply(x, Ld, Rd, y)
+
+    for _ in range(max(1, warmup)):
+        step(plan)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(plan)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / (steps * NVEC)
+    # a second pass with HIP events around every launch on the engine's stream
+    eng = engine(0)
+    eng.profile_reset()
+    eng.profile_enable(True)
+    step(plan)
+    torch.cuda.synchronize()
+    eng.profile_enable(False)
+    gms, gn, gfl = eng.profile_read(_lib.MH_K_GEMM)
+    wms, wn, wfl = eng.profile_read(_lib.MH_K_WSTAGE)
+    y0 = yd[0].cpu().numpy()
+    out = {"sec_per_matvec": sec, "flops": F, "tflops": F / sec / 1e12,
+           "gemm": {"ms": gms, "launches": gn, "flops": gfl},
+           "mode": {"ms": wms, "launches": wn, "flops": wfl},
+           "sectors": len(xl.lspace.sectors), "max_sector": max(d for _, d in xl.lspace.sectors),
+           "vector_len": xl.size}
+    if not extras:
+        return out
+    # mixed precisions and the plain per-sector FP64 loop on the same inputs
+    mixed = {}
+    for prec in ("f32", "bf16x3"):
+        pm = bs.make_plan("heff2", xl, Lay, Ray, W1, W2, prec)
+        step(pm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(pm)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / (steps * NVEC)
+        ym = yd[0].cpu().numpy()
+        mixed[prec] = {"ms": round(ms * 1e3, 4), "tflops": round(F / ms / 1e12, 3),
+                       "speedup_vs_f64": round(sec / ms, 3),
+                       "rel_err_vs_f64": float(np.linalg.norm(ym - y0) / np.linalg.norm(y0))}
+    out["mixed"] = mixed
+    with bm.plain_path():
+        pp = bs.make_plan("heff2", xl, Lay, Ray, W1, W2, "f64")
+        step(pp)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step(pp)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / NVEC
+    out["f64_plain_per_sector"] = {"ms": round(ms * 1e3, 4), "tflops": round(F / ms / 1e12, 3),
+                                   "grouped_speedup": round(ms / sec, 2)}
+    if cpu:
+        out["cpu"] = cpu_baseline_blocksparse(phys, xl, Lay, Ray, W1, W2, Le, Re, xs, F, y0)
+    return out
+
+
+def cpu_baseline_blocksparse(phys, xl, Lay, Ray, W1, W2, Le, Re, xs, F, y0, runs=5):
+    """The per-sector CPU restatement (oracle/blocksparse.py: one host BLAS
+    GEMM per coupled sector and step, as @tensor on graded tensors does) on
+    the same middle-bond blocks: median of `runs` matvecs after a warm-up on
+    all BLAS threads, and of 3 on one thread.  Also the full-size parity of
+    the FP64 device result against it."""
+    from threadpoolctl import threadpool_info, threadpool_limits
+    f = _oracle_blocks(phys, xl, Lay, Ray, W1, W2, Le, Re, xs[0])
+    yc = f()   # warm-up
+    parity = float(np.linalg.norm(y0 - yc) / np.linalg.norm(yc))
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        f()
+        ts.append(time.perf_counter() - t0)
+    threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    t1 = []
+    with threadpool_limits(1):
+        for _ in range(3):
+            t0 = time.perf_counter()
+            f()
+            t1.append(time.perf_counter() - t0)
+    med, med1 = float(np.median(ts)), float(np.median(t1))
+    return {"value": round(F / med / 1e12, 5), "unit": "TFLOP/s", "cores": threads,
+            "os_cpu_count": os.cpu_count(), "kind": "port",
+            "one_thread": {"value": round(F / med1 / 1e12, 5), "s_per_matvec": round(med1, 4)},
+            "s_per_matvec": round(med, 4),
+            "sample": (f"the D=2048 middle-bond two-site H_eff matvec (same blocks as the GPU "
+                       f"line), median of {runs} after a warm-up on {threads} BLAS threads "
+                       f"and of 3 on one thread; numpy per-sector restatement "
+                       f"(oracle/blocksparse.py)"),
+            "gpu_f64_rel_err_vs_cpu": parity}
+
+
 def complex_tfim_matvec(D=512, nvec=8, steps=5, precs=("f64", "f32", "bf16x3")):
     """The unit of work of the TDVP quench at its bond-dimension cap: the
     two-site H_eff of the TFIM chain (w = 3, d = 2) on ComplexF64 data at
@@ -147,34 +312,10 @@ def complex_tfim_matvec(D=512, nvec=8, steps=5, precs=("f64", "f32", "bf16x3")):
     return out
 
 
-def cpu_baseline(kind, D, budget_s=12.0):
-    """The oracle (numpy + host BLAS, same pairwise order as the reference)
-    on the host cores: as many of the same matvecs as fit in the budget."""
-    from oracle import contractions as C
-    W, L, R, xs = make_inputs(kind, D)
-    f = (lambda x: C.ac2_prime(x, W, W, L, R)) if kind == 2 else (lambda x: C.ac_prime(x, W, L, R))
-    f(xs[0])  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        f(xs[n % NVEC])
-        n += 1
-        if time.perf_counter() - t0 > budget_s or n >= NVEC:
-            break
-    sec = (time.perf_counter() - t0) / n
-    try:
-        from threadpoolctl import threadpool_info
-        cores = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
-    except Exception:
-        cores = os.cpu_count()
-    return {"value": round(flops(kind, D) / sec / 1e12, 4), "unit": "TFLOP/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{n} two-site H_eff matvecs at D={D} (numpy oracle, host BLAS), {sec * 1e3:.1f} ms each"}
-
-
-def pmc_traffic():
-    """HBM bytes per GEMM launch of the timed D=512 matvec, as measured by the
-    two counter runs summarised in profiles/ (None if the summary is absent)."""
-    path = os.path.join(ROOT, "profiles", "heff_f64_ac2_D512_pmc_traffic.md")
+def stored_traffic(name):
+    """HBM bytes per launch from a committed counter summary under profiles/
+    (its line "traffic_bytes_per_launch: N"), or None if absent."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -184,12 +325,12 @@ def pmc_traffic():
     return None
 
 
-def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
+def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048)):
     """Charge-conserving two-site DMRG of the Hubbard chain (t = 1, U = 4,
     half filling, U1 x U1): ramp from the charge-density-wave product state
-    through D = 256 .. 2048 (one sweep each), then (a) the grouped-GEMM
-    matvec on the middle bond in each precision and (b) one more sweep per
-    precision from the same D = 2048 state."""
+    through D = 256 .. 2048 (one sweep each), then one more sweep per
+    precision from the same D = 2048 state (each followed by an FP64 polish
+    sweep), and the same sweep as the plain per-sector FP64 loop."""
     import torch
     from finitemptools_amd import blockmps as bm
     from finitemptools_amd import blocksparse as bs
@@ -207,56 +348,20 @@ def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
         ts = time.perf_counter()
         e, _ = bm.sweep(env, alg)
         ramp_log.append({"D": D, "s": round(time.perf_counter() - ts, 2), "E_per_site": e[-1] / L})
+        print(f"  ramp D={D}: {ramp_log[-1]['s']} s", file=sys.stderr, flush=True)
     ramp_s = time.perf_counter() - t0
-    # (a) matvec on the middle bond
+    # planning cost of the middle bond's H_eff (native C planner vs Python)
     site = L // 2 - 1
     x = bm.two_site(psi[site], psi[site + 1])
     Le, Re = env.hstorage[site], env.hstorage[site + 2]
     args = (x.layout, Le.layout, Re.layout, mpo[site], mpo[site + 1])
     ts = time.perf_counter()
-    tab = bs.plan_heff2(*args)
+    bs.plan_heff2(*args)
     py_plan_ms = 1e3 * (time.perf_counter() - ts)
     ts = time.perf_counter()
     bs.host_native_tables("heff2", *args, "f64")
     native_plan_ms = 1e3 * (time.perf_counter() - ts)
-    mid = psi.bond_space(site + 1)
-    mv = {"sectors": len(mid.sectors), "max_sector": max(d for _, d in mid.sectors),
-          "gflop_per_matvec": round(tab.flops() / 1e9, 3),
-          "plan_ms": {"python": round(py_plan_ms, 2), "native": round(native_plan_ms, 2)},
-          "dense_equivalent_gflop": round((4.0 * 2048 ** 3 * 6 * 16 + 4.0 * 2048 ** 2 * 36 * 64) / 1e9, 1)}
-    yref = None
-    for prec in ("f64", "f32", "bf16x3"):
-        plan = bs.make_plan("heff2", *args, prec)
-        y = torch.empty_like(x.data)
-        plan.apply(x.data, Le.data, Re.data, y)
-        torch.cuda.synchronize()
-        ts = time.perf_counter()
-        for _ in range(nmatvec):
-            plan.apply(x.data, Le.data, Re.data, y)
-        torch.cuda.synchronize()
-        sec = (time.perf_counter() - ts) / nmatvec
-        ent = {"ms": round(sec * 1e3, 4), "tflops": round(tab.flops() / sec / 1e12, 3)}
-        if yref is None:
-            yref = y.clone()
-        else:
-            ent["rel_err_vs_f64"] = float(torch.linalg.vector_norm(y - yref) / torch.linalg.vector_norm(yref))
-        mv[prec] = ent
-    # the plain per-sector FP64 loop: one GEMM launch per sector product
-    plan = bs.make_plan("heff2", *args, "f64")
-    y = torch.empty_like(x.data)
-    with bm.plain_path():
-        plan.apply(x.data, Le.data, Re.data, y)
-        torch.cuda.synchronize()
-        ts = time.perf_counter()
-        for _ in range(nmatvec):
-            plan.apply(x.data, Le.data, Re.data, y)
-        torch.cuda.synchronize()
-        sec = (time.perf_counter() - ts) / nmatvec
-    mv["f64_plain_per_sector"] = {
-        "ms": round(sec * 1e3, 4), "tflops": round(tab.flops() / sec / 1e12, 3),
-        "rel_err_vs_f64": float(torch.linalg.vector_norm(y - yref) / torch.linalg.vector_norm(yref))}
-    for prec in ("f64", "f32", "bf16x3"):
-        mv[prec]["speedup_vs_plain_f64"] = round(mv["f64_plain_per_sector"]["ms"] / mv[prec]["ms"], 3)
+    plan_ms = {"python": round(py_plan_ms, 2), "native": round(native_plan_ms, 2)}
     # (b) one sweep per precision from the same state
     base_sites = [bm.BlockTensor(t.layout, t.data.clone()) for t in psi.sites]
     base_env = [bm.BlockEnv(q.layout, q.data.clone()) for q in env.hstorage]
@@ -271,9 +376,11 @@ def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
         e, delta = bm.sweep(env, alg)
         torch.cuda.synchronize()
         st = alg.stats
+        print(f"  sweep D=2048 {prec}: {time.perf_counter() - ts:.2f} s", file=sys.stderr, flush=True)
         sweeps[prec] = {"s": round(time.perf_counter() - ts, 3), "E_per_site": e[-1] / L,
                         "lanczos_s": round(st["lanczos_s"], 3), "split_s": round(st.get("split_s", 0), 3),
                         "plan_s": round(st.get("plan_s", 0), 3), "env_s": round(st["env_s"], 3),
+                        "matvec_flops": st["matvec_flops"],
                         "lanczos_tflops": round(st["matvec_flops"] / max(st["lanczos_s"], 1e-9) / 1e12, 3)}
         # schedule: this sweep, then one FP64 polish sweep from its state
         env.prec = "f64"
@@ -316,7 +423,7 @@ def blocksparse_hubbard(L=64, ramp=(256, 512, 1024, 2048), nmatvec=20):
         ent["meets_1e-9"] = bool(abs(ent["dE_per_site_vs_f64+f64"]) <= 1e-9)
         ent["speedup_vs_f64+f64"] = round(ref["s"] / ent["s"], 3)
     return {"workload": f"Hubbard L={L} t=1 U=4 half filling U1xU1, DMRG2 ramp {list(ramp)}",
-            "ramp_s": round(ramp_s, 2), "ramp": ramp_log, "matvec_middle_bond": mv,
+            "ramp_s": round(ramp_s, 2), "ramp": ramp_log, "plan_ms_middle_bond": plan_ms,
             "sweep_D2048": sweeps, "schedule_mixed_then_f64": schedule}
 
 
@@ -484,8 +591,10 @@ def main():
     ap.add_argument("--no-blocksparse", action="store_true")
     ap.add_argument("--no-gradients", action="store_true")
     ap.add_argument("--no-tdvp", action="store_true")
-    ap.add_argument("--only", choices=("all", "blocksparse", "complex"), default="all",
-                    help="run only the block-sparse section or the complex matvec (profiling)")
+    ap.add_argument("--only", choices=("all", "headline", "blocksparse", "complex", "dense"),
+                    default="all",
+                    help="run one section only (profiling): the D=2048 middle-bond matvec, "
+                         "the block-sparse ramp and sweeps, the complex or the dense D=512 matvec")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="first copy MiB of device memory three times (counter calibration)")
     args = ap.parse_args()
@@ -509,6 +618,10 @@ def main():
             torch.mul(a, 1.0, out=b)   # elementwise: reads and writes MiB each
         torch.cuda.synchronize()
         del a, b
+    if args.only == "headline":
+        hb = blocksparse_headline(args.steps, args.warmup, cpu=False, extras=False)
+        print(json.dumps(hb))
+        return
     if args.only == "blocksparse":
         print(json.dumps({"blocksparse_hubbard_L64_D2048": blocksparse_hubbard()}))
         return
@@ -516,6 +629,113 @@ def main():
         print(json.dumps({"ac2_prime_D512_c64_tfim": complex_tfim_matvec()}))
         return
 
+    import ctypes
+    from finitemptools_amd.engine import engine
+    eng = engine(0)
+    peak_meas = ctypes.c_double()
+    eng.check(eng.lib.mh_probe_mfma_f64(eng.bind_stream(), ctypes.byref(peak_meas)))
+
+    # ---- headline: the D = 2048 middle-bond block-sparse matvec
+    print("block-sparse D=2048 middle-bond matvec ...", file=sys.stderr, flush=True)
+    hb = blocksparse_headline(args.steps, args.warmup, cpu=not args.no_cpu)
+    print(f"  {hb['tflops']:.2f} TFLOP/s", file=sys.stderr, flush=True)
+    gms, gn, gfl = hb["gemm"]["ms"], hb["gemm"]["launches"], hb["gemm"]["flops"]
+    achieved = gfl / (gms * 1e-3) / 1e12
+    # algorithmic bytes of a grouped-GEMM launch: its two operand buffers read
+    # once and its output written once, averaged over the plan's two stages
+    # (step 1: L, x -> T1; step 3: T2, R -> y)
+    from finitemptools_amd import blocksparse as bs
+    _, xl, Lay, Ray, W1, W2, _, _, _ = middle_bond_case()
+    sz = bs.make_plan("heff2", xl, Lay, Ray, W1, W2, "f64").sizes()
+    alg_bytes = 8.0 * (sz[0] + sz[2] + sz[4] + sz[5] + sz[3] + sz[1]) / 2.0
+    traffic = stored_traffic("blocksparse_hubbard_L64_D2048_pmc.md")
+
+    if args.only == "dense":
+        print(json.dumps({"ac2_prime_D512_f64": dense_d512(args, C, peak_meas.value)}))
+        return
+    out = {
+        "metric": METRIC,
+        "value": round(hb["tflops"], 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(hb["sec_per_matvec"] * NVEC * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": ("synthetic: charge sectors of the Hubbard L=64 D=2048 middle bond "
+                 "(tests/golden/hubbard_L64_D2048_bonds.json, from a DMRG ramp), seeded "
+                 "Gaussian environment blocks and unit input vectors"),
+        "config": {"workload": ("Hubbard L=64 D=2048 (U1xU1, d=4, w=6) middle-bond two-site "
+                                "H_eff, grouped block-sparse FP64, 32 vectors per step"),
+                   "L": HUBBARD_L, "D": 2048, "site": MID_SITE, "sectors": hb["sectors"],
+                   "max_sector": hb["max_sector"], "vector_len": hb["vector_len"],
+                   "gflop_per_matvec": round(hb["flops"] / 1e9, 4),
+                   "vectors_per_step": NVEC},
+        "roofline": {"bound": "mfma",
+                     "kernel": "k_ggemm: the two grouped-GEMM launches of the block plan",
+                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": traffic,
+                     "traffic_source": ("stored: profiles/blocksparse_hubbard_L64_D2048_pmc.md "
+                                        "(rocprofv3 FETCH_SIZE + WRITE_SIZE of the same launches, "
+                                        "copy-kernel calibrated)"),
+                     "algorithmic_bytes": alg_bytes,
+                     "measured_peak": round(peak_meas.value, 2),
+                     "frac_of_measured_peak": round(achieved / peak_meas.value, 4),
+                     "avg_launch_ms": round(gms / max(gn, 1), 5), "launches": gn,
+                     "gflop_per_launch": round(gfl / max(gn, 1) / 1e9, 4),
+                     "mode_stage": {"kernel": "k_bmode",
+                                    "avg_launch_ms": round(hb["mode"]["ms"] / max(hb["mode"]["launches"], 1), 5),
+                                    "launches": hb["mode"]["launches"]}},
+        "blocksparse_matvec_D2048": {"f64_ms": round(hb["sec_per_matvec"] * 1e3, 4),
+                                     "mixed": hb["mixed"],
+                                     "f64_plain_per_sector": hb["f64_plain_per_sector"]},
+    }
+    if "cpu" in hb:
+        out["cpu_baseline"] = hb["cpu"]
+    print("dense D=512 matvecs ...", file=sys.stderr, flush=True)
+    out["ac2_prime_D512_f64"] = dense_d512(args, C, peak_meas.value)
+    if not args.no_blocksparse:
+        print("block-sparse Hubbard ramp ...", file=sys.stderr, flush=True)
+        bsh = blocksparse_hubbard()
+        out["blocksparse_hubbard_L64_D2048"] = bsh
+        sw = bsh["sweep_D2048"]
+        out["sweep_D2048_s"] = {k: sw[k]["s"] for k in sw}
+        out["sweep_D2048_split_s"] = {k: sw[k].get("split_s") for k in sw}
+        if "cpu_baseline" in out:
+            # the CPU restatement's matvec rate applied to the FP64 sweep's
+            # Lanczos matvec flops (splits and environment updates left out)
+            rate = out["cpu_baseline"]["value"] * 1e12
+            out["cpu_baseline"]["sweep_D2048_lanczos_s_extrapolated"] = round(
+                sw["f64"]["matvec_flops"] / rate, 1)
+    if not args.no_gradients:
+        print("gradients ...", file=sys.stderr, flush=True)
+        out["gradients_j1j2_L40_D256"] = gradients_j1j2()
+        conv = {}
+        for j2 in (0.0, 0.25, 0.5):
+            conv[f"J2={j2}"] = gradients_converged_j1j2(J2=j2)
+            print(f"  converged gradient J2={j2} done", file=sys.stderr, flush=True)
+        # Majumdar-Ghosh point: the open even chain's ground state is the
+        # nearest-neighbour dimer product, so dE/dJ2 = sum <S_i . S_i+2> = 0
+        conv["majumdar_ghosh_abs_dE_dJ2"] = abs(conv["J2=0.5"]["dE_dJ2_adjoint"])
+        out["gradients_j1j2_L40_D256_converged"] = conv
+    if not args.no_tdvp:
+        print("tdvp TFIM L=100 ...", file=sys.stderr, flush=True)
+        out["tdvp_tfim_L100_quench"] = tdvp_tfim_quench()
+        print("complex TFIM matvec D=512 ...", file=sys.stderr, flush=True)
+        out["ac2_prime_D512_c64_tfim"] = complex_tfim_matvec()
+    print("xxz L=20 D=64 ...", file=sys.stderr, flush=True)
+    out["dmrg_xxz_L20_D64"] = xxz_dmrg_L20_D64(cpu=not args.no_cpu)
+    print(json.dumps(out))
+
+
+def dense_d512(args, C, peak_meas):
+    """The dense microbenchmark: 32 two-site matvecs at D = 512 (XXZ w = 5,
+    d = 2) and the one-site matvec at D = 1024, FP64 with its own launch
+    roofline, and the mixed precisions against FP64."""
     D2, D1 = 512, 1024
     r2 = gpu_run(2, D2, args.steps, args.warmup, profile=True)
     W, L, R, xs = r2["inputs"]
@@ -537,65 +757,22 @@ def main():
                        "ms": round(rm["sec_per_matvec"] * 1e3, 4),
                        "speedup_vs_f64": round(r2["sec_per_matvec"] / rm["sec_per_matvec"], 3),
                        "rel_err_vs_f64_oracle": float(np.linalg.norm(rm["y0"] - ref) / np.linalg.norm(ref))}
-    import ctypes
-    from finitemptools_amd.engine import engine
-    eng = engine(0)
-    peak_meas = ctypes.c_double()
-    eng.check(eng.lib.mh_probe_mfma_f64(eng.bind_stream(), ctypes.byref(peak_meas)))
     gms, gn, gfl = r2["prof"]["gemm"]
     avg_ms = gms / max(gn, 1)
-    per_launch = gfl / max(gn, 1)  # 2*M*N*K = 2 D^3 w d^2 for both launches here
-    achieved = per_launch / (avg_ms * 1e-3) / 1e12
-    wms, wn, _ = r2["prof"]["wstage"]
-    out = {
-        "metric": METRIC,
-        "value": round(r2["tflops"], 3),
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(r2["sec_per_matvec"] * NVEC * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded iid Gaussian environments, XXZ MPO, unit-norm vectors)",
-        "config": {"workload": "ac2_prime two-site H_eff matvec, D=512, d=2, w=5 (XXZ), 32 vectors per step",
-                   "D": D2, "d": 2, "w": 5, "vectors_per_step": NVEC},
-        "ac2_prime_D512_f64": {"tflops": round(r2["tflops"], 3), "ms": round(r2["sec_per_matvec"] * 1e3, 4),
-                               "rel_err_vs_oracle": err2},
-        "ac_prime_D1024_f64": {"tflops": round(r1["tflops"], 3), "ms": round(r1["sec_per_matvec"] * 1e3, 4)},
-        "ac2_prime_D512_mixed": mixed,
-        "roofline": {"bound": "mfma",
-                     "kernel": "the two launches of ac2_prime: k_heff_step1<5,4,5> (GEMM 1 with the "
-                               "mode product in its epilogue) and k_gemm (GEMM 3)",
-                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": pmc_traffic(),
-                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                                     "copy-kernel calibrated)",
-                     "algorithmic_bytes": 60.82e6,
-                     "measured_peak": round(peak_meas.value, 2),
-                     "frac_of_measured_peak": round(achieved / peak_meas.value, 4),
-                     "avg_launch_ms": round(avg_ms, 5), "launches": gn,
-                     "wstage_avg_ms": round(wms / max(wn, 1), 5)},
-    }
-    if not args.no_blocksparse:
-        print("block-sparse Hubbard ramp ...", file=sys.stderr, flush=True)
-        out["blocksparse_hubbard_L64_D2048"] = blocksparse_hubbard()
-    if not args.no_gradients:
-        print("gradients ...", file=sys.stderr, flush=True)
-        out["gradients_j1j2_L40_D256"] = gradients_j1j2()
-        out["gradients_j1j2_L40_D256_converged"] = gradients_converged_j1j2()
-    if not args.no_tdvp:
-        print("tdvp TFIM L=100 ...", file=sys.stderr, flush=True)
-        out["tdvp_tfim_L100_quench"] = tdvp_tfim_quench()
-        print("complex TFIM matvec D=512 ...", file=sys.stderr, flush=True)
-        out["ac2_prime_D512_c64_tfim"] = complex_tfim_matvec()
-    print("xxz L=20 D=64 ...", file=sys.stderr, flush=True)
-    out["dmrg_xxz_L20_D64"] = xxz_dmrg_L20_D64(cpu=not args.no_cpu)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(2, D2)
-    print(json.dumps(out))
+    achieved = gfl / max(gn, 1) / (avg_ms * 1e-3) / 1e12
+    return {"workload": "ac2_prime two-site H_eff matvec, D=512, d=2, w=5 (XXZ), 32 vectors per step",
+            "tflops": round(r2["tflops"], 3), "ms": round(r2["sec_per_matvec"] * 1e3, 4),
+            "rel_err_vs_oracle": err2,
+            "roofline": {"kernel": "k_heff_step1<5,4,5> and k_gemm (GEMM 3)",
+                         "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                         "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                         "frac_of_measured_peak": round(achieved / peak_meas, 4),
+                         "avg_launch_ms": round(avg_ms, 5), "launches": gn,
+                         "traffic": stored_traffic("heff_f64_ac2_D512_pmc_traffic.md"),
+                         "traffic_source": "stored: profiles/heff_f64_ac2_D512_pmc_traffic.md"},
+            "ac_prime_D1024_f64": {"tflops": round(r1["tflops"], 3),
+                                   "ms": round(r1["sec_per_matvec"] * 1e3, 4)},
+            "mixed": mixed}
 
 
 if __name__ == "__main__":

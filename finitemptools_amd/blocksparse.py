@@ -406,6 +406,12 @@ class Plan:
         self.eng.check(self.lib.mh_bs_plan_flops(self.ptr, _lib.ctypes.byref(f)), "flops")
         return f.value
 
+    def sizes(self):
+        """elements of the plan's six buffers (x, y, p1, p2, t1, t2)"""
+        out = (_lib.c_int64 * 6)()
+        self.eng.check(self.lib.mh_bs_plan_sizes(self.ptr, out), "sizes")
+        return list(out)
+
     def variants(self):
         n = _lib.c_int()
         self.eng.check(self.lib.mh_bs_plan_variants(self.ptr, _lib.ctypes.byref(n)), "variants")

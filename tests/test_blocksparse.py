@@ -465,3 +465,42 @@ def test_density_matrix_split_where_D_binds_reads_values_off_the_projection(abso
     assert sorted(s_ref) == sorted(s_rho)
     for c in s_ref:
         assert np.allclose(s_ref[c], s_rho[c], rtol=1e-12, atol=0)
+
+
+def x_entries(lay):
+    """(charge key, offset, rows, cols) of a two-site layout's blocks"""
+    return [((b.qa, b.p[0], b.p[1], b.qb), b.off, b.rows, b.cols) for b in lay.blocks]
+
+
+def env_entries(lay):
+    out = []
+    for (q, w), off in lay.off.items():
+        Dq = lay.space.dim[q]
+        Dk = lay.space.dim[bs.qsub(q, lay.qw[w])]
+        out.append(((q, w), off, Dq, Dk))
+    return out
+
+
+@pytest.mark.parametrize("site", [0, 2, 4])
+def test_per_sector_oracle_matches_dense_oracle(site):
+    """oracle.blocksparse.ac2_prime (the per-sector loop, derivatives.jl:21
+    on graded tensors) equals the dense oracle on the dense embedding of
+    random Hubbard U1 x U1 blocks, at edge and bulk bonds."""
+    from oracle import blocksparse as OB
+    rng = np.random.default_rng(60 + site)
+    L = 6
+    phys = bm.hubbard_charges()
+    mpo = bm.BlockMPO(models.hubbard_mpo(L, 1.0, 4.0), phys)
+    Sl = (bs.Space([((0, 0), 1)]) if site == 0 else
+          rand_space(rng, [(a, b) for a in range(1, 4) for b in range(1, 4)]))
+    Sr = (bs.Space([((3, 3), 1)]) if site + 2 == L else rand_space(rng, reachable(Sl, phys, 2)))
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    Lay, Ray = bs.EnvLayout(Sl, mpo.qw[site]), bs.EnvLayout(Sr, mpo.qw[site + 2])
+    x, Le, Re = rand_block(rng, xl), rand_env(rng, Lay), rand_env(rng, Ray)
+    y = OB.ac2_prime(OB.blocks_from_flat(x.data.numpy(), x_entries(xl)), mpo[site], mpo[site + 1],
+                     OB.blocks_from_flat(Le.data.numpy(), env_entries(Lay)),
+                     OB.blocks_from_flat(Re.data.numpy(), env_entries(Ray)), phys,
+                     mpo.qw[site], mpo.qw[site + 2], dict(Sl.sectors), dict(Sr.sectors))
+    flat = OB.flat_from_blocks(y, x_entries(xl), xl.size)
+    ref = OC.ac2_prime(x.to_dense(), mpo[site], mpo[site + 1], Le.to_dense(), Re.to_dense())
+    check_block_result(xl, flat, ref)
