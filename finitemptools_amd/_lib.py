@@ -127,6 +127,12 @@ SIGNATURES = {
                                        P(c_void_p)]),
     "mh_bs_plan_two_site": (c_int, [c_void_p, P(BsSectors), P(BsSectors), P(BsSectors), c_int,
                                     c_void_p, P(c_void_p)]),
+    "mh_bs_coupled_info": (c_int, [P(BsSectors), P(BsSectors), c_int, c_void_p, c_int, c_int,
+                                   P(c_int), c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mh_bs_to_coupled": (c_int, [c_void_p, P(BsSectors), P(BsSectors), c_int, c_void_p, c_int,
+                                 c_int, c_void_p, c_void_p]),
+    "mh_bs_from_coupled": (c_int, [c_void_p, P(BsSectors), P(BsSectors), c_int, c_void_p, c_int,
+                                   c_int, c_void_p, c_void_p]),
     "mh_bs_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mh_bs_lanczos_eig": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                   c_double, c_int, P(c_double), c_void_p, P(c_int), P(c_int)]),

@@ -831,3 +831,72 @@ def plan_two_site(l1, l2, xl):
     tab = PlanTables([l1.size, xl.size, l2.size, 0, 0, 0])
     tab.add_gemm(X, P1, Y, True, True, tasks)
     return tab
+
+
+# ------------------------------------------------ TensorKit coupled-sector form
+
+def coupled_info(layout, ncod):
+    """The coupled-sector structure of a BlockLayout seen as a TensorMap with
+    codomain (a, s_1..s_{ncod-1}) and domain (the rest, b): [(charge, rows,
+    cols, offset)] in ascending charge order (mh_bs_coupled_info, host only)."""
+    lib = _lib.load()
+    nq = len(layout.phys[0])
+    L, R = _Sectors(layout.lspace), _Sectors(layout.rspace)
+    ph = _charges(layout.phys, nq)
+    nc = _lib.c_int()
+    _lib.raise_for(lib.mh_bs_coupled_info(L.ref(), R.ref(), layout.d, _hp(ph), layout.nphys, ncod,
+                                          _lib.ctypes.byref(nc), None, None, None, None), None,
+                   "coupled_info")
+    n = nc.value
+    q = np.zeros((max(n, 1), nq), dtype=np.int32)
+    rows = np.zeros(max(n, 1), dtype=np.int32)
+    cols = np.zeros(max(n, 1), dtype=np.int32)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    _lib.raise_for(lib.mh_bs_coupled_info(L.ref(), R.ref(), layout.d, _hp(ph), layout.nphys, ncod,
+                                          _lib.ctypes.byref(nc), _hp(q), _hp(rows), _hp(cols),
+                                          _hp(offs)), None, "coupled_info")
+    return [(tuple(int(v) for v in q[i]), int(rows[i]), int(cols[i]), int(offs[i]))
+            for i in range(n)]
+
+
+def _coupled_call(name, t_layout, ncod, src, dst):
+    from .engine import engine
+    lib = _lib.load()
+    eng = engine(src.device)
+    nq = len(t_layout.phys[0])
+    L, R = _Sectors(t_layout.lspace), _Sectors(t_layout.rspace)
+    ph = _charges(t_layout.phys, nq)
+    h = eng.bind_stream()
+    eng.check(getattr(lib, name)(h, L.ref(), R.ref(), t_layout.d, _hp(ph), t_layout.nphys, ncod,
+                                 _dp(src), _dp(dst)), name)
+
+
+def to_coupled(t, ncod):
+    """BlockTensor -> {charge: (rows x cols) device matrix}: TensorKit's
+    blocks(t)[c] of the tensor with codomain (a, s_1..s_{ncod-1}), domain
+    (the other physical legs, b) (mh_bs_to_coupled)."""
+    info = coupled_info(t.layout, ncod)
+    total = info[-1][3] + info[-1][1] * info[-1][2] if info else 0
+    buf = torch.empty(max(total, 1), dtype=torch.float64, device=t.data.device)
+    _coupled_call("mh_bs_to_coupled", t.layout, ncod, t.data, buf)
+    return {q: buf[o:o + r * c].view(c, r).t() for q, r, c, o in info}
+
+
+def from_coupled(layout, ncod, mats):
+    """{charge: matrix} (TensorKit's blocks, as to_coupled returns them) ->
+    BlockTensor of ``layout`` (mh_bs_from_coupled)."""
+    info = coupled_info(layout, ncod)
+    if sorted(mats) != [q for q, *_ in info]:
+        raise _lib.DimensionMismatch(2, "coupled sectors do not match the layout")
+    dev = next(iter(mats.values())).device if mats else "cuda"
+    parts = []
+    for q, r, c, o in info:
+        m = mats[q]
+        if tuple(m.shape) != (r, c):
+            raise _lib.DimensionMismatch(2, f"coupled block {q}: {tuple(m.shape)} vs {(r, c)}")
+        parts.append(m.t().contiguous().reshape(-1))
+    buf = torch.cat(parts).to(torch.float64) if parts else torch.zeros(1, dtype=torch.float64,
+                                                                        device=dev)
+    out = torch.empty(max(layout.size, 1), dtype=torch.float64, device=buf.device)
+    _coupled_call("mh_bs_from_coupled", layout, ncod, buf, out)
+    return BlockTensor(layout, out[:layout.size])

@@ -579,6 +579,55 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ------------------------------------------------- coupled-sector copies
+// Block (rows x cols, column-major at blk_off) <-> region of a coupled
+// matrix (column-major, leading dimension ld, first element at mat_off).
+// One workgroup per 4096-element chunk of a piece.
+struct CPiece {
+  int64_t blk_off, mat_off;
+  int rows, cols, ld, e0;
+};
+
+__global__ void __launch_bounds__(256)
+    k_coupled_copy(const CPiece *__restrict__ pieces, const double *__restrict__ src,
+                   double *__restrict__ dst, int to_coupled) {
+  const CPiece p = pieces[blockIdx.x];
+  const int n = min(4096, p.rows * p.cols - p.e0);
+  for (int t = threadIdx.x; t < n; t += 256) {
+    const int e = p.e0 + t;
+    const int i = e % p.rows, j = e / p.rows;
+    const int64_t b = p.blk_off + e, m = p.mat_off + i + (int64_t)p.ld * j;
+    if (to_coupled) dst[m] = src[b];
+    else dst[b] = src[m];
+  }
+}
+
+// pieces: {blk_off, mat_off, rows, cols, ld} per piece (host); the table goes
+// to the device through the handle's small buffer.
+int bs_coupled_copy(mh_handle h, const std::vector<int64_t> &pc, const double *src, double *dst,
+                    int to_coupled) {
+  std::vector<CPiece> tab;
+  for (size_t k = 0; k + 5 <= pc.size(); k += 5) {
+    const int rows = (int)pc[k + 2], cols = (int)pc[k + 3];
+    for (int e0 = 0; e0 < rows * cols; e0 += 4096)
+      tab.push_back(CPiece{pc[k], pc[k + 1], rows, cols, (int)pc[k + 4], e0});
+  }
+  if (tab.empty()) return MH_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(h->stream, &cs);
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(h, MH_ERR_ARG, "coupled-sector copies cannot be captured (they upload a table)");
+  const size_t bytes = tab.size() * sizeof(CPiece);
+  MH_TRY(ensure_small(h, bytes));
+  MH_HIP(h, hipMemcpyAsync(h->small, tab.data(), bytes, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_coupled_copy, dim3((unsigned)tab.size()), dim3(256), 0, h->stream,
+                     (const CPiece *)h->small, src, dst, to_coupled);
+  MH_TRY(check_launch(h, "coupled-sector copy"));
+  // the table lives in the handle's small buffer, which a later call may reuse
+  MH_HIP(h, hipStreamSynchronize(h->stream));
+  return MH_OK;
+}
+
 // One launch of variant v of a grouped-GEMM stage.  The layout with neither
 // operand contiguous along its row index is served by the 64x64 FP64 /
 // 64x64 mixed variant only (no plan of this library produces it).

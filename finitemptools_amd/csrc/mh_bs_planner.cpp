@@ -12,6 +12,7 @@
 //
 // The loops and their orders are those of the Python planner
 // (finitemptools_amd/blocksparse.py), so both produce the same tables.
+#include <algorithm>
 #include <cstdint>
 #include <array>
 #include <cstring>
@@ -305,6 +306,122 @@ std::vector<int64_t> read_qw(const int32_t *q, int w, int nq) {
 
 using namespace mh;
 
+namespace {
+
+// TensorKit's coupled-sector form of an MPS-type tensor (the layout's
+// blocks regrouped into one column-major matrix per coupled charge): the
+// coupled charges in ascending lexicographic order, their shapes and element
+// offsets, and the block <-> matrix pieces {blk_off, mat_off, rows, cols, ld}.
+struct Coupled {
+  std::vector<std::vector<int32_t>> charge;
+  std::vector<int> rows, cols;
+  std::vector<int64_t> off;     // nc + 1
+  std::vector<int64_t> pieces;  // 5 per block
+  int64_t layout_size = 0;
+};
+
+int coupled_structure(mh_handle h, const mh_bs_sectors *left, const mh_bs_sectors *right, int d,
+                      const int32_t *phys, int nphys, int ncod, Coupled &out) {
+  if (!left || !right || !phys) return fail(h, MH_ERR_ARG, "coupled form: null argument");
+  if (nphys < 1 || nphys > 2) return fail(h, MH_ERR_ARG, "coupled form: nphys must be 1 or 2");
+  if (ncod < 1 || ncod > nphys + 1)
+    return fail(h, MH_ERR_ARG, "coupled form: ncod must lie in [1, %d]", nphys + 1);
+  const int nq = left->nq;
+  if (nq < 1 || nq > MAXQ) return fail(h, MH_ERR_ARG, "1..%d charge components", MAXQ);
+  if (d < 1 || d > 1000) return fail(h, MH_ERR_SHAPE, "physical dimension %d", d);
+  if (!charges_ok(phys, (size_t)d * nq)) return fail(h, MH_ERR_ARG, "physical charges out of range");
+  Sp Sl, Sr;
+  MH_TRY(mk_space(h, left, nq, Sl));
+  MH_TRY(mk_space(h, right, nq, Sr));
+  std::vector<int64_t> ph(d);
+  for (int s = 0; s < d; ++s) ph[s] = qkey(phys + (size_t)s * nq, nq);
+  Lay lay;
+  mk_layout(Sl, Sr, ph, nphys, lay, nq);
+  const int k = ncod - 1;  // physical legs in the codomain
+  // physical index tuples in lexicographic order (first index slowest)
+  std::vector<std::array<int, 2>> combos_cod, combos_dom;
+  auto combos = [&](int n, std::vector<std::array<int, 2>> &v) {
+    if (n == 0) v.push_back({-1, -1});
+    else if (n == 1)
+      for (int a = 0; a < d; ++a) v.push_back({a, -1});
+    else
+      for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b) v.push_back({a, b});
+  };
+  combos(k, combos_cod);
+  combos(nphys - k, combos_dom);
+  auto qsum = [&](const std::array<int, 2> &c) {
+    int64_t q = qkey(std::vector<int32_t>(nq, 0).data(), nq);
+    for (int x : c)
+      if (x >= 0) q = qadd(q, ph[x], nq);
+    return q;
+  };
+  // column pieces per coupled charge: right sector outer, then the domain
+  // physical indices; row pieces: left sector outer, then the codomain ones
+  std::unordered_map<int64_t, std::vector<std::pair<int64_t, int>>> colp, rowp;  // key -> (id, dim)
+  for (int b = 0; b < (int)Sr.q.size(); ++b)
+    for (size_t t = 0; t < combos_dom.size(); ++t)
+      colp[qsub(Sr.q[b], qsum(combos_dom[t]), nq)].push_back({(int64_t)b * 4096 + (int64_t)t, Sr.dim[b]});
+  for (int a = 0; a < (int)Sl.q.size(); ++a)
+    for (size_t t = 0; t < combos_cod.size(); ++t) {
+      const int64_t c = qadd(Sl.q[a], qsum(combos_cod[t]), nq);
+      if (colp.count(c)) rowp[c].push_back({(int64_t)a * 4096 + (int64_t)t, Sl.dim[a]});
+    }
+  std::vector<std::pair<std::vector<int32_t>, int64_t>> keys;
+  for (auto &kv : rowp) {
+    std::vector<int32_t> q(nq);
+    for (int i = 0; i < nq; ++i) q[i] = (int32_t)((kv.first >> (16 * i)) & 0xffff) - 32768;
+    keys.push_back({q, kv.first});
+  }
+  std::sort(keys.begin(), keys.end());
+  std::unordered_map<int64_t, int> cidx;
+  std::unordered_map<int64_t, int64_t> rstart, cstart;  // (c index, piece id) -> start
+  out.off.assign(1, 0);
+  for (auto &kv : keys) {
+    const int ci = (int)out.charge.size();
+    cidx[kv.second] = ci;
+    out.charge.push_back(kv.first);
+    int R = 0, C = 0;
+    for (auto &pr : rowp[kv.second]) {
+      rstart[(int64_t)ci * (1LL << 40) + pr.first] = R;
+      R += pr.second;
+    }
+    for (auto &pc : colp[kv.second]) {
+      cstart[(int64_t)ci * (1LL << 40) + pc.first] = C;
+      C += pc.second;
+    }
+    out.rows.push_back(R);
+    out.cols.push_back(C);
+    out.off.push_back(out.off.back() + (int64_t)R * C);
+  }
+  auto index_of = [&](const std::vector<std::array<int, 2>> &v, int x, int y) {
+    for (size_t t = 0; t < v.size(); ++t)
+      if (v[t][0] == x && v[t][1] == y) return (int64_t)t;
+    return (int64_t)-1;
+  };
+  for (const Blk &blk : lay.blocks) {
+    const int p[2] = {blk.p0, blk.p1};
+    std::array<int, 2> cod{-1, -1}, dom{-1, -1};
+    for (int i = 0, ic = 0, id = 0; i < nphys; ++i) {
+      if (i < k) cod[ic++] = p[i];
+      else dom[id++] = p[i];
+    }
+    const int64_t c = qadd(Sl.q[blk.qa], qsum(cod), nq);
+    const int ci = cidx.at(c);
+    const int64_t r0 = rstart.at((int64_t)ci * (1LL << 40) + (int64_t)blk.qa * 4096 +
+                                 index_of(combos_cod, cod[0], cod[1]));
+    const int64_t c0 = cstart.at((int64_t)ci * (1LL << 40) + (int64_t)blk.qb * 4096 +
+                                 index_of(combos_dom, dom[0], dom[1]));
+    const int R = out.rows[ci];
+    out.pieces.insert(out.pieces.end(),
+                      {blk.off, out.off[ci] + r0 + (int64_t)R * c0, blk.rows, blk.cols, R});
+  }
+  out.layout_size = lay.size;
+  return MH_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int mh_bs_plan_heff2(mh_handle h, int prec, const mh_bs_sectors *left,
@@ -577,6 +694,45 @@ int mh_bs_plan_two_site(mh_handle h, const mh_bs_sectors *left, const mh_bs_sect
   }
   const int64_t sizes[6] = {l1.size, xl.size, l2.size, 0, 0, 0};
   return finish(h, MH_PREC_F64, sizes, {0}, g, md, {{MH_BS_X, MH_BS_P1, MH_BS_Y, 1, 1}}, plan);
+}
+
+int mh_bs_coupled_info(const mh_bs_sectors *left, const mh_bs_sectors *right, int d,
+                       const int32_t *phys, int nphys, int ncod, int *nc, int32_t *charges,
+                       int32_t *rows, int32_t *cols, int64_t *offsets) {
+  if (!nc) return MH_ERR_ARG;
+  Coupled cp;
+  MH_TRY(coupled_structure(nullptr, left, right, d, phys, nphys, ncod, cp));
+  *nc = (int)cp.rows.size();
+  const int nq = left->nq;
+  for (int i = 0; i < *nc; ++i) {
+    if (charges) std::memcpy(charges + (size_t)i * nq, cp.charge[i].data(), nq * sizeof(int32_t));
+    if (rows) rows[i] = cp.rows[i];
+    if (cols) cols[i] = cp.cols[i];
+  }
+  if (offsets) std::copy(cp.off.begin(), cp.off.end(), offsets);
+  return MH_OK;
+}
+
+int mh_bs_to_coupled(mh_handle h, const mh_bs_sectors *left, const mh_bs_sectors *right, int d,
+                     const int32_t *phys, int nphys, int ncod, const double *blocks,
+                     double *coupled) {
+  if (!h) return MH_ERR_ARG;
+  Coupled cp;
+  MH_TRY(coupled_structure(h, left, right, d, phys, nphys, ncod, cp));
+  if ((cp.layout_size > 0 && !blocks) || (cp.off.back() > 0 && !coupled))
+    return fail(h, MH_ERR_ARG, "null buffer");
+  return bs_coupled_copy(h, cp.pieces, blocks, coupled, 1);
+}
+
+int mh_bs_from_coupled(mh_handle h, const mh_bs_sectors *left, const mh_bs_sectors *right, int d,
+                       const int32_t *phys, int nphys, int ncod, const double *coupled,
+                       double *blocks) {
+  if (!h) return MH_ERR_ARG;
+  Coupled cp;
+  MH_TRY(coupled_structure(h, left, right, d, phys, nphys, ncod, cp));
+  if ((cp.layout_size > 0 && !blocks) || (cp.off.back() > 0 && !coupled))
+    return fail(h, MH_ERR_ARG, "null buffer");
+  return bs_coupled_copy(h, cp.pieces, coupled, blocks, 0);
 }
 
 }  // extern "C"

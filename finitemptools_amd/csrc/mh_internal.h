@@ -165,6 +165,11 @@ int bs_prepare(mh_handle h, mh_bs_plan_s &p);
 // An operator plan (x and y of one size) with the environment buffers it needs.
 int bs_check_square(mh_handle h, const mh_bs_plan_s &p, const void *p1, const void *p2);
 
+// Block <-> coupled-matrix element copies; pieces: {blk_off, mat_off, rows,
+// cols, ld} per piece (mh_blocksparse.hip).
+int bs_coupled_copy(mh_handle h, const std::vector<int64_t> &pieces, const double *src,
+                    double *dst, int to_coupled);
+
 // Release the eigensolver's streams, library handles and scratch.
 void eig_release(mh_handle h);
 

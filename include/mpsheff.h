@@ -405,6 +405,35 @@ int mh_bs_lanczos_eig(mh_handle h, mh_bs_plan plan, const void *p1, const void *
                       const void *x0, int kmax, double btol, int reorth, double *eigval,
                       void *xout, int *info, int *niter);
 
+/* ---- TensorKit's coupled-sector block form ------------------------------
+ * A graded MPS-type tensor A[a, s1..sn, b] (nphys = n in {1, 2}; layout as
+ * the native planner's) seen as a TensorMap with codomain (a, s1..sk) and
+ * domain (s_{k+1}..sn, b), k = ncod - 1 (ncod in 1..n+1): TensorKit stores
+ * it as one column-major matrix per coupled charge c, blocks(t)[c]
+ * (the matrices behind abelian_matrix_from_dense,
+ * auxiliary/abelianmatrix.jl:112-124, and the graded states of
+ * mps/initializers.jl:34-73): rows are the codomain pieces (qa, s1..sk) with
+ * q(a) + sum q(s) = c, columns the domain pieces (s_{k+1}..sn, qb) with
+ * q(b) - sum q(s) = c.  Convention here: coupled charges in ascending
+ * lexicographic order; row pieces by left sector (caller's order), then
+ * s1..sk (s1 slowest); column pieces by right sector, then s_{k+1}..sn; the
+ * matrices concatenated in one buffer (element offsets[c]).  A host binding
+ * whose fusion-tree order differs permutes row or column pieces of equal
+ * charge.  mh_bs_coupled_info: nc coupled sectors; charges [nc * nq], rows
+ * [nc], cols [nc], offsets [nc + 1] (any output but nc may be NULL; host
+ * only, no device).  mh_bs_to_coupled / mh_bs_from_coupled: device copies
+ * between the block buffer (the layout's size) and the coupled buffer
+ * (offsets[nc] elements) on the handle's stream; they wait for the stream. */
+int mh_bs_coupled_info(const mh_bs_sectors *left, const mh_bs_sectors *right, int d,
+                       const int32_t *phys, int nphys, int ncod, int *nc, int32_t *charges,
+                       int32_t *rows, int32_t *cols, int64_t *offsets);
+int mh_bs_to_coupled(mh_handle h, const mh_bs_sectors *left, const mh_bs_sectors *right, int d,
+                     const int32_t *phys, int nphys, int ncod, const double *blocks,
+                     double *coupled);
+int mh_bs_from_coupled(mh_handle h, const mh_bs_sectors *left, const mh_bs_sectors *right, int d,
+                       const int32_t *phys, int nphys, int ncod, const double *coupled,
+                       double *blocks);
+
 /* ---- batched symmetric eigensolver (block-sparse two-site split) ---------
  * The FP64 step of the graded two-site split: the truncated SVD of every
  * charge sector (TensorKit tsvd under auxiliary/stable_svd.jl:4-18,

@@ -504,3 +504,60 @@ def test_per_sector_oracle_matches_dense_oracle(site):
     flat = OB.flat_from_blocks(y, x_entries(xl), xl.size)
     ref = OC.ac2_prime(x.to_dense(), mpo[site], mpo[site + 1], Le.to_dense(), Re.to_dense())
     check_block_result(xl, flat, ref)
+
+
+# ------------------------------------------------ coupled-sector form (host)
+
+def test_coupled_info_matches_the_two_site_map():
+    """mh_bs_coupled_info on a two-site layout with codomain (a, s1) and
+    domain (s2, b) gives the charges and matrix shapes of the split's
+    TwoSiteMap (the (a, s1) | (s2, b) matrices of stable_tsvd!, dmrg.jl:104),
+    in ascending charge order, contiguous."""
+    rng = np.random.default_rng(70)
+    phys = bm.hubbard_charges()
+    Sl = rand_space(rng, [(a, b) for a in range(1, 4) for b in range(1, 4)], 1, 6)
+    Sr = rand_space(rng, reachable(Sl, phys, 2), 1, 6)
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    tmap = bm.TwoSiteMap(xl)
+    info = bs.coupled_info(xl, 2)
+    assert [q for q, *_ in info] == sorted(tmap.charges)
+    off = 0
+    for q, r, c, o in info:
+        assert (r, c) == tmap.shape[q]
+        assert o == off
+        off += r * c
+    assert off == xl.size
+
+
+@pytest.mark.parametrize("ncod", [1, 2, 3])
+def test_coupled_info_one_and_two_site_splits(ncod):
+    """Every split point: element counts add up to the layout; for a one-site
+    tensor with codomain (a, s) the coupled charges are the right sectors and
+    each matrix stacks that sector's blocks (finitemps.jl:130-141)."""
+    rng = np.random.default_rng(71 + ncod)
+    phys = bm.hubbard_charges()
+    Sl = rand_space(rng, [(a, b) for a in range(1, 4) for b in range(1, 4)], 1, 5)
+    Sr = rand_space(rng, reachable(Sl, phys, 1), 1, 5)
+    one = bs.BlockLayout(Sl, phys, 1, Sr)
+    if ncod <= 2:
+        info = bs.coupled_info(one, ncod)
+        assert sum(r * c for _, r, c, _ in info) == one.size
+        if ncod == 2:
+            assert [q for q, *_ in info] == [q for q, _ in Sr.sectors if q in one.by_right]
+            for q, r, c, _ in info:
+                assert c == Sr.dim[q] and r == sum(b.rows for b in one.by_right[q])
+        else:
+            assert [q for q, *_ in info] == [q for q, _ in Sl.sectors if q in one.group]
+    Sr2 = rand_space(rng, reachable(Sl, phys, 2), 1, 5)
+    two = bs.BlockLayout(Sl, phys, 2, Sr2)
+    info = bs.coupled_info(two, ncod)
+    assert sum(r * c for _, r, c, _ in info) == two.size
+
+
+def test_coupled_info_rejects_bad_split():
+    from finitemptools_amd import _lib
+    phys = bm.hubbard_charges()
+    Sl = bs.Space([((1, 1), 2)])
+    lay = bs.BlockLayout(Sl, phys, 2, bs.Space([((2, 2), 3)]))
+    with pytest.raises(_lib.MHError):
+        bs.coupled_info(lay, 4)

@@ -376,3 +376,42 @@ def test_block_dmrg2_hubbard_L8_mixed_then_f64_polish(gpu, prec):
     e_pol, _ = bm.ground_state(psi, mpo, alg, nsweeps=1)
     de_pol = abs(e_pol[-1] - e_ed) / L
     assert de_pol <= POLISH_DE_PER_SITE, de_pol
+
+
+def test_coupled_form_matches_the_two_site_matrices(gpu):
+    """mh_bs_to_coupled on a Hubbard two-site tensor (codomain (a, s1), domain
+    (s2, b)) gives the sector matrices of the split's TwoSiteMap (pinned by
+    the SVD-split tests) and the dense tensor's (a, s1) x (s2, b) blocks; the
+    way back restores the block buffer bit for bit (one- and two-site, every
+    split point)."""
+    from tests_helpers_rand import rand_layouts
+    rng = np.random.default_rng(72)
+    phys = bm.hubbard_charges()
+    xl, one = rand_layouts(rng, phys)
+    x = bs.BlockTensor(xl, torch.as_tensor(rng.standard_normal(xl.size), device="cuda"))
+    print("coupled form: two-site", flush=True)
+    mats = bs.to_coupled(x, 2)
+    tmap = bm.TwoSiteMap(xl)
+    ref = tmap.matrices(x.data.cpu().numpy())
+    assert sorted(mats) == sorted(ref)
+    for q in ref:
+        assert np.array_equal(mats[q].cpu().numpy(), ref[q])
+    # against the dense tensor: rows (qa, s1) and columns (s2, qb) of charge q
+    dense = x.to_dense()
+    Sl, Sr = xl.lspace, xl.rspace
+    for q, M in mats.items():
+        rows = [(qa, s1) for qa, _ in Sl.sectors for s1 in range(xl.d)
+                if bs.qadd(qa, phys[s1]) == q and any(
+                    bs.qadd(q, phys[s2]) in Sr.dim for s2 in range(xl.d))]
+        cols = [(s2, qb) for qb, _ in Sr.sectors for s2 in range(xl.d)
+                if bs.qsub(qb, phys[s2]) == q]
+        R = np.vstack([np.hstack([dense[Sl.offset[qa]:Sl.offset[qa] + Sl.dim[qa], s1, s2,
+                                        Sr.offset[qb]:Sr.offset[qb] + Sr.dim[qb]]
+                                  for s2, qb in cols]) for qa, s1 in rows])
+        assert np.array_equal(M.cpu().numpy(), R)
+    for lay in (xl, one):
+        t = bs.BlockTensor(lay, torch.as_tensor(rng.standard_normal(lay.size), device="cuda"))
+        for ncod in range(1, lay.nphys + 2):
+            print(f"coupled round trip nphys={lay.nphys} ncod={ncod}", flush=True)
+            back = bs.from_coupled(lay, ncod, bs.to_coupled(t, ncod))
+            assert torch.equal(back.data, t.data)
