@@ -81,7 +81,7 @@ def test_every_tile_variant(gpu, prec, tol):
     tab, xl, x, Le, Re = heff_case(5, 3, 150)
     ref = execute(tab, [x, np.zeros(xl.size), Le, Re, None, None])[1]
     plan = bs.Plan(tab, prec)
-    assert plan.variants() == (4 if prec == "f64" else 3)
+    assert plan.variants() == (5 if prec == "f64" else 3)
     for v in range(plan.variants()):
         plan.force_variant(v)
         y = torch.full((xl.size,), float("nan"), dtype=torch.float64, device="cuda")
