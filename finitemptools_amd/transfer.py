@@ -20,17 +20,25 @@ def _prep(hold, psiA, hj, psiB):
     return [as_colmajor(t) for t in ts]
 
 
+_IDENTITY = {}
+
+
 def _identity_mpo(d, w, dtype, device, scale=1.0):
     """W[w, s', w, s] = scale * delta(s', s) delta(w_l, w_r): the reference's
     ``hj::Nothing`` / ``hj::Number`` branches (mpo/transfer.jl:37-55) keep the
-    MPO leg of ``hold`` and contract only the physical legs."""
+    MPO leg of ``hold`` and contract only the physical legs.  Built on the
+    device (column-major, as the engine reads it) once per (d, w, dtype,
+    device, scale) and cached."""
     import torch
-    W = torch.zeros((w, d, w, d), dtype=dtype)
-    for a in range(w):
-        for s in range(d):
-            W[a, s, a, s] = scale
-    from .engine import to_device
-    return to_device(W, device)
+    key = (int(d), int(w), dtype, str(device), complex(scale))
+    W = _IDENTITY.get(key)
+    if W is None:
+        Id = torch.eye(d, dtype=dtype, device=device) * scale
+        Iw = torch.eye(w, dtype=dtype, device=device)
+        dense = torch.einsum("ab,st->asbt", Iw, Id)          # [wl, s', wr, s]
+        W = dense.permute(3, 2, 1, 0).contiguous().permute(3, 2, 1, 0)
+        _IDENTITY[key] = W
+    return W
 
 
 def updateleft(hold, psiA, hj, psiB, prec="f64"):
