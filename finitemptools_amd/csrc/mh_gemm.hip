@@ -776,6 +776,257 @@ static void launch_step1(int Dl, int Dr, const double *L, const double *X, const
                      Dl, Dr, tiles_a, tiles_b, chunk, group);
 }
 
+// ComplexF64 form of k_heff_step1: L, x, Mt and T2 hold interleaved (re, im)
+// pairs.  The tile is BM = 32 WL rows (a, w) by BN = 32 complex columns
+// (s, b'); both operands go to LDS as re / im planes (A [k][r], B [c][k]);
+// each 16x16x4 step runs four real MFMAs (re re, -im im into the real part,
+// re im, im re into the imaginary part).  The epilogue stages the whole
+// 32-column tile ([plane][column][row], pitch BM + 1) and each thread forms
+// T2[a, j, b'] = sum_i T1[a, i, b'] Mt[i, j] in complex arithmetic for its
+// (a, b') pairs.  Tile order as k_heff_step1 (XCD-contiguous groups).
+template <int WL, int DS, int WR>
+__global__ void __launch_bounds__(256)
+    k_heff_step1_c(const double *__restrict__ L, const double *__restrict__ X,
+                   const double *__restrict__ Mt, double *__restrict__ T2, int Dl, int Dr,
+                   int tiles_a, int tiles_b, int xcd_chunk, int group) {
+  constexpr int BA = 32, BM = BA * WL, BN = 32, BK = 16;
+  constexpr int NI = WL * DS, NJ = DS * WR;
+  constexpr int WTM = BM / 2, WTN = BN / 2, TM = WTM / 16, TN = WTN / 16;
+  constexpr int BB = BN / DS;            // b' values per tile
+  constexpr int NPAIRS = BA * BB;        // (a, b') pairs per tile
+  static_assert(BN % DS == 0 && NPAIRS % 256 == 0, "tile");
+  using SA = Stage<false, BM, BK, true>;   // L rows, staged [k][r]
+  using SB = Stage<false, BN, BK, false>;  // x columns, staged [c][k]
+  constexpr int CP = BM + 1;
+  constexpr int NSTAGE = 2 * 2 * SA::SIZE + 2 * 2 * SB::SIZE;  // buffers x planes
+  constexpr int NEPI = 2 * BN * CP;
+  __shared__ __align__(16) double smem[NSTAGE > NEPI ? NSTAGE : NEPI];
+  // plane q of LDS buffer b: A at smem + (2 b + q) SA::SIZE, B after all of A
+  auto sA = [&](int b, int q) { return smem + (2 * b + q) * SA::SIZE; };
+  auto sB = [&](int b, int q) { return smem + 4 * SA::SIZE + (2 * b + q) * SB::SIZE; };
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int local = blockIdx.x >> 3;
+  const int t = (blockIdx.x & 7) * xcd_chunk + local;
+  if (local >= xcd_chunk || t >= tiles_a * tiles_b) return;
+  const int span = group * tiles_b, first = (t / span) * group;
+  const int gsz = min(tiles_a - first, group), wt = t % span;
+  const int a0 = (first + wt % gsz) * BA, c0 = (wt / gsz) * BN;
+  const int K = Dl, NC = DS * Dr;
+  const int64_t ldL = (int64_t)Dl * WL;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  int64_t aoff[SA::NPAIR], boff[SB::NPAIR];
+  bool aok[SA::NPAIR][2], bok[SB::NPAIR];
+#pragma unroll
+  for (int i = 0; i < SA::NPAIR; ++i) {
+    int r, k;
+    SA::coord(tid, i, r, k);
+    const int a = a0 + r % BA, w = r / BA;
+    aok[i][0] = a < Dl;
+    aok[i][1] = a + 1 < Dl;
+    // a is even and Dl is even: the clamped pair stays inside channel w
+    aoff[i] = (int64_t)min(a, Dl - 2) + (int64_t)Dl * w + (int64_t)k * ldL;
+  }
+#pragma unroll
+  for (int i = 0; i < SB::NPAIR; ++i) {
+    int r, k;
+    SB::coord(tid, i, r, k);
+    const int c = c0 + r;
+    bok[i] = c < NC;
+    boff[i] = (int64_t)min(c, NC - 1) * Dl + k;
+  }
+
+  // [plane][pair][element]
+  double ra[2][SA::NPAIR][2], rb[2][SB::NPAIR][2];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      const int kc = min(k0 + k, K - 1);
+      const double4 v = *reinterpret_cast<const double4 *>(L + 2 * (aoff[i] + (int64_t)(kc - k) * ldL));
+      ra[0][i][0] = v.x; ra[1][i][0] = v.y; ra[0][i][1] = v.z; ra[1][i][1] = v.w;
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+      const int kc = min(k0 + k, K - 2);
+      const double4 v = *reinterpret_cast<const double4 *>(X + 2 * (boff[i] + (kc - k)));
+      rb[0][i][0] = v.x; rb[1][i][0] = v.y; rb[0][i][1] = v.z; rb[1][i][1] = v.w;
+    }
+  };
+  auto store = [&](int buf, int k0) {
+#pragma unroll
+    for (int i = 0; i < SA::NPAIR; ++i) {
+      int r, k;
+      SA::coord(tid, i, r, k);
+      const bool kok = k0 + k < K;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        double2 v;
+        v.x = (aok[i][0] && kok) ? ra[q][i][0] : 0.0;
+        v.y = (aok[i][1] && kok) ? ra[q][i][1] : 0.0;
+        *reinterpret_cast<double2 *>(sA(buf, q) + k * SA::PITCH + r) = v;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < SB::NPAIR; ++i) {
+      int r, k;
+      SB::coord(tid, i, r, k);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        double2 v;
+        v.x = (bok[i] && k0 + k < K) ? rb[q][i][0] : 0.0;
+        v.y = (bok[i] && k0 + k + 1 < K) ? rb[q][i][1] : 0.0;
+        *reinterpret_cast<double2 *>(sB(buf, q) + r * SB::PITCH + k) = v;
+      }
+    }
+  };
+
+  d4_t acc[2][TM][TN];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[q][i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int kb = 0; kb < BK; kb += 16) {
+      double a[2][TM][4], b[2][TN][4];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const double *A = sA(buf, q);
+        const double *B = sB(buf, q);
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          const int r = wm * WTM + i * 16 + lr;
+#pragma unroll
+          for (int s = 0; s < 4; ++s) a[q][i][s] = A[(kb + 4 * lk + s) * SA::PITCH + r];
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const int r = wn * WTN + j * 16 + lr;
+          const double2 *src = reinterpret_cast<const double2 *>(&B[r * SB::PITCH + kb + 4 * lk]);
+          const double2 u = src[0], v = src[1];
+          b[q][j][0] = u.x; b[q][j][1] = u.y; b[q][j][2] = v.x; b[q][j][3] = v.y;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            const double nai = -a[1][i][s];
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[1][j][s], acc[0][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][i][s], b[0][j][s], acc[1][i][j], 0, 0, 0);
+          }
+    }
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  load(0);
+  store(0, 0);
+  __syncthreads();
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int k1 = (kt + 1) * BK;
+    load(k1);
+    compute(kt & 1);
+    store((kt + 1) & 1, k1);
+    __syncthreads();
+  }
+  compute((nk - 1) & 1);
+
+  // Epilogue: the whole tile to LDS, then the complex mode product.
+  double *const Cr = smem, *const Ci = smem + BN * CP;
+  __syncthreads();  // every wave is past its last compute: LDS is free
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = wm * WTM + i * 16 + lk + 4 * r;
+        const int n = wn * WTN + j * 16 + lr;
+        Cr[n * CP + m] = acc[0][i][j][r];
+        Ci[n * CP + m] = acc[1][i][j][r];
+      }
+  __syncthreads();
+  const int bq0 = c0 / DS;
+#pragma unroll
+  for (int q = 0; q < NPAIRS / 256; ++q) {
+    const int pr = tid + 256 * q;
+    const int al = pr % BA, bl = pr / BA;
+    const int a = a0 + al, bq = bq0 + bl;
+    double vr[NI], vi[NI];
+#pragma unroll
+    for (int s2 = 0; s2 < DS; ++s2)
+#pragma unroll
+      for (int w = 0; w < WL; ++w) {
+        vr[w + WL * s2] = Cr[(bl * DS + s2) * CP + al + BA * w];
+        vi[w + WL * s2] = Ci[(bl * DS + s2) * CP + al + BA * w];
+      }
+    if (a < Dl && bq < Dr) {
+      double *dst = T2 + 2 * (a + (int64_t)Dl * NJ * bq);
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        double orr = 0.0, oi = 0.0;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const double mr = Mt[2 * (i + NI * j)], mi = Mt[2 * (i + NI * j) + 1];
+          orr = __builtin_fma(vr[i], mr, orr);
+          orr = __builtin_fma(-vi[i], mi, orr);
+          oi = __builtin_fma(vr[i], mi, oi);
+          oi = __builtin_fma(vi[i], mr, oi);
+        }
+        *reinterpret_cast<double2 *>(dst + 2 * (int64_t)Dl * j) = make_double2(orr, oi);
+      }
+    }
+  }
+}
+
+template <int WL, int DS>
+static void launch_step1_c(int Dl, int Dr, const double *L, const double *X, const double *Mt,
+                           double *T2, hipStream_t s) {
+  const int tiles_a = (Dl + 31) / 32, tiles_b = (DS * Dr + 31) / 32;
+  const int chunk = (tiles_a * tiles_b + 7) / 8;
+  const int group = min(tiles_a, 8);
+  hipLaunchKernelGGL((k_heff_step1_c<WL, DS, WL>), dim3(8 * chunk), dim3(256), 0, s, L, X, Mt, T2,
+                     Dl, Dr, tiles_a, tiles_b, chunk, group);
+}
+
+// T2 = mode(L x) in one launch for ComplexF64 operators with wl = wr in
+// {3, 4, 5, 6} and a physical extent of 2 or 4; MH_ERR_UNSUPPORTED otherwise.
+int heff_step1_fused_c(mh_handle h, int Dl, int Dr, int wl, int ds, int wr, const void *L,
+                       const void *x, const void *Mt, void *T2) {
+  auto al16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+  if (wl != wr || Dl < 2 || Dl % 2 != 0 || !al16(L) || !al16(x) || !al16(T2) || !al16(Mt))
+    return MH_ERR_UNSUPPORTED;
+  const double *Lp = (const double *)L, *Xp = (const double *)x, *Mp = (const double *)Mt;
+  double *Tp = (double *)T2;
+  hipStream_t s = h->stream;
+  const double flops = 8.0 * Dl * (double)wl * ds * Dr * Dl;  // complex GEMM flops
+  int slot;
+#define MH_S1C(W, D)                                  \
+  if (wl == W && ds == D) {                           \
+    prof_begin(h, MH_K_GEMM, flops, &slot);           \
+    launch_step1_c<W, D>(Dl, Dr, Lp, Xp, Mp, Tp, s);  \
+    prof_end(h, slot);                                \
+    return check_launch(h, "fused complex H_eff step 1"); \
+  }
+  MH_S1C(3, 4) MH_S1C(3, 2) MH_S1C(4, 4) MH_S1C(4, 2) MH_S1C(5, 4) MH_S1C(5, 2)
+  MH_S1C(6, 4) MH_S1C(6, 2)
+#undef MH_S1C
+  return MH_ERR_UNSUPPORTED;
+}
+
 // T2 = mode(L x) in one launch for FP64 real operators with wl = wr in
 // {3,4,5,6,8} and a physical extent of 2 or 4 (one-site d = 2, 4; two-site
 // d = 2); MH_ERR_UNSUPPORTED otherwise (the caller runs GEMM + mode product).

@@ -391,9 +391,10 @@ int heff_apply_ws(mh_handle h, const mh_heff_op &op, const void *x, void *y,
                              wl, op.wm, wr, d1, op.d2, op.W1, op.W2, Mt));
   }
   int fused = MH_ERR_UNSUPPORTED;
-  if (h->fusion && op.dtype == MH_F64) {
+  if (h->fusion) {
     // T2 = mode(L x) without T1 in HBM
-    fused = heff_step1_fused(h, Dl, Dr, wl, ds, wr, op.L, x, Mt, T2);
+    fused = op.dtype == MH_F64 ? heff_step1_fused(h, Dl, Dr, wl, ds, wr, op.L, x, Mt, T2)
+                               : heff_step1_fused_c(h, Dl, Dr, wl, ds, wr, op.L, x, Mt, T2);
     if (fused != MH_OK && fused != MH_ERR_UNSUPPORTED) return fused;
   }
   if (fused != MH_OK) {
@@ -573,8 +574,9 @@ int mh_ac_prime_open(mh_handle h, int dtype, int Dl, int d, int Dr, int wl, int 
   char *T1 = (char *)h->ws;
   char *Mt = T1 + al(t1 * b);
   MH_TRY(build_mode_matrix(h, dtype, MODE_SITE, wl, 0, wr, d, 1, W, nullptr, Mt));
-  if (h->fusion && dtype == MH_F64) {
-    const int st = heff_step1_fused(h, Dl, Dr, wl, d, wr, L, x, Mt, P);
+  if (h->fusion) {
+    const int st = dtype == MH_F64 ? heff_step1_fused(h, Dl, Dr, wl, d, wr, L, x, Mt, P)
+                                   : heff_step1_fused_c(h, Dl, Dr, wl, d, wr, L, x, Mt, P);
     if (st != MH_ERR_UNSUPPORTED) return st;
   }
   GemmDesc g1 = gd(Dl * wl, d * Dr, Dl, L, 1, (int64_t)Dl * wl, 0, x, 1, Dl, T1, 1,

@@ -135,6 +135,10 @@ int build_mode_matrix(mh_handle h, int dtype, int kind, int wl, int wm, int wr,
 // MH_ERR_UNSUPPORTED when the shape has no fused kernel.
 int heff_step1_fused(mh_handle h, int Dl, int Dr, int wl, int ds, int wr, const void *L,
                      const void *x, const void *Mt, void *T2);
+// The same for ComplexF64 operators (interleaved re, im): wl = wr in
+// {3, 4, 5, 6} and a physical extent of 2 or 4.
+int heff_step1_fused_c(mh_handle h, int Dl, int Dr, int wl, int ds, int wr, const void *L,
+                       const void *x, const void *Mt, void *T2);
 // Packed-operand form for the mixed precisions (mh_gemm_mixed.hip): L, x and
 // T2 are packed words of prec's family; same support and return convention.
 int heff_step1_fused_mx(mh_handle h, int prec, int Dl, int Dr, int wl, int ds, int wr,

@@ -327,32 +327,34 @@ FUSED_CASES = [  # (kind, Wk or (w, d), Dl, Dr): shapes the fused first step tak
 ]
 
 
+@pytest.mark.parametrize("cplx", [False, True], ids=["f64", "c64"])
 @pytest.mark.parametrize("kind,Wk,Dl,Dr", FUSED_CASES)
-def test_fused_first_step_matches_unfused_and_oracle(gpu, kind, Wk, Dl, Dr):
-    """The fused GEMM + mode-product kernel (FP64 real, wl = wr in {3,4,5,6,8},
-    physical extent 2 or 4, even Dl) against the oracle and against the
-    two-launch path; ragged Dl, Dr that are not multiples of the 32 x 64/ds
-    block tile."""
+def test_fused_first_step_matches_unfused_and_oracle(gpu, kind, Wk, Dl, Dr, cplx):
+    """The fused GEMM + mode-product kernels (FP64 real: wl = wr in
+    {3,4,5,6,8}; ComplexF64: wl = wr in {3,4,5,6}; physical extent 2 or 4,
+    even Dl) against the oracle and against the two-launch path; ragged Dl,
+    Dr that are not multiples of the block tile (32 x 64 real, 32 x 32
+    complex)."""
     from finitemptools_amd.derivatives import ac2_prime, ac_prime
     from finitemptools_amd.engine import engine
     rng = np.random.default_rng(Dl * 31 + Dr)
     if isinstance(Wk, tuple):
         w, d = Wk
-        W = rand(rng, (w, d, w, d))
+        W = rand(rng, (w, d, w, d), cplx)
     else:
-        W = mpo_tensor(Wk)
+        W = mpo_tensor(Wk, cplx)
         w, d = W.shape[0], W.shape[1]
-    L, R = _envs(rng, Dl, w, Dr, w, False)
+    L, R = _envs(rng, Dl, w, Dr, w, cplx)
     eng = engine()
     ys = []
     for fused in (True, False):
         eng.set_fusion(fused)
         try:
             if kind == 2:
-                x = rand(np.random.default_rng(5), (Dl, d, d, Dr))
+                x = rand(np.random.default_rng(5), (Dl, d, d, Dr), cplx)
                 ys.append(host(ac2_prime(dev(x), dev(W), dev(W), dev(L), dev(R))))
             else:
-                x = rand(np.random.default_rng(5), (Dl, d, Dr))
+                x = rand(np.random.default_rng(5), (Dl, d, Dr), cplx)
                 ys.append(host(ac_prime(dev(x), dev(W), dev(L), dev(R))))
         finally:
             eng.set_fusion(True)
