@@ -57,8 +57,17 @@ struct Stage {
 };
 
 // One kernel for real (CPLX = false) and complex FP64.  Complex operands are
-// split into re/im LDS planes at the LDS store; each 16x16x4 step then runs
-// four real MFMAs (re*re, -im*im, re*im, im*re).
+// split into re/im LDS planes at the LDS store.  Wave tiles of at most four
+// 16x16 MFMA tiles (M3) run three real MFMAs per 16x16x4 step (Gauss's
+// three-multiplication form, as in zgemm3m):
+// P1 += Ar Br, P2 += Ai Bi, P3 += (Ar + Ai)(Br + Bi), and after the K loop
+// Re C = P1 - P2, Im C = P3 - P1 - P2.  The error is normwise of the same
+// order as the four-product form (Higham, "Stability of a method for
+// multiplying complex matrices with three real matrix multiplications",
+// SIAM J. Matrix Anal. Appl. 13 (1992)); the imaginary part's bound carries
+// (|Ar| + |Ai|)(|Br| + |Bi|) in place of |Ar||Bi| + |Ai||Br|.  Larger wave
+// tiles keep the four-product form (Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br),
+// whose two accumulator planes fit their registers.
 //
 // Split-K: blockIdx.z selects the K range [z*kchunk, (z+1)*kchunk).  With
 // gridDim.z == 2 the partial tiles are added with FP64 atomics onto a C that
@@ -243,9 +252,11 @@ __global__ void __launch_bounds__(64 * NW)
     }
   };
 
-  d4_t acc[NP][TM][TN];
+  constexpr bool M3 = CPLX && TM * TN <= 4;
+  constexpr int NQ = M3 ? 3 : NP;  // accumulator planes (P1, P2, P3 for M3)
+  d4_t acc[NQ][TM][TN];
 #pragma unroll
-  for (int q = 0; q < NP; ++q)
+  for (int q = 0; q < NQ; ++q)
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -299,21 +310,40 @@ __global__ void __launch_bounds__(64 * NW)
   };
   auto mma = [&](const double (&a)[NP][TM][4], const double (&b)[NP][TN][4]) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 4; ++s) {
+      if constexpr (M3) {
+        double sa[TM], sb[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+        for (int i = 0; i < TM; ++i) sa[i] = a[0][i][s] + a[1][i][s];
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          if constexpr (CPLX) {
+        for (int j = 0; j < TN; ++j) sb[j] = b[0][j][s] + b[1][j][s];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
+            acc[2][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[i], sb[j], acc[2][i][j], 0, 0, 0);
+          }
+      } else if constexpr (CPLX) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
             const double nai = -a[1][i][s];
             acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
             acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
             acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[1][j][s], acc[0][i][j], 0, 0, 0);
             acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][i][s], b[0][j][s], acc[1][i][j], 0, 0, 0);
-          } else {
-            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
           }
-        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
+      }
+    }
   };
   // One K step from LDS buffer buf.  A wave runs the slices of its K group
   // (all of them, or every other one for KG = 2), software-pipelined: the
@@ -373,6 +403,20 @@ __global__ void __launch_bounds__(64 * NW)
       __syncthreads();
       compute(1);
     }
+  }
+
+  if constexpr (M3) {
+    // (P1, P2, P3) -> (Re, Im) in planes 0 and 1
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double p1 = acc[0][i][j][r], p2 = acc[1][i][j][r];
+          acc[0][i][j][r] = p1 - p2;
+          acc[1][i][j][r] = acc[2][i][j][r] - p1 - p2;
+        }
   }
 
   if constexpr (KG == 2) {
@@ -779,8 +823,7 @@ static void launch_step1(int Dl, int Dr, const double *L, const double *X, const
 // ComplexF64 form of k_heff_step1: L, x, Mt and T2 hold interleaved (re, im)
 // pairs.  The tile is BM = 32 WL rows (a, w) by BN = 32 complex columns
 // (s, b'); both operands go to LDS as re / im planes (A [k][r], B [c][k]);
-// each 16x16x4 step runs four real MFMAs (re re, -im im into the real part,
-// re im, im re into the imaginary part).  The epilogue stages the whole
+// each 16x16x4 step runs the three real MFMAs of k_gemm's complex form.  The epilogue stages the whole
 // 32-column tile ([plane][column][row], pitch BM + 1) and each thread forms
 // T2[a, j, b'] = sum_i T1[a, i, b'] Mt[i, j] in complex arithmetic for its
 // (a, b') pairs.  Tile order as k_heff_step1 (XCD-contiguous groups).
@@ -886,9 +929,9 @@ __global__ void __launch_bounds__(256)
     }
   };
 
-  d4_t acc[2][TM][TN];
+  d4_t acc[3][TM][TN];  // P1, P2, P3 of the three-product form (as k_gemm)
 #pragma unroll
-  for (int q = 0; q < 2; ++q)
+  for (int q = 0; q < 3; ++q)
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -917,17 +960,21 @@ __global__ void __launch_bounds__(256)
         }
       }
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < 4; ++s) {
+        double sa[TM], sb[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) sa[i] = a[0][i][s] + a[1][i][s];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) sb[j] = b[0][j][s] + b[1][j][s];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
-            const double nai = -a[1][i][s];
             acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[0][j][s], acc[0][i][j], 0, 0, 0);
-            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
-            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[1][j][s], acc[0][i][j], 0, 0, 0);
-            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][i][s], b[0][j][s], acc[1][i][j], 0, 0, 0);
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][i][s], b[1][j][s], acc[1][i][j], 0, 0, 0);
+            acc[2][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[i], sb[j], acc[2][i][j], 0, 0, 0);
           }
+      }
     }
   };
 
@@ -955,8 +1002,8 @@ __global__ void __launch_bounds__(256)
       for (int r = 0; r < 4; ++r) {
         const int m = wm * WTM + i * 16 + lk + 4 * r;
         const int n = wn * WTN + j * 16 + lr;
-        Cr[n * CP + m] = acc[0][i][j][r];
-        Ci[n * CP + m] = acc[1][i][j][r];
+        Cr[n * CP + m] = acc[0][i][j][r] - acc[1][i][j][r];
+        Ci[n * CP + m] = acc[2][i][j][r] - acc[0][i][j][r] - acc[1][i][j][r];
       }
   __syncthreads();
   const int bq0 = c0 / DS;
