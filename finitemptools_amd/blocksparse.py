@@ -626,16 +626,61 @@ def _gemm1_env_vec(Lay, xl, t1_off):
     return tasks, t1
 
 
-def plan_heff2(xl, Lay, Ray, W1, W2):
+def _gemm1_used(Lay, xl, used):
+    """Stage 1 of the two-site H_eff restricted to the input slots the MPO
+    pair reads: T1[(qa, w)] = L[(qa, w)] * (the columns of x_group[qa - q(w)]
+    whose block (s1, s2) has (w, s1, s2) in ``used``), stored compactly (the
+    used blocks side by side in group order).  One task per run of adjacent
+    used blocks.  Returns (tasks, t1_off[(qa, w)], t1_col[(qa, w, p)], size)."""
+    tasks, t1_off, t1_col, t1 = [], {}, {}, 0
+    S = Lay.space
+    for qa, Da in S.sectors:
+        for w in range(Lay.w):
+            if (qa, w) not in Lay.off:
+                continue
+            qk = qsub(qa, Lay.qw[w])
+            if qk not in xl.group:
+                continue
+            g0, _ = xl.group[qk]
+            Dk = S.dim[qk]
+            runs, c = [], 0
+            for b in xl.by_left[qk]:
+                if (w,) + tuple(b.p) not in used:
+                    continue
+                t1_col[(qa, w, b.p)] = c
+                if runs and runs[-1][0] + runs[-1][1] == b.colstart:
+                    runs[-1][1] += b.cols
+                else:
+                    runs.append([b.colstart, b.cols, c])
+                c += b.cols
+            if c == 0:
+                continue
+            t1_off[(qa, w)] = t1
+            for x0, n, c0 in runs:
+                tasks.append((Da, n, t1 + Da * c0, Da, [(Lay.off[(qa, w)], g0 + Dk * x0, Dk, Da, Dk)]))
+            t1 += Da * c
+    return tasks, t1_off, t1_col, t1
+
+
+def plan_heff2(xl, Lay, Ray, W1, W2, skip_zeros=True):
     """Tables of y = ac2_prime(x, W1, W2, L, R) (derivatives.jl:20-22) for an
     x of layout ``xl`` (two physical legs).  Buffers: X = x, Y = y,
     P1 = L (layout Lay over xl.lspace), P2 = R (layout Ray over xl.rspace).
 
-    Stage 1: T1[(qa,w)] (D_qa x ncols of x's group qa-q(w)) = L x.
-    Stage 2: for every right sector qb a matrix T2_qb with rows the blocks
-             (qa, s1', s2') of y ending in qb and columns (w', b') in the order
-             of R's sector-qb matrix:  T2 = sum Mt-coefficients * T1 blocks.
-    Stage 3: y block (qa, s1', s2', qb) = T2_qb[rows of the block] R_qb^T."""
+    Stage 1: T1[(qa,w)] = L x over the blocks (s1, s2) of x's group qa-q(w)
+             that the MPO pair reads from channel w.
+    Stage 2: for every right sector qb, its output blocks (qa, s1', s2') are
+             grouped by the set of channels w' that receive MPO entries; per
+             group a matrix T2 with rows the group's blocks and columns the
+             (w', b') of those channels (in the order of R's sector-qb
+             matrix):  T2 = sum Mt-coefficients * T1 blocks.
+    Stage 3: y block (qa, s1', s2', qb) = T2[rows of the block] R_qb^T over
+             the group's channels.
+    The slots and channels left out hold only structural zeros of the MPO
+    pair (zero coefficients inside charge-allowed blocks), which the
+    reference's graded contraction multiplies; ``skip_zeros=False`` keeps
+    them (every (w, s1, s2) column of T1, every channel of R) and gives the
+    flop count of that contraction."""
     if xl.nphys != 2:
         raise _lib.DimensionMismatch(2, "plan_heff2 needs a two-site layout")
     if Lay.space != xl.lspace or Ray.space != xl.rspace:
@@ -644,40 +689,28 @@ def plan_heff2(xl, Lay, Ray, W1, W2):
     _, _, wr, _ = W2.shape
     if wl != Lay.w or wr != Ray.w or d != xl.d:
         raise _lib.DimensionMismatch(2, "MPO dimensions do not match the environments")
-    Sl, Sr, phys = xl.lspace, xl.rspace, xl.phys
-    t1_off = {}
-    g1, t1 = _gemm1_env_vec(Lay, xl, t1_off)
-    # T2 layout
-    t2_off, rowoff, R_rows, t2 = {}, {}, {}, 0
+    Sr = xl.rspace
+    entries = mode_entries_two(W1, W2)
+    if skip_zeros:
+        used = {(w, s1, s2) for lst in entries.values() for w, s1, s2, _ in lst}
+    else:
+        used = {(w, s1, s2) for w in range(wl) for s1 in range(d) for s2 in range(d)}
+    g1, t1_off, t1_col, t1 = _gemm1_used(Lay, xl, used)
+    dests, g3, t2 = [], [], 0
     for qb, Db in Sr.sectors:
         if qb not in xl.by_right:
             continue
         if qb not in Ray.group:
             raise ValueError(f"right environment has no block for sector {qb}")
-        r = 0
-        for b in xl.by_right[qb]:
-            rowoff[(b.qa, b.p)] = r
-            r += b.rows
-        R_rows[qb] = r
-        t2_off[qb] = t2
-        t2 += r * Ray.group[qb][1]
-    entries = mode_entries_two(W1, W2)
-    dests = []
-    for qb, Db in Sr.sectors:
-        if qb not in t2_off:
-            continue
-        Rq = R_rows[qb]
+        chans = [w2 for w2 in range(wr) if (qb, w2) in Ray.off]
+        groups, order = {}, []
         for b in xl.by_right[qb]:
             qa, Da = b.qa, b.rows
-            s1o, s2o = b.p
-            for w2 in range(wr):
-                if (qb, w2) not in Ray.off:
-                    continue
+            per = {}
+            for w2 in chans:
                 qbk = qsub(qb, Ray.qw[w2])
-                cols = Sr.dim[qbk]
-                dst = t2_off[qb] + rowoff[(qa, b.p)] + Rq * Ray.colstart[(qb, w2)]
                 cl = []
-                for w, s1, s2, c in entries.get((s1o, s2o, w2), ()):
+                for w, s1, s2, c in entries.get((b.p[0], b.p[1], w2), ()):
                     if (qa, w) not in t1_off:
                         continue
                     qk = qsub(qa, Lay.qw[w])
@@ -686,19 +719,54 @@ def plan_heff2(xl, Lay, Ray, W1, W2):
                         continue
                     if src.qb != qbk:
                         raise ValueError("MPO entry does not conserve the charges")
-                    cl.append((t1_off[(qa, w)] + Da * src.colstart, Da, c))
-                dests.append((dst, Da, cols, Rq, cl))
-    g3 = []
-    for b in xl.blocks:
-        qb = b.qb
-        g0, K = Ray.group[qb]
-        g3.append((b.rows, b.cols, b.off, b.rows,
-                   [(t2_off[qb] + rowoff[(b.qa, b.p)], g0, K, R_rows[qb], b.cols)]))
+                    cl.append((t1_off[(qa, w)] + Da * t1_col[(qa, w, (s1, s2))], Da, c))
+                if cl or not skip_zeros:
+                    per[w2] = cl
+            if not per:
+                per = {chans[0]: []}   # a block nothing reaches: one zero channel
+            pat = tuple(sorted(per))
+            if pat not in groups:
+                groups[pat] = []
+                order.append(pat)
+            groups[pat].append((b, per))
+        g0R = Ray.group[qb][0]
+        for pat in order:
+            members = groups[pat]
+            Rg = sum(b.rows for b, _ in members)
+            col, c = {}, 0
+            for w2 in pat:
+                col[w2] = c
+                c += Sr.dim[qsub(qb, Ray.qw[w2])]
+            base, t2 = t2, t2 + Rg * c
+            # K-segments of stage 3: runs of channels adjacent in R's columns
+            runs = []
+            for w2 in pat:
+                K = Sr.dim[qsub(qb, Ray.qw[w2])]
+                rc = Ray.colstart[(qb, w2)]
+                if runs and runs[-1][1] + runs[-1][2] == rc:
+                    runs[-1][2] += K
+                else:
+                    runs.append([col[w2], rc, K])
+            r = 0
+            for b, per in members:
+                for w2 in pat:
+                    dests.append((base + r + Rg * col[w2], b.rows, Sr.dim[qsub(qb, Ray.qw[w2])],
+                                  Rg, per[w2]))
+                g3.append((b.rows, b.cols, b.off, b.rows,
+                           [(base + r + Rg * c0, g0R + Db * rc, K, Rg, b.cols) for c0, rc, K in runs]))
+                r += b.rows
     tab = PlanTables([xl.size, xl.size, Lay.size, Ray.size, t1, t2])
     tab.add_gemm(P1, X, T1, True, True, g1)
     tab.add_mode(T1, T2, dests)
     tab.add_gemm(T2, P2, Y, True, False, g3)
     return tab
+
+
+def heff2_reference_flops(xl, Lay, Ray, W1, W2):
+    """Flops of the two-site H_eff as the reference's graded contraction does
+    them (every charge-allowed block, MPO zeros included): the count of
+    plan_heff2(..., skip_zeros=False)."""
+    return plan_heff2(xl, Lay, Ray, W1, W2, skip_zeros=False).flops()
 
 
 def plan_updateleft(al, Lay, W, Lay_new):

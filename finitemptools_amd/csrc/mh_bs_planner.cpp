@@ -459,41 +459,78 @@ int mh_bs_plan_heff2(mh_handle h, int prec, const mh_bs_sectors *left,
                 c += wat(W1, wl, d, wm, w, s1o, m, s1) * wat(W2, wm, d, wr, m, s2o, wo, s2);
               if (c != 0.0) ent[(size_t)s1o + d * (s2o + (size_t)d * wo)].push_back({w, s1, s2, c});
             }
+  // input slots (w, s1, s2) that some entry reads
+  std::vector<char> used((size_t)wl * d * d, 0);
+  for (const auto &lst : ent)
+    for (const Ent &e : lst) used[(size_t)e.w + wl * (e.s1 + (size_t)d * e.s2)] = 1;
   std::vector<GemmTab> g(2);
   std::vector<ModeTab> md(1);
-  std::vector<int64_t> t1off;
-  const int64_t t1 = gemm1_env_vec(Le, x, g[0], t1off, nq);
-  // T2 per right sector: rows = the blocks ending there, columns = R's sector matrix.
+  // Stage 1 over the used blocks of each group, stored compactly; one task per
+  // run of adjacent used blocks.  t1col[block] = compact column for (qa, w).
+  const int nsl = (int)Sl.q.size();
+  std::vector<int64_t> t1off((size_t)nsl * wl, -1);
+  std::vector<std::vector<int>> t1col((size_t)nsl * wl);
+  int64_t t1 = 0;
+  for (int qa = 0; qa < nsl; ++qa) {
+    const int Da = Sl.dim[qa];
+    for (int w = 0; w < wl; ++w) {
+      const int64_t eo = Le.off[(size_t)qa * wl + w];
+      if (eo < 0) continue;
+      const int qk = Sl.find(qsub(Sl.q[qa], Le.qw[w], nq));
+      if (qk < 0 || x.gcols[qk] == 0) continue;
+      const int Dk = Sl.dim[qk];
+      const auto &grp = x.by_left[qk];
+      std::vector<int> &cols = t1col[(size_t)qa * wl + w];
+      cols.assign(grp.size(), -1);
+      struct Run { int x0, n, c0; };
+      std::vector<Run> runs;
+      int c = 0;
+      for (size_t j = 0; j < grp.size(); ++j) {
+        const Blk &b = x.blocks[grp[j]];
+        if (!used[(size_t)w + wl * (b.p0 + (size_t)d * b.p1)]) continue;
+        cols[j] = c;
+        if (!runs.empty() && runs.back().x0 + runs.back().n == b.colstart)
+          runs.back().n += b.cols;
+        else
+          runs.push_back({b.colstart, b.cols, c});
+        c += b.cols;
+      }
+      if (c == 0) continue;
+      t1off[(size_t)qa * wl + w] = t1;
+      for (const Run &r : runs) {
+        g[0].task(Da, r.n, t1 + (int64_t)Da * r.c0, Da);
+        g[0].seg(eo, x.g0[qk] + (int64_t)Dk * r.x0, Dk, Da, Dk);
+      }
+      t1 += (int64_t)Da * c;
+    }
+  }
+  // position of each block inside its left group (for t1col)
+  std::vector<int> ingrp(x.blocks.size(), 0);
+  for (const auto &grp : x.by_left)
+    for (size_t j = 0; j < grp.size(); ++j) ingrp[grp[j]] = (int)j;
+  // Stages 2 and 3 per right sector: output blocks grouped by the channels w'
+  // that receive entries (first-appearance order), T2 per group compact.
   const int ns = (int)Sr.q.size();
-  std::vector<int64_t> t2off(ns, -1);
-  std::vector<int> Rrows(ns, 0);
-  std::vector<int> rowoff(x.blocks.size(), 0);
   int64_t t2 = 0;
+  struct Contrib { int64_t off; double c; };
   for (int qb = 0; qb < ns; ++qb) {
     if (x.by_right[qb].empty()) continue;
     if (Re.K[qb] == 0) return fail(h, MH_ERR_ARG, "right environment has no block for sector %d", qb);
-    int r = 0;
-    for (int id : x.by_right[qb]) {
-      rowoff[id] = r;
-      r += x.blocks[id].rows;
-    }
-    Rrows[qb] = r;
-    t2off[qb] = t2;
-    t2 += (int64_t)r * Re.K[qb];
-  }
-  for (int qb = 0; qb < ns; ++qb) {
-    if (t2off[qb] < 0) continue;
-    const int Rq = Rrows[qb];
-    for (int id : x.by_right[qb]) {
-      const Blk &b = x.blocks[id];
+    std::vector<int> chans;
+    for (int w2 = 0; w2 < wr; ++w2)
+      if (Re.off[(size_t)qb * wr + w2] >= 0) chans.push_back(w2);
+    const int Db = Sr.dim[qb];
+    // per output block: channel mask and the contribution lists per channel
+    const auto &rows = x.by_right[qb];
+    std::vector<std::vector<char>> mask(rows.size(), std::vector<char>(wr, 0));
+    std::vector<std::vector<std::vector<Contrib>>> per(rows.size(),
+                                                       std::vector<std::vector<Contrib>>(wr));
+    for (size_t i = 0; i < rows.size(); ++i) {
+      const Blk &b = x.blocks[rows[i]];
       const int qa = b.qa, Da = b.rows;
-      for (int w2 = 0; w2 < wr; ++w2) {
-        const int64_t ro = Re.off[(size_t)qb * wr + w2];
-        if (ro < 0) continue;
+      for (int w2 : chans) {
         const int qbk = Sr.find(qsub(Sr.q[qb], Re.qw[w2], nq));
-        const int cols = Sr.dim[qbk];
-        md[0].dest(t2off[qb] + rowoff[id] + (int64_t)Rq * Re.colstart[(size_t)qb * wr + w2], Da,
-                   cols, Rq);
+        auto &cl = per[i][w2];
         for (const Ent &e : ent[(size_t)b.p0 + d * (b.p1 + (size_t)d * w2)]) {
           const int64_t to = t1off[(size_t)qa * wl + e.w];
           if (to < 0) continue;
@@ -501,15 +538,57 @@ int mh_bs_plan_heff2(mh_handle h, int prec, const mh_bs_sectors *left,
           const int src = x.find(qk, e.s1, e.s2);
           if (src < 0) continue;
           if (x.blocks[src].qb != qbk) return fail(h, MH_ERR_ARG, "MPO entry does not conserve the charges");
-          md[0].contrib(to + (int64_t)Da * x.blocks[src].colstart, Da, e.c);
+          const int cc = t1col[(size_t)qa * wl + e.w][ingrp[src]];
+          cl.push_back({to + (int64_t)Da * cc, e.c});
         }
+        if (!cl.empty()) mask[i][w2] = 1;
+      }
+      if (std::find(mask[i].begin(), mask[i].end(), 1) == mask[i].end())
+        mask[i][chans[0]] = 1;  // a block nothing reaches: one zero channel
+    }
+    std::vector<std::vector<char>> order;
+    for (const auto &m : mask)
+      if (std::find(order.begin(), order.end(), m) == order.end()) order.push_back(m);
+    for (const auto &pat : order) {
+      int Rg = 0;
+      for (size_t i = 0; i < rows.size(); ++i)
+        if (mask[i] == pat) Rg += x.blocks[rows[i]].rows;
+      std::vector<int> col(wr, -1);
+      int c = 0;
+      for (int w2 = 0; w2 < wr; ++w2) {
+        if (!pat[w2]) continue;
+        col[w2] = c;
+        c += Sr.dim[Sr.find(qsub(Sr.q[qb], Re.qw[w2], nq))];
+      }
+      const int64_t base = t2;
+      t2 += (int64_t)Rg * c;
+      struct Run { int c0, rc, K; };
+      std::vector<Run> runs;
+      for (int w2 = 0; w2 < wr; ++w2) {
+        if (!pat[w2]) continue;
+        const int K = Sr.dim[Sr.find(qsub(Sr.q[qb], Re.qw[w2], nq))];
+        const int rc = Re.colstart[(size_t)qb * wr + w2];
+        if (!runs.empty() && runs.back().rc + runs.back().K == rc)
+          runs.back().K += K;
+        else
+          runs.push_back({col[w2], rc, K});
+      }
+      int r = 0;
+      for (size_t i = 0; i < rows.size(); ++i) {
+        if (mask[i] != pat) continue;
+        const Blk &b = x.blocks[rows[i]];
+        for (int w2 = 0; w2 < wr; ++w2) {
+          if (!pat[w2]) continue;
+          md[0].dest(base + r + (int64_t)Rg * col[w2], b.rows,
+                     Sr.dim[Sr.find(qsub(Sr.q[qb], Re.qw[w2], nq))], Rg);
+          for (const Contrib &ct : per[i][w2]) md[0].contrib(ct.off, b.rows, ct.c);
+        }
+        g[1].task(b.rows, b.cols, b.off, b.rows);
+        for (const Run &ru : runs)
+          g[1].seg(base + r + (int64_t)Rg * ru.c0, Re.g0[qb] + (int64_t)Db * ru.rc, ru.K, Rg, b.cols);
+        r += b.rows;
       }
     }
-  }
-  for (size_t id = 0; id < x.blocks.size(); ++id) {
-    const Blk &b = x.blocks[id];
-    g[1].task(b.rows, b.cols, b.off, b.rows);
-    g[1].seg(t2off[b.qb] + rowoff[id], Re.g0[b.qb], Re.K[b.qb], Rrows[b.qb], b.cols);
   }
   const int64_t sizes[6] = {x.size, x.size, Le.size, Re.size, t1, t2};
   return finish(h, prec, sizes, {0, 1, 0}, g, md,
