@@ -57,9 +57,10 @@ struct Stage {
 };
 
 // One kernel for real (CPLX = false) and complex FP64.  Complex operands are
-// split into re/im LDS planes at the LDS store.  Wave tiles of at most four
-// 16x16 MFMA tiles (M3) run three real MFMAs per 16x16x4 step (Gauss's
-// three-multiplication form, as in zgemm3m):
+// split into re/im LDS planes at the LDS store.  Wave tiles small enough for
+// a third accumulator plane (M3: at most two 16-row tiles, TM * TN * BK <= 64)
+// run three real MFMAs per 16x16x4 step (Gauss's three-multiplication form,
+// as in zgemm3m):
 // P1 += Ar Br, P2 += Ai Bi, P3 += (Ar + Ai)(Br + Bi), and after the K loop
 // Re C = P1 - P2, Im C = P3 - P1 - P2.  The error is normwise of the same
 // order as the four-product form (Higham, "Stability of a method for
@@ -67,7 +68,7 @@ struct Stage {
 // SIAM J. Matrix Anal. Appl. 13 (1992)); the imaginary part's bound carries
 // (|Ar| + |Ai|)(|Br| + |Bi|) in place of |Ar||Bi| + |Ai||Br|.  Larger wave
 // tiles keep the four-product form (Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br),
-// whose two accumulator planes fit their registers.
+// whose two accumulator planes fit their registers without spilling.
 //
 // Split-K: blockIdx.z selects the K range [z*kchunk, (z+1)*kchunk).  With
 // gridDim.z == 2 the partial tiles are added with FP64 atomics onto a C that
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(64 * NW)
     }
   };
 
-  constexpr bool M3 = CPLX && TM * TN <= 4;
+  constexpr bool M3 = CPLX && TM <= 2 && TM * TN * BK <= 64;
   constexpr int NQ = M3 ? 3 : NP;  // accumulator planes (P1, P2, P3 for M3)
   d4_t acc[NQ][TM][TN];
 #pragma unroll

@@ -415,3 +415,49 @@ def test_coupled_form_matches_the_two_site_matrices(gpu):
             print(f"coupled round trip nphys={lay.nphys} ncod={ncod}", flush=True)
             back = bs.from_coupled(lay, ncod, bs.to_coupled(t, ncod))
             assert torch.equal(back.data, t.data)
+
+
+BONDS = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "hubbard_L64_D2048_bonds.json")))["bonds"]
+
+
+@pytest.mark.parametrize("site", sorted(BONDS, key=int))
+def test_d2048_bond_plan_matches_per_sector_oracle(gpu, site):
+    """The FP64 grouped plan at the sector sizes of a D = 2048 Hubbard ramp
+    (L = 64, U1 x U1; middle bond 47 sectors up to 205, multi-tile blocks)
+    against the independent per-coupled-sector restatement
+    (oracle/blocksparse.py, derivatives.jl:21 on graded tensors): 1e-12."""
+    from oracle import blocksparse as OB
+    b = BONDS[site]
+
+    def sp(lst):
+        return bs.Space([(tuple(q), int(n)) for q, n in lst])
+
+    phys = bm.hubbard_charges()
+    xl = bs.BlockLayout(sp(b["x_l"]), phys, 2, sp(b["x_r"]))
+    Lay = bs.EnvLayout(sp(b["L_space"]), [tuple(q) for q in b["L_qw"]])
+    Ray = bs.EnvLayout(sp(b["R_space"]), [tuple(q) for q in b["R_qw"]])
+    mpo = models.hubbard_mpo(64, 1.0, 4.0)
+    W1, W2 = mpo[int(site)], mpo[int(site) + 1]
+    rng = np.random.default_rng(int(site))
+    Le = rng.standard_normal(Lay.size) / np.sqrt(2048 * 6)
+    Re = rng.standard_normal(Ray.size) / np.sqrt(2048 * 6)
+    x = rng.standard_normal(xl.size)
+    print(f"D=2048 bond {site}: {len(xl.lspace.sectors)} sectors, vector {xl.size}", flush=True)
+    plan = bs.make_plan("heff2", xl, Lay, Ray, W1, W2, "f64")
+    y = torch.empty(xl.size, dtype=torch.float64, device="cuda")
+    plan.apply(dev(x), dev(Le), dev(Re), y)
+    got = y.cpu().numpy()
+
+    def env_entries(lay):
+        return [((q, w), off, lay.space.dim[q], lay.space.dim[bs.qsub(q, lay.qw[w])])
+                for (q, w), off in lay.off.items()]
+
+    xe = [((blk.qa, blk.p[0], blk.p[1], blk.qb), blk.off, blk.rows, blk.cols)
+          for blk in xl.blocks]
+    yb = OB.ac2_prime(OB.blocks_from_flat(x, xe), W1, W2,
+                      OB.blocks_from_flat(Le, env_entries(Lay)),
+                      OB.blocks_from_flat(Re, env_entries(Ray)), phys, Lay.qw, Ray.qw,
+                      dict(xl.lspace.sectors), dict(xl.rspace.sectors))
+    ref = OB.flat_from_blocks(yb, xe, xl.size)
+    assert rel(got, ref) < 1e-12
