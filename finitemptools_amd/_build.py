@@ -15,6 +15,7 @@ SOURCES = [
     "mh_blocksparse.hip",
     "mh_adjoint.hip",
     "mh_eigen.hip",
+    "mh_tridc.hip",
     "mh_heev.hip",
     "mh_bs_planner.cpp",
     "mh_schur.cpp",

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <thread>
 #include <vector>
 
@@ -221,6 +222,225 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_group(SyTable tab, double 
   }
 }
 
+// The same reduction with every workgroup's columns resident in LDS for the
+// whole launch (orders up to SL_MAXN).  The step is the one above; what
+// changes is where the data lives:
+//   - a workgroup keeps its columns (cyclic, j = g + q G) in LDS, so the lazy
+//     rank-2 update and the dot products of a step touch no global memory;
+//   - wavefront 0 forms w_{i-1}, the column i of A^(i) and v_i in registers
+//     (lane l holds rows l + 64 k) and hands them to the other waves through
+//     LDS; only p and the next column travel through global memory;
+//   - the workgroups of a matrix meet at a flag barrier (each writes its own
+//     counter, wavefront 0 polls all of them), not at one contended atomic.
+constexpr int SL_KR = 16;                 // rows per lane
+constexpr int SL_MAXN = 64 * SL_KR;       // largest order of the LDS form
+
+struct SlTask {
+  int64_t a_off, v_off, x_off;  // matrix, D/E/tau, exchange rows
+  int n, G, g0, ncl;            // order, workgroups, first workgroup, LDS columns each
+  int f_off, pad;               // first barrier flag
+};
+
+__device__ inline double wave_sum64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// value at row idx of a lane-distributed vector (lane idx % 64, slot idx / 64)
+__device__ inline double row_value(const double (&x)[SL_KR], int idx) {
+  const int kk = idx >> 6;
+  double t = 0.0;
+#pragma unroll
+  for (int k = 0; k < SL_KR; ++k)
+    if (k == kk) t = x[k];
+  return __shfl(t, idx & 63);
+}
+
+// Barrier of the G workgroups of one matrix: workgroup g raises its own flag
+// to `step`; wavefront 0 polls every flag (lanes in parallel).  Bounded like
+// group_barrier: past the spin limit the error flag is raised, and a raised
+// error flag ends every later wait at once.
+__device__ inline void flag_barrier(unsigned *flags, int G, int g, unsigned step, int *err) {
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    if (threadIdx.x == 0) {
+      __threadfence();
+      __hip_atomic_store(flags + g, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    unsigned spins = 0;
+    for (;;) {
+      bool ok = true;
+      for (int q = threadIdx.x; q < G; q += 64)
+        ok = ok && (__hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= step);
+      if (__ballot(!ok) == 0) break;
+      const int e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__ballot(e != 0) != 0) break;
+      if (++spins > SY_SPIN_LIMIT) {
+        if (threadIdx.x == 0) atomicOr(err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restrict__ tabv, int nt,
+                                                          double *__restrict__ A,
+                                                          double *__restrict__ D,
+                                                          double *__restrict__ E,
+                                                          double *__restrict__ tau,
+                                                          double *__restrict__ X, int64_t xhalf,
+                                                          unsigned *__restrict__ flags,
+                                                          int *__restrict__ err) {
+  extern __shared__ double sm[];
+  __shared__ double scal[2];
+  int lo = 0, hi = nt - 1;
+  while (lo < hi) {
+    const int md = (lo + hi + 1) >> 1;
+    if (tabv[md].g0 <= (int)blockIdx.x) lo = md; else hi = md - 1;
+  }
+  const SlTask tk = tabv[lo];
+  const int n = tk.n, G = tk.G, g = (int)blockIdx.x - tk.g0, ncl = tk.ncl;
+  double *a = A + tk.a_off;
+  double *cols = sm;                         // ncl columns of n rows
+  double *sV = sm + (size_t)ncl * n;         // v_i
+  double *sW = sV + n;                       // w_{i-1}
+  double *sP = sW + n;                       // v_{i-1}
+  double *const P0 = X + tk.x_off, *const P1 = P0 + xhalf;
+  double *const B0 = P1 + xhalf, *const B1 = B0 + xhalf;
+  unsigned *fl = flags + tk.f_off;
+  unsigned nbar = 0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+
+  for (int q = 0; q < ncl; ++q) {
+    const int j = g + q * G;
+    if (j >= n) break;
+    for (int r = threadIdx.x; r < n; r += SY_THREADS) cols[(size_t)q * n + r] = a[r + (int64_t)j * n];
+  }
+  if (g == 0)
+    for (int r = threadIdx.x; r < n; r += SY_THREADS) xstore(B0 + r, a[r]);
+  flag_barrier(fl, G, g, ++nbar, err);
+
+  double vp[SL_KR], vv[SL_KR];  // wavefront 0: v_{i-1}, v_i
+#pragma unroll
+  for (int k = 0; k < SL_KR; ++k) vp[k] = vv[k] = 0.0;
+  double tau_prev = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const bool last = (i == n - 1);
+    if (wv == 0) {
+      const double *bc = (i & 1) ? B1 : B0;
+      double wp[SL_KR];
+      if (i > 0) {
+        const double *pp = ((i - 1) & 1) ? P1 : P0;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < SL_KR; ++k) {
+          const int r = lane + 64 * k;
+          wp[k] = (r >= i && r < n) ? xload(pp + r) : 0.0;
+          s += wp[k] * vp[k];
+        }
+        const double hh = 0.5 * tau_prev * wave_sum64(s);
+#pragma unroll
+        for (int k = 0; k < SL_KR; ++k) wp[k] -= hh * vp[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < SL_KR; ++k) wp[k] = 0.0;
+      }
+      const double vpi = row_value(vp, i), wpi = row_value(wp, i);
+      double x[SL_KR];
+      double s2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < SL_KR; ++k) {
+        const int r = lane + 64 * k;
+        x[k] = (r >= i && r < n) ? xload(bc + r) - vp[k] * wpi - wp[k] * vpi : 0.0;
+        if (r >= i + 2) s2 += x[k] * x[k];
+      }
+      const double xn2 = wave_sum64(s2);
+      const double di = row_value(x, i);
+      double beta = 0.0, tu = 0.0;
+      if (!last) {
+        const double alpha = row_value(x, i + 1);
+        double scale = 1.0;
+        beta = alpha;
+        if (xn2 > 0.0) {
+          beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+          tu = (beta - alpha) / beta;
+          scale = 1.0 / (alpha - beta);
+        }
+        const bool own = (i % G == g);
+#pragma unroll
+        for (int k = 0; k < SL_KR; ++k) {
+          const int r = lane + 64 * k;
+          vv[k] = (r >= i + 2) ? x[k] * scale : ((r == i + 1) ? 1.0 : 0.0);
+          if (r < n) {
+            sV[r] = vv[k];
+            sW[r] = wp[k];
+            sP[r] = vp[k];
+            if (own && r >= i + 2) a[r + (int64_t)i * n] = vv[k];
+          }
+        }
+        if (own && lane == 0) {
+          a[i + (int64_t)i * n] = di;
+          a[i + 1 + (int64_t)i * n] = beta;
+        }
+      } else if (i % G == g && lane == 0) {
+        a[i + (int64_t)i * n] = di;
+      }
+      if (lane == 0) {
+        if (g == 0) {
+          D[tk.v_off + i] = di;
+          E[tk.v_off + i] = beta;
+          tau[tk.v_off + i] = tu;
+        }
+        scal[0] = tu;
+      }
+    }
+    __syncthreads();
+    if (last) break;
+    const double tu = scal[0];
+    {
+      double vpB[SL_KR], wpB[SL_KR], vcB[SL_KR];
+#pragma unroll
+      for (int k = 0; k < SL_KR; ++k) {
+        const int r = lane + 64 * k;
+        const bool in = r < n;
+        vpB[k] = in ? sP[r] : 0.0;
+        wpB[k] = in ? sW[r] : 0.0;
+        vcB[k] = in ? sV[r] : 0.0;
+      }
+      double *pc = (i & 1) ? P1 : P0;
+      double *bn = ((i + 1) & 1) ? B1 : B0;
+      const int q0 = std::max(0, (i + 1 - g + G - 1) / G);
+      for (int q = q0 + wv; q < ncl; q += SY_WAVES) {
+        const int j = g + q * G;
+        if (j >= n) break;
+        const double vpj = sP[j], wpj = sW[j];
+        const bool pub = (j == i + 1);
+        double *col = cols + (size_t)q * n;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < SL_KR; ++k) {
+          const int r = lane + 64 * k;
+          if (r >= i + 1 && r < n) {
+            const double x = col[r] - vpB[k] * wpj - wpB[k] * vpj;
+            col[r] = x;
+            if (pub) xstore(bn + r, x);
+            acc += x * vcB[k];
+          }
+        }
+        acc = wave_sum64(acc);
+        if (lane == 0) xstore(pc + j, tu * acc);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SL_KR; ++k) vp[k] = vv[k];
+    tau_prev = tu;
+    flag_barrier(fl, G, g, ++nbar, err);
+  }
+}
+
 // z <- Q z for selected columns, Q = H_0 H_1 ... H_{n-2}; one wavefront per
 // column, the column held in LDS, reflectors read from A in reverse order.
 constexpr int OR_THREADS = 256;
@@ -340,6 +560,90 @@ void eig_release(mh_handle h) {
   h->eig_bytes = 0;
 }
 
+// Plan and launch k_sytrd_lds: every matrix gets G workgroups holding
+// ceil(n / G) columns each, within an LDS budget per workgroup; the budget
+// is the largest (fewest workgroups, cheapest barriers) for which every
+// workgroup of the batch is resident at once.  MH_ERR_UNSUPPORTED when no
+// budget fits (the caller then takes the global-memory form).
+int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const int64_t *v_off,
+              double *A, double *D, double *E, double *tau) {
+  int ncu = 0, lds_cu = 0;
+  MH_HIP(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+  MH_HIP(h, hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                  h->device));
+  if (lds_cu <= 0) lds_cu = 64 * 1024;
+  std::vector<SlTask> tab(nmat);
+  int64_t xsum = 0;
+  for (int m = 0; m < nmat; ++m) xsum += n[m];
+  // largest matrices first: their workgroups start first
+  std::vector<int> order(nmat);
+  for (int m = 0; m < nmat; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+  size_t lds = 0;
+  int gtot = 0;
+  bool ok = false;
+  for (int k = 1; k <= 4 && !ok; ++k) {
+    const int64_t budget = (int64_t)(lds_cu / k) - 2048;  // bytes for columns + 3 vectors
+    gtot = 0;
+    lds = 0;
+    int64_t xo = 0;
+    bool fits = true;
+    for (int pi = 0; pi < nmat; ++pi) {
+      const int m = order[pi];
+      const int nm = n[m];
+      const int64_t capc = budget / 8 / nm - 3;  // columns that fit
+      if (capc < 1) {
+        fits = false;
+        break;
+      }
+      int G = (int)((nm + capc - 1) / capc);
+      const int ncl = (nm + G - 1) / G;
+      SlTask &t = tab[pi];
+      t.a_off = a_off[m];
+      t.v_off = v_off[m];
+      t.x_off = xo;
+      t.n = nm;
+      t.G = G;
+      t.g0 = gtot;
+      t.ncl = ncl;
+      t.f_off = gtot;
+      t.pad = 0;
+      xo += nm;
+      gtot += G;
+      lds = std::max(lds, (size_t)(ncl + 3) * nm * sizeof(double));
+    }
+    if (!fits) continue;
+    int per_cu = 0;
+    MH_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_sytrd_lds,
+                                                            SY_THREADS, lds));
+    ok = per_cu >= 1 && gtot <= per_cu * ncu;
+  }
+  if (!ok) return MH_ERR_UNSUPPORTED;
+  // scratch: 4 exchange halves of xsum doubles, the flags, the task table,
+  // the error flag
+  const size_t xbytes = (size_t)xsum * 4 * sizeof(double);
+  const size_t fbytes = ((size_t)gtot * sizeof(unsigned) + 255) & ~size_t(255);
+  const size_t tbytes = ((size_t)nmat * sizeof(SlTask) + 255) & ~size_t(255);
+  const size_t xalign = (xbytes + 255) & ~size_t(255);
+  MH_TRY(ensure_eig_scratch(h, xalign + fbytes + tbytes + 256));
+  char *base = (char *)h->eig;
+  double *X = (double *)base;
+  unsigned *flags = (unsigned *)(base + xalign);
+  SlTask *dtab = (SlTask *)(base + xalign + fbytes);
+  int *err = (int *)(base + xalign + fbytes + tbytes);
+  h->eig_err = err;
+  MH_HIP(h, hipMemsetAsync(err, 0, sizeof(int), h->stream));
+  MH_HIP(h, hipMemsetAsync(flags, 0, (size_t)gtot * sizeof(unsigned), h->stream));
+  MH_HIP(h, hipMemcpyAsync(dtab, tab.data(), (size_t)nmat * sizeof(SlTask), hipMemcpyHostToDevice,
+                           h->stream));
+  int nt = nmat;
+  int64_t xhalf = xsum;
+  void *args[] = {&dtab, &nt, &A, &D, &E, &tau, &X, &xhalf, &flags, &err};
+  MH_HIP(h, hipLaunchCooperativeKernel((const void *)k_sytrd_lds, dim3(gtot), dim3(SY_THREADS),
+                                       args, (unsigned)lds, h->stream));
+  return MH_OK;
+}
+
 }  // namespace mh
 
 using namespace mh;
@@ -363,6 +667,10 @@ extern "C" int mh_sytrd_batched(mh_handle h, int nmat, const int *n, const int64
   }
   MH_HIP(h, hipSetDevice(h->device));
   EigState *st = state(h);
+  if (nmax <= SL_MAXN && !getenv("MH_SYTRD_GLOBAL")) {
+    int status = sytrd_lds(h, nmat, n, a_off, v_off, A, D, E, tau);
+    if (status != MH_ERR_UNSUPPORTED) return status;  // else: does not fit, global form
+  }
   const size_t lds = (size_t)3 * nmax * sizeof(double);
   if (st->coresident_lds != (int)lds) {
     int per_cu = 0, ncu = 0;
@@ -538,6 +846,13 @@ extern "C" int mh_eig_check(mh_handle h, int nmat) {
     int err = 0;
     MH_HIP(h, hipMemcpy(&err, h->eig_err, sizeof(int), hipMemcpyDeviceToHost));
     if (err) return fail(h, MH_ERR_HIP, "mh_sytrd_batched: a group barrier timed out");
+  }
+  if (int *dce = dc_error_flag(h)) {
+    int err = 0;
+    MH_HIP(h, hipMemcpy(&err, dce, sizeof(int), hipMemcpyDeviceToHost));
+    if (err)
+      return fail(h, MH_ERR_NOCONV,
+                  "mh_stedc_dc_batched: the QL iteration of a leaf did not converge");
   }
   if (nmat > 0 && st->info) {
     std::vector<int> info(std::min(nmat, st->info_len));

@@ -66,6 +66,9 @@ struct mh_context {
   size_t eig_bytes = 0;
   int *eig_err = nullptr;
   void *eig_state = nullptr;
+  // Divide-and-conquer tridiagonal eigensolver (mh_tridc.hip): work buffer
+  // and leaf convergence flag.
+  void *dc_state = nullptr;
   int autotune = 1;
   // Set for the duration of an allocation-free entry point: no tuning runs,
   // no tuner-table insertions and no profiling events (host or device).
@@ -176,6 +179,8 @@ int bs_coupled_copy(mh_handle h, const std::vector<int64_t> &pieces, const doubl
 
 // Release the eigensolver's streams, library handles and scratch.
 void eig_release(mh_handle h);
+void dc_release(mh_handle h);
+int *dc_error_flag(mh_handle h);
 
 // Device address of the allocation-free entry points' static scratch array
 // on h's device (looked up once per device when the first handle of that

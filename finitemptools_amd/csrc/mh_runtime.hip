@@ -170,6 +170,7 @@ int mh_destroy(mh_handle h) {
   if (h->red_host) (void)hipHostFree(h->red_host);
   if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
   mh::eig_release(h);
+  mh::dc_release(h);
   delete h;
   return MH_OK;
 }

@@ -9,8 +9,10 @@ for all sectors of one bond as
 
 1. `mh_sytrd_batched`: Householder tridiagonalisation of every matrix in one
    cooperative launch (LAPACK dsytrd, uplo = 'L', output format);
-2. `mh_stedc_batched`: rocSOLVER divide and conquer on each tridiagonal, the
-   calls spread over side streams;
+2. `mh_stedc_dc_batched`: divide and conquer on all tridiagonals at once
+   (every merge of one tree height over all matrices in the same launches;
+   TRIDIAG = "library" takes rocSOLVER's dstedc per matrix instead,
+   `mh_stedc_batched`);
 3. `mh_ormtr_batched`: back-transformation of only the eigenvector columns
    the truncation keeps.
 
@@ -26,6 +28,9 @@ from . import _lib
 from .engine import engine, ptr
 
 MAX_ORDER = 2048   # largest matrix order of mh_sytrd_batched
+# Tridiagonal eigensolver of SectorEigh: "dc" (mh_stedc_dc_batched, batched
+# divide and conquer) or "library" (rocSOLVER dstedc per matrix).
+TRIDIAG = "dc"
 
 
 def _i32(a):
@@ -82,9 +87,9 @@ class SectorEigh:
         _lib.raise_for(lib.mh_sytrd_batched(h, nm, self._n[1], self._a[1], self._v[1],
                                             ptr(self.A), ptr(self.D), ptr(self.E),
                                             ptr(self.tau)), h, "mh_sytrd_batched")
-        _lib.raise_for(lib.mh_stedc_batched(h, nm, self._n[1], self._v[1], self._a[1],
-                                            ptr(self.D), ptr(self.E), ptr(self.Z)), h,
-                       "mh_stedc_batched")
+        fn = "mh_stedc_dc_batched" if TRIDIAG == "dc" else "mh_stedc_batched"
+        _lib.raise_for(getattr(lib, fn)(h, nm, self._n[1], self._v[1], self._a[1],
+                                        ptr(self.D), ptr(self.E), ptr(self.Z)), h, fn)
         return [self.D[int(o):int(o) + n] for o, n in zip(self.v_off, self.sizes)]
 
     def check(self):
