@@ -257,16 +257,19 @@ __device__ inline double row_value(const double (&x)[SL_KR], int idx) {
 }
 
 // Barrier of the G workgroups of one matrix: workgroup g raises its own flag
-// to `step`; wavefront 0 polls every flag (lanes in parallel).  Bounded like
-// group_barrier: past the spin limit the error flag is raised, and a raised
-// error flag ends every later wait at once.
+// to `step`; wavefront 0 polls every flag (lanes in parallel).  Everything
+// the workgroups exchange inside the launch is written with device-scope
+// (sc1) stores and read with device-scope loads, so ordering needs no cache
+// write-back or invalidation: every wavefront waits until its own stores are
+// acknowledged (s_waitcnt 0) before the workgroup barrier, and only then is
+// the flag stored.  Bounded like group_barrier: past the spin limit the error
+// flag is raised, and a raised error flag ends every later wait at once.
 __device__ inline void flag_barrier(unsigned *flags, int G, int g, unsigned step, int *err) {
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x < 64) {
-    if (threadIdx.x == 0) {
-      __threadfence();
+    if (threadIdx.x == 0)
       __hip_atomic_store(flags + g, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     unsigned spins = 0;
     for (;;) {
       bool ok = true;
@@ -281,7 +284,6 @@ __device__ inline void flag_barrier(unsigned *flags, int G, int g, unsigned step
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();
   }
   __syncthreads();
 }
@@ -331,16 +333,20 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
     const bool last = (i == n - 1);
     if (wv == 0) {
       const double *bc = (i & 1) ? B1 : B0;
-      double wp[SL_KR];
+      double wp[SL_KR], bcv[SL_KR];
+      // p_{i-1} and the column i of A^(i-1) are loaded together (one round trip)
+      const double *pp = ((i - 1) & 1) ? P1 : P0;
+#pragma unroll
+      for (int k = 0; k < SL_KR; ++k) {
+        const int r = lane + 64 * k;
+        const bool in = r >= i && r < n;
+        wp[k] = (in && i > 0) ? xload(pp + r) : 0.0;
+        bcv[k] = in ? xload(bc + r) : 0.0;
+      }
       if (i > 0) {
-        const double *pp = ((i - 1) & 1) ? P1 : P0;
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < SL_KR; ++k) {
-          const int r = lane + 64 * k;
-          wp[k] = (r >= i && r < n) ? xload(pp + r) : 0.0;
-          s += wp[k] * vp[k];
-        }
+        for (int k = 0; k < SL_KR; ++k) s += wp[k] * vp[k];
         const double hh = 0.5 * tau_prev * wave_sum64(s);
 #pragma unroll
         for (int k = 0; k < SL_KR; ++k) wp[k] -= hh * vp[k];
@@ -354,7 +360,7 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
 #pragma unroll
       for (int k = 0; k < SL_KR; ++k) {
         const int r = lane + 64 * k;
-        x[k] = (r >= i && r < n) ? xload(bc + r) - vp[k] * wpi - wp[k] * vpi : 0.0;
+        x[k] = (r >= i && r < n) ? bcv[k] - vp[k] * wpi - wp[k] * vpi : 0.0;
         if (r >= i + 2) s2 += x[k] * x[k];
       }
       const double xn2 = wave_sum64(s2);

@@ -112,7 +112,7 @@ def test_selected_columns_and_many_workgroups(gpu):
 
 
 @pytest.mark.parametrize("absorb", ["left", "right"])
-@pytest.mark.parametrize("solver", ["sector", "library"])
+@pytest.mark.parametrize("solver", ["sector", "sector_kept", "library"])
 def test_device_split_matches_svd_split(gpu, absorb, solver, monkeypatch):
     """The device density-matrix split (either sector eigensolver) keeps the
     singular values of the host sector SVDs and reproduces the same truncated
