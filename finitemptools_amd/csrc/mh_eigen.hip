@@ -492,6 +492,180 @@ __global__ __launch_bounds__(OR_THREADS) void k_ormtr_cols(OrTable tab, const do
   for (int r = lane; r < n; r += 64) zg[r] = z[r];
 }
 
+// Blocked back-transformation (compact WY, Schreiber & Van Loan 1989; the
+// LAPACK dlarft/dlarfb scheme): the reflectors H_i = I - tau_i v_i v_i^T of
+// the tridiagonalisation (v_i[i+1] = 1, v_i[r] = A[r, i] for r >= i+2) are
+// grouped in blocks of WY_NB, Q_b = H_{i0} ... H_{i0+NB-1} = I - V T V^T with
+// T upper triangular, and Z <- Q Z is applied block by block from the last:
+// W = V^T Z, W = T W, Z -= V W.  k_wy_t forms T for every (matrix, block);
+// k_wy_apply keeps a tile of WY_CT columns of Z in LDS for all the blocks.
+constexpr int WY_NB = 32;
+constexpr int WY_CT = 16;
+
+struct WyTask {
+  int64_t a_off, v_off, z_off, t_off;  // matrix, tau, Z, first T block
+  int n, col0, ncol, w0;               // w0: first workgroup (k_wy_apply) or block (k_wy_t)
+};
+
+__device__ inline double refl(const double *a, int n, int i, int r) {
+  // entry r of v_i
+  return (r < i + 1) ? 0.0 : ((r == i + 1) ? 1.0 : a[r + (int64_t)i * n]);
+}
+
+__global__ __launch_bounds__(256) void k_wy_t(const WyTask *__restrict__ tab, int nt,
+                                              const double *__restrict__ A,
+                                              const double *__restrict__ tau,
+                                              double *__restrict__ T) {
+  __shared__ double G[WY_NB][WY_NB + 1];
+  __shared__ double Ts[WY_NB][WY_NB + 1];
+  int lo = 0, hi = nt - 1;
+  while (lo < hi) {
+    const int md = (lo + hi + 1) >> 1;
+    if (tab[md].w0 <= (int)blockIdx.x) lo = md; else hi = md - 1;
+  }
+  const WyTask tk = tab[lo];
+  const int n = tk.n, b = (int)blockIdx.x - tk.w0;
+  const int nrefl = n - 1;  // H_0 .. H_{n-2}
+  const int i0 = b * WY_NB, nb = min(WY_NB, nrefl - i0);
+  const double *a = A + tk.a_off;
+  const double *tv = tau + tk.v_off;
+  // G[l][j] = v_l . v_j (l < j); v_j is zero above row i0 + j + 1
+  for (int e = threadIdx.x; e < WY_NB * WY_NB; e += 256) {
+    const int l = e / WY_NB, j = e % WY_NB;
+    double s = 0.0;
+    if (l < j && j < nb) {
+      const int il = i0 + l, ij = i0 + j;
+      s = refl(a, n, il, ij + 1);  // v_j[ij+1] = 1
+      for (int r = ij + 2; r < n; ++r) s += a[r + (int64_t)il * n] * a[r + (int64_t)ij * n];
+    }
+    G[l][j] = s;
+    Ts[l][j] = 0.0;
+  }
+  __syncthreads();
+  // T[j][j] = tau_j; T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]
+  for (int j = 0; j < nb; ++j) {
+    const double tj = tv[i0 + j];
+    if (threadIdx.x < j) {
+      const int l = threadIdx.x;
+      double s = 0.0;
+      for (int m = l; m < j; ++m) s += Ts[l][m] * G[m][j];
+      Ts[l][j] = -tj * s;
+    }
+    if (threadIdx.x == 0) Ts[j][j] = tj;
+    __syncthreads();
+  }
+  double *to = T + tk.t_off + (int64_t)b * WY_NB * WY_NB;
+  for (int e = threadIdx.x; e < WY_NB * WY_NB; e += 256) to[e] = Ts[e / WY_NB][e % WY_NB];
+}
+
+typedef double wy_d4 __attribute__((ext_vector_type(4)));
+
+// entry r of reflector i (zero for i outside [0, nrefl) or r outside the column)
+__device__ inline double vref(const double *a, int n, int nrefl, int i, int r) {
+  if (i >= nrefl || r >= n || r <= i) return 0.0;
+  return (r == i + 1) ? 1.0 : a[r + (int64_t)i * n];
+}
+
+// One workgroup per WY_CT-column tile of Z (all n rows in LDS, row-major),
+// on FP64 MFMA (16x16x4: lane l holds A[l%16][l/16], B[l/16][l%16]; the
+// accumulator's register j is C[l/16 + 4j][l%16]).  Per block, from the last:
+//   W = V^T Z   (32 x WY_CT; the four waves take alternate 4-row K steps and
+//                add their partial tiles through LDS),
+//   W = T W,
+//   Z -= V W    (16-row tiles of Z over the waves, K = the 32 reflectors).
+// V is read in MFMA operand order straight from A (the L2 holds it: every
+// tile workgroup of a matrix reads the same reflectors).
+__global__ __launch_bounds__(256) void k_wy_apply(const WyTask *__restrict__ tab, int nt,
+                                                  const double *__restrict__ A,
+                                                  const double *__restrict__ T,
+                                                  double *__restrict__ Z) {
+  extern __shared__ double zt[];  // n rows x WY_CT columns, row-major
+  __shared__ double Wp[4][WY_NB][WY_CT];
+  __shared__ double W2[WY_NB][WY_CT + 1];
+  __shared__ double Tb[WY_NB][WY_NB + 1];
+  int lo = 0, hi = nt - 1;
+  while (lo < hi) {
+    const int md = (lo + hi + 1) >> 1;
+    if (tab[md].w0 <= (int)blockIdx.x) lo = md; else hi = md - 1;
+  }
+  const WyTask tk = tab[lo];
+  const int n = tk.n, tile = (int)blockIdx.x - tk.w0;
+  const int c0 = tk.col0 + tile * WY_CT, nc = min(WY_CT, tk.col0 + tk.ncol - c0);
+  const double *a = A + tk.a_off;
+  double *z = Z + tk.z_off;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int e = threadIdx.x; e < WY_CT * n; e += 256) {
+    const int c = e / n, r = e % n;
+    zt[r * WY_CT + c] = (c < nc) ? z[r + (int64_t)(c0 + c) * n] : 0.0;
+  }
+  const int nrefl = n - 1;
+  const int nblk = (nrefl + WY_NB - 1) / WY_NB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int i0 = b * WY_NB, rstart = i0 + 1;
+    const double *tb = T + tk.t_off + (int64_t)b * WY_NB * WY_NB;
+    for (int e = threadIdx.x; e < WY_NB * WY_NB; e += 256) Tb[e / WY_NB][e % WY_NB] = tb[e];
+    __syncthreads();  // also orders the previous block's Z update before these reads
+    // W = V^T Z, partial per wave
+    wy_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    // four K steps per pass: all operand loads first, then the MFMAs
+    for (int ks = wv; rstart + 4 * ks < n; ks += 16) {
+      double a0[4], a1[4], bz[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = rstart + 4 * (ks + 4 * u) + lk;
+        a0[u] = vref(a, n, nrefl, i0 + li, r);
+        a1[u] = vref(a, n, nrefl, i0 + 16 + li, r);
+        bz[u] = (r < n) ? zt[r * WY_CT + li] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], bz[u], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], bz[u], acc1, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      Wp[wv][lk + 4 * j][li] = acc0[j];
+      Wp[wv][16 + lk + 4 * j][li] = acc1[j];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < WY_NB * WY_CT; e += 256) {
+      const int l = e / WY_CT, c = e % WY_CT;
+      Wp[0][l][c] = Wp[0][l][c] + Wp[1][l][c] + Wp[2][l][c] + Wp[3][l][c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < WY_NB * WY_CT; e += 256) {
+      const int l = e / WY_CT, c = e % WY_CT;
+      double s2 = 0.0;
+      for (int m = l; m < WY_NB; ++m) s2 += Tb[l][m] * Wp[0][m][c];
+      W2[l][c] = s2;
+    }
+    __syncthreads();
+    // Z -= V W2
+    for (int t = wv; rstart + 16 * t < n; t += 4) {
+      const int rb = rstart + 16 * t;
+      wy_d4 acc = {0.0, 0.0, 0.0, 0.0};
+      double av[WY_NB / 4];
+#pragma unroll
+      for (int ks = 0; ks < WY_NB / 4; ++ks) av[ks] = vref(a, n, nrefl, i0 + 4 * ks + lk, rb + li);
+#pragma unroll
+      for (int ks = 0; ks < WY_NB / 4; ++ks)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], W2[4 * ks + lk][li], acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = rb + lk + 4 * j;
+        if (r < n) zt[r * WY_CT + li] -= acc[j];
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < WY_CT * n; e += 256) {
+    const int c = e / n, r = e % n;
+    if (c < nc) z[r + (int64_t)(c0 + c) * n] = zt[r * WY_CT + c];
+  }
+}
+
 struct EigState {
   std::vector<hipStream_t> streams;
   std::vector<rocblas_handle> rb;
@@ -501,6 +675,8 @@ struct EigState {
   int info_len = 0;
   int coresident = 0;  // sytrd workgroups that fit on the device at once
   int coresident_lds = -1;
+  void *wy = nullptr;  // back-transformation: task table and T blocks
+  size_t wy_bytes = 0;
 };
 
 EigState *state(mh_handle h) {
@@ -557,6 +733,7 @@ void eig_release(mh_handle h) {
     for (auto ev : s->done) (void)hipEventDestroy(ev);
     if (s->ready) (void)hipEventDestroy(s->ready);
     if (s->info) (void)hipFree(s->info);
+    if (s->wy) (void)hipFree(s->wy);
     delete s;
   }
   h->eig_state = nullptr;
@@ -647,6 +824,61 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
   void *args[] = {&dtab, &nt, &A, &D, &E, &tau, &X, &xhalf, &flags, &err};
   MH_HIP(h, hipLaunchCooperativeKernel((const void *)k_sytrd_lds, dim3(gtot), dim3(SY_THREADS),
                                        args, (unsigned)lds, h->stream));
+  return MH_OK;
+}
+
+constexpr size_t WY_MAX_LDS = 144 * 1024;  // Z tile of k_wy_apply
+
+// Blocked back-transformation of the selected columns: T blocks for every
+// (matrix, reflector block), then one workgroup per WY_CT-column tile.
+int ormtr_wy(mh_handle h, int nmat, const int *n, const int64_t *a_off, const int64_t *v_off,
+             const double *A, const double *tau, const int64_t *z_off, const int *col0,
+             const int *ncol, double *Z) {
+  std::vector<WyTask> tt, ta;
+  int64_t toff = 0;
+  int nb_tot = 0, wg_tot = 0, nmax = 0;
+  for (int m = 0; m < nmat; ++m) {
+    if (ncol[m] == 0 || n[m] < 2) continue;  // order 1: Q = I
+    const int nblk = (n[m] - 1 + WY_NB - 1) / WY_NB;
+    WyTask t{a_off[m], v_off[m], z_off[m], toff, n[m], col0[m], ncol[m], nb_tot};
+    tt.push_back(t);
+    t.w0 = wg_tot;
+    ta.push_back(t);
+    nb_tot += nblk;
+    wg_tot += (ncol[m] + WY_CT - 1) / WY_CT;
+    toff += (int64_t)nblk * WY_NB * WY_NB;
+    nmax = std::max(nmax, n[m]);
+  }
+  if (tt.empty()) return MH_OK;
+  EigState *st = state(h);
+  const size_t tab_bytes = (tt.size() * sizeof(WyTask) + 255) & ~size_t(255);
+  const size_t need = 2 * tab_bytes + (size_t)toff * sizeof(double);
+  if (st->wy_bytes < need) {
+    MH_HIP(h, hipStreamSynchronize(h->stream));
+    if (st->wy) MH_HIP(h, hipFree(st->wy));
+    st->wy = nullptr;
+    st->wy_bytes = 0;
+    hipError_t e = hipMalloc(&st->wy, need);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      st->wy = nullptr;
+      return fail(h, MH_ERR_OOM, "mh_ormtr_batched: %zu bytes: %s", need, hipGetErrorString(e));
+    }
+    st->wy_bytes = need;
+  }
+  char *base = (char *)st->wy;
+  WyTask *dtt = (WyTask *)base, *dta = (WyTask *)(base + tab_bytes);
+  double *T = (double *)(base + 2 * tab_bytes);
+  MH_HIP(h, hipMemcpyAsync(dtt, tt.data(), tt.size() * sizeof(WyTask), hipMemcpyHostToDevice,
+                           h->stream));
+  MH_HIP(h, hipMemcpyAsync(dta, ta.data(), ta.size() * sizeof(WyTask), hipMemcpyHostToDevice,
+                           h->stream));
+  const int nt = (int)tt.size();
+  hipLaunchKernelGGL(k_wy_t, dim3(nb_tot), dim3(256), 0, h->stream, dtt, nt, A, tau, T);
+  MH_TRY(check_launch(h, "k_wy_t"));
+  const size_t lds = (size_t)WY_CT * nmax * sizeof(double);
+  hipLaunchKernelGGL(k_wy_apply, dim3(wg_tot), dim3(256), lds, h->stream, dta, nt, A, T, Z);
+  MH_TRY(check_launch(h, "k_wy_apply"));
   return MH_OK;
 }
 
@@ -817,6 +1049,8 @@ extern "C" int mh_ormtr_batched(mh_handle h, int nmat, const int *n, const int64
     nmax = std::max(nmax, n[m]);
   }
   MH_HIP(h, hipSetDevice(h->device));
+  if ((size_t)WY_CT * nmax * sizeof(double) <= WY_MAX_LDS && !getenv("MH_ORMTR_COLS"))
+    return ormtr_wy(h, nmat, n, a_off, v_off, A, tau, z_off, col0, ncol, Z);
   const size_t lds = (size_t)OR_WAVES * nmax * sizeof(double);
   int m = 0;
   while (m < nmat) {

@@ -87,6 +87,29 @@ def test_eigenpairs(gpu, sizes):
         assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-12 * n
 
 
+def test_blocked_back_transform_matches_reflector_by_reflector(gpu, monkeypatch):
+    """The blocked (compact WY) back-transformation equals the reflector-by-
+    reflector one column for column, for orders around the block size and a
+    selection of columns that ends inside a column tile."""
+    import torch
+    rng = np.random.default_rng(11)
+    sizes = [1, 2, 3, 31, 32, 33, 34, 65, 97, 300, 744]
+    mats = _mats(rng, sizes)
+    out = {}
+    for mode in ("blocked", "columns"):
+        if mode == "columns":
+            monkeypatch.setenv("MH_ORMTR_COLS", "1")
+        se = eigen.SectorEigh(sizes, "cuda")
+        for j, M in enumerate(mats):
+            se.matrix(j).copy_(torch.from_numpy(M))
+        se.eigenvalues()
+        keep = [max(1, n - 5) for n in sizes]
+        out[mode] = [v.cpu().numpy() for v in se.vectors(keep)]
+        se.check()
+    for n, a, b in zip(sizes, out["blocked"], out["columns"]):
+        assert np.abs(a - b).max() <= 1e-13 * max(n, 10), n
+
+
 def test_selected_columns_and_many_workgroups(gpu):
     """Top-k back-transform equals the same columns of the full one; 40
     matrices of order 1000 need more workgroups than are resident at once,
