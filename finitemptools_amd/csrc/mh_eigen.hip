@@ -238,12 +238,39 @@ constexpr int SL_MAXN = 64 * SL_KR;       // largest order of the LDS form
 struct SlTask {
   int64_t a_off, v_off, x_off;  // matrix, D/E/tau, exchange rows
   int n, G, g0, ncl;            // order, workgroups, first workgroup, LDS columns each
-  int f_off, pad;               // first barrier flag
+  int f_off, np;                // first barrier flag, order padded to a multiple of 64
 };
 
+// A double moved between lanes by one DPP move of each 32-bit half.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// value of v in lane src (uniform src), read into scalar registers
+__device__ inline double lane_value(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wavefront (every lane active), returned to all:
+// pairs and quads by quad permutations, rows of 16 by two rotations, then
+// row 0 + 1 and row 2 + 3 by the row-15 broadcast and the total into row 3 by
+// the row-31 broadcast; lane 63 holds it.  The order of additions is fixed,
+// so every workgroup that sums the same data gets the same bits.
 __device__ inline double wave_sum64(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  v += dpp_f64<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x124, 0xF>(v);  // row_ror:4
+  v += dpp_f64<0x128, 0xF>(v);  // row_ror:8
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return lane_value(v, 63);
 }
 
 // value at row idx of a lane-distributed vector (lane idx % 64, slot idx / 64)
@@ -253,7 +280,7 @@ __device__ inline double row_value(const double (&x)[SL_KR], int idx) {
 #pragma unroll
   for (int k = 0; k < SL_KR; ++k)
     if (k == kk) t = x[k];
-  return __shfl(t, idx & 63);
+  return lane_value(t, idx & 63);
 }
 
 // Barrier of the G workgroups of one matrix: workgroup g raises its own flag
@@ -304,12 +331,17 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
     if (tabv[md].g0 <= (int)blockIdx.x) lo = md; else hi = md - 1;
   }
   const SlTask tk = tabv[lo];
-  const int n = tk.n, G = tk.G, g = (int)blockIdx.x - tk.g0, ncl = tk.ncl;
+  const int n = tk.n, np = tk.np, G = tk.G, g = (int)blockIdx.x - tk.g0, ncl = tk.ncl;
+  const int kr = np >> 6;  // row slots per lane in use
   double *a = A + tk.a_off;
-  double *cols = sm;                         // ncl columns of n rows
-  double *sV = sm + (size_t)ncl * n;         // v_i
-  double *sW = sV + n;                       // w_{i-1}
-  double *sP = sW + n;                       // v_{i-1}
+  // LDS: ncl columns of np rows (rows n..np-1 zero), v_i, w_{i-1}, v_{i-1}
+  // (np each)
+  double *cols = sm;
+  double *sV = sm + (size_t)ncl * np;
+  double *sW = sV + np;
+  double *sP = sW + np;
+  // exchange rows (np per matrix): p of even / odd steps, next column of
+  // even / odd steps
   double *const P0 = X + tk.x_off, *const P1 = P0 + xhalf;
   double *const B0 = P1 + xhalf, *const B1 = B0 + xhalf;
   unsigned *fl = flags + tk.f_off;
@@ -319,7 +351,8 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
   for (int q = 0; q < ncl; ++q) {
     const int j = g + q * G;
     if (j >= n) break;
-    for (int r = threadIdx.x; r < n; r += SY_THREADS) cols[(size_t)q * n + r] = a[r + (int64_t)j * n];
+    for (int r = threadIdx.x; r < np; r += SY_THREADS)
+      cols[(size_t)q * np + r] = (r < n) ? a[r + (int64_t)j * n] : 0.0;
   }
   if (g == 0)
     for (int r = threadIdx.x; r < n; r += SY_THREADS) xstore(B0 + r, a[r]);
@@ -333,15 +366,22 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
     const bool last = (i == n - 1);
     if (wv == 0) {
       const double *bc = (i & 1) ? B1 : B0;
-      double wp[SL_KR], bcv[SL_KR];
-      // p_{i-1} and the column i of A^(i-1) are loaded together (one round trip)
       const double *pp = ((i - 1) & 1) ? P1 : P0;
+      // p_{i-1} and the column i of A^(i-1) in one round trip; every load
+      // lies inside the matrix's padded exchange rows, the mask selects
+      double wp[SL_KR], bcv[SL_KR];
+      const int k0 = i >> 6;  // slots below k0 hold only finished rows
 #pragma unroll
       for (int k = 0; k < SL_KR; ++k) {
-        const int r = lane + 64 * k;
-        const bool in = r >= i && r < n;
-        wp[k] = (in && i > 0) ? xload(pp + r) : 0.0;
-        bcv[k] = in ? xload(bc + r) : 0.0;
+        wp[k] = bcv[k] = 0.0;
+        if (k >= k0 && k < kr) {
+          const int r = lane + 64 * k;
+          const double pv = (i > 0) ? xload(pp + r) : 0.0;
+          const double bv = xload(bc + r);
+          const bool in = r >= i && r < n;
+          wp[k] = in ? pv : 0.0;
+          bcv[k] = in ? bv : 0.0;
+        }
       }
       if (i > 0) {
         double s = 0.0;
@@ -350,9 +390,6 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
         const double hh = 0.5 * tau_prev * wave_sum64(s);
 #pragma unroll
         for (int k = 0; k < SL_KR; ++k) wp[k] -= hh * vp[k];
-      } else {
-#pragma unroll
-        for (int k = 0; k < SL_KR; ++k) wp[k] = 0.0;
       }
       const double vpi = row_value(vp, i), wpi = row_value(wp, i);
       double x[SL_KR];
@@ -360,12 +397,13 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
 #pragma unroll
       for (int k = 0; k < SL_KR; ++k) {
         const int r = lane + 64 * k;
-        x[k] = (r >= i && r < n) ? bcv[k] - vp[k] * wpi - wp[k] * vpi : 0.0;
+        x[k] = bcv[k] - vp[k] * wpi - wp[k] * vpi;  // zero outside [i, n)
         if (r >= i + 2) s2 += x[k] * x[k];
       }
       const double xn2 = wave_sum64(s2);
       const double di = row_value(x, i);
       double beta = 0.0, tu = 0.0;
+      const bool own = (i % G == g);
       if (!last) {
         const double alpha = row_value(x, i + 1);
         double scale = 1.0;
@@ -375,23 +413,28 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
           tu = (beta - alpha) / beta;
           scale = 1.0 / (alpha - beta);
         }
-        const bool own = (i % G == g);
 #pragma unroll
         for (int k = 0; k < SL_KR; ++k) {
           const int r = lane + 64 * k;
           vv[k] = (r >= i + 2) ? x[k] * scale : ((r == i + 1) ? 1.0 : 0.0);
-          if (r < n) {
+          if (k < kr) {
             sV[r] = vv[k];
             sW[r] = wp[k];
             sP[r] = vp[k];
-            if (own && r >= i + 2) a[r + (int64_t)i * n] = vv[k];
           }
         }
-        if (own && lane == 0) {
-          a[i + (int64_t)i * n] = di;
-          a[i + 1 + (int64_t)i * n] = beta;
+        if (own) {
+#pragma unroll
+          for (int k = 0; k < SL_KR; ++k) {
+            const int r = lane + 64 * k;
+            if (r >= i + 2 && r < n) a[r + (int64_t)i * n] = vv[k];
+          }
+          if (lane == 0) {
+            a[i + (int64_t)i * n] = di;
+            a[i + 1 + (int64_t)i * n] = beta;
+          }
         }
-      } else if (i % G == g && lane == 0) {
+      } else if (own && lane == 0) {
         a[i + (int64_t)i * n] = di;
       }
       if (lane == 0) {
@@ -407,14 +450,21 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
     if (last) break;
     const double tu = scal[0];
     {
+      // step i-1's rank-2 update of the own columns and their p_i entries,
+      // branch-free over the padded rows: v_{i-1} and w_{i-1} vanish above
+      // row i and v_i above row i + 1, so rows that are already final are
+      // left as they are or touched only where no later step reads them, and
+      // the zero padding stays zero
       double vpB[SL_KR], wpB[SL_KR], vcB[SL_KR];
 #pragma unroll
       for (int k = 0; k < SL_KR; ++k) {
-        const int r = lane + 64 * k;
-        const bool in = r < n;
-        vpB[k] = in ? sP[r] : 0.0;
-        wpB[k] = in ? sW[r] : 0.0;
-        vcB[k] = in ? sV[r] : 0.0;
+        vpB[k] = wpB[k] = vcB[k] = 0.0;
+        if (k < kr) {
+          const int r = lane + 64 * k;
+          vpB[k] = sP[r];
+          wpB[k] = sW[r];
+          vcB[k] = sV[r];
+        }
       }
       double *pc = (i & 1) ? P1 : P0;
       double *bn = ((i + 1) & 1) ? B1 : B0;
@@ -423,17 +473,25 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
         const int j = g + q * G;
         if (j >= n) break;
         const double vpj = sP[j], wpj = sW[j];
-        const bool pub = (j == i + 1);
-        double *col = cols + (size_t)q * n;
+        double *col = cols + (size_t)q * np;
+        double xr[SL_KR];
+#pragma unroll
+        for (int k = 0; k < SL_KR; ++k)
+          if (k < kr) xr[k] = col[lane + 64 * k];
         double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < SL_KR; ++k) {
-          const int r = lane + 64 * k;
-          if (r >= i + 1 && r < n) {
-            const double x = col[r] - vpB[k] * wpj - wpB[k] * vpj;
-            col[r] = x;
-            if (pub) xstore(bn + r, x);
-            acc += x * vcB[k];
+          if (k < kr) {
+            xr[k] = xr[k] - vpB[k] * wpj - wpB[k] * vpj;
+            col[lane + 64 * k] = xr[k];
+            acc += xr[k] * vcB[k];
+          }
+        }
+        if (j == i + 1) {  // the next step's column, for every workgroup
+#pragma unroll
+          for (int k = 0; k < SL_KR; ++k) {
+            const int r = lane + 64 * k;
+            if (k < kr && r >= i + 1 && r < n) xstore(bn + r, xr[k]);
           }
         }
         acc = wave_sum64(acc);
@@ -746,8 +804,9 @@ void eig_release(mh_handle h) {
 // Plan and launch k_sytrd_lds: every matrix gets G workgroups holding
 // ceil(n / G) columns each, within an LDS budget per workgroup; the budget
 // is the largest (fewest workgroups, cheapest barriers) for which every
-// workgroup of the batch is resident at once.  MH_ERR_UNSUPPORTED when no
-// budget fits (the caller then takes the global-memory form).
+// workgroup of the batch is resident at once.  If no budget admits one
+// launch, the matrices go, largest first, into consecutive launches that each
+// fit at the largest budget.
 int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const int64_t *v_off,
               double *A, double *D, double *E, double *tau) {
   int ncu = 0, lds_cu = 0;
@@ -755,53 +814,75 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
   MH_HIP(h, hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                                   h->device));
   if (lds_cu <= 0) lds_cu = 64 * 1024;
-  std::vector<SlTask> tab(nmat);
-  int64_t xsum = 0;
-  for (int m = 0; m < nmat; ++m) xsum += n[m];
-  // largest matrices first: their workgroups start first
   std::vector<int> order(nmat);
   for (int m = 0; m < nmat; ++m) order[m] = m;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
-  size_t lds = 0;
-  int gtot = 0;
-  bool ok = false;
-  for (int k = 1; k <= 4 && !ok; ++k) {
-    const int64_t budget = (int64_t)(lds_cu / k) - 2048;  // bytes for columns + 3 vectors
-    gtot = 0;
-    lds = 0;
+  std::vector<SlTask> tab(nmat);
+  // plan at budget level k: fills tab (in `order`), returns the LDS bytes
+  // per workgroup (0 if a matrix does not fit) and the workgroup total
+  auto plan = [&](int k, int *gsum) -> size_t {
+    const int64_t budget = (int64_t)(lds_cu / k) - 2048;  // columns + 3 vectors
+    size_t lds = 0;
     int64_t xo = 0;
-    bool fits = true;
+    int gt = 0;
     for (int pi = 0; pi < nmat; ++pi) {
       const int m = order[pi];
       const int nm = n[m];
-      const int64_t capc = budget / 8 / nm - 3;  // columns that fit
-      if (capc < 1) {
-        fits = false;
-        break;
-      }
-      int G = (int)((nm + capc - 1) / capc);
+      const int np = (nm + 63) & ~63;
+      const int64_t capc = (budget / 8 - 3 * (int64_t)np) / np;  // columns that fit
+      if (capc < 1) return 0;
+      const int G = (int)((nm + capc - 1) / capc);
       const int ncl = (nm + G - 1) / G;
-      SlTask &t = tab[pi];
-      t.a_off = a_off[m];
-      t.v_off = v_off[m];
-      t.x_off = xo;
-      t.n = nm;
-      t.G = G;
-      t.g0 = gtot;
-      t.ncl = ncl;
-      t.f_off = gtot;
-      t.pad = 0;
-      xo += nm;
-      gtot += G;
-      lds = std::max(lds, (size_t)(ncl + 3) * nm * sizeof(double));
+      tab[pi] = SlTask{a_off[m], v_off[m], xo, nm, G, 0, ncl, gt, np};
+      xo += np;
+      gt += G;
+      lds = std::max(lds, ((size_t)ncl + 3) * np * sizeof(double));
     }
-    if (!fits) continue;
+    *gsum = gt;
+    return lds;
+  };
+  auto capacity = [&](size_t lds, int *cap) -> int {
     int per_cu = 0;
     MH_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_sytrd_lds,
                                                             SY_THREADS, lds));
-    ok = per_cu >= 1 && gtot <= per_cu * ncu;
+    *cap = per_cu * ncu;
+    return MH_OK;
+  };
+  size_t lds = 0;
+  int gtot = 0, cap = 0;
+  bool single = false;
+  for (int k = 1; k <= 4 && !single; ++k) {
+    lds = plan(k, &gtot);
+    if (lds == 0) continue;
+    MH_TRY(capacity(lds, &cap));
+    single = cap >= 1 && gtot <= cap;
   }
-  if (!ok) return MH_ERR_UNSUPPORTED;
+  if (!single) {
+    lds = plan(1, &gtot);
+    if (lds == 0) return MH_ERR_UNSUPPORTED;
+    MH_TRY(capacity(lds, &cap));
+    if (cap < 1) return MH_ERR_UNSUPPORTED;
+    for (const SlTask &t : tab)
+      if (t.G > cap) return MH_ERR_UNSUPPORTED;
+  }
+  // launches: consecutive runs of the plan whose workgroups fit at once;
+  // g0 counts from the launch's first workgroup, flags stay distinct
+  std::vector<std::pair<int, int>> launches;  // (first task, task count)
+  {
+    int first = 0, g = 0;
+    for (int pi = 0; pi < nmat; ++pi) {
+      if (g + tab[pi].G > cap) {
+        launches.push_back({first, pi - first});
+        first = pi;
+        g = 0;
+      }
+      tab[pi].g0 = g;
+      g += tab[pi].G;
+    }
+    launches.push_back({first, nmat - first});
+  }
+  int64_t xsum = 0;
+  for (int m = 0; m < nmat; ++m) xsum += (n[m] + 63) & ~63;
   // scratch: 4 exchange halves of xsum doubles, the flags, the task table,
   // the error flag
   const size_t xbytes = (size_t)xsum * 4 * sizeof(double);
@@ -819,15 +900,30 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
   MH_HIP(h, hipMemsetAsync(flags, 0, (size_t)gtot * sizeof(unsigned), h->stream));
   MH_HIP(h, hipMemcpyAsync(dtab, tab.data(), (size_t)nmat * sizeof(SlTask), hipMemcpyHostToDevice,
                            h->stream));
-  int nt = nmat;
   int64_t xhalf = xsum;
-  void *args[] = {&dtab, &nt, &A, &D, &E, &tau, &X, &xhalf, &flags, &err};
-  MH_HIP(h, hipLaunchCooperativeKernel((const void *)k_sytrd_lds, dim3(gtot), dim3(SY_THREADS),
-                                       args, (unsigned)lds, h->stream));
+  for (const auto &L : launches) {
+    SlTask *t0 = dtab + L.first;
+    int nt = L.second;
+    int g = 0;
+    for (int pi = L.first; pi < L.first + L.second; ++pi) g += tab[pi].G;
+    void *args[] = {&t0, &nt, &A, &D, &E, &tau, &X, &xhalf, &flags, &err};
+    MH_HIP(h, hipLaunchCooperativeKernel((const void *)k_sytrd_lds, dim3(g), dim3(SY_THREADS), args,
+                                         (unsigned)lds, h->stream));
+  }
   return MH_OK;
 }
 
-constexpr size_t WY_MAX_LDS = 144 * 1024;  // Z tile of k_wy_apply
+// Largest dynamic LDS (the Z tile) k_wy_apply may take: the device's limit
+// per workgroup less the kernel's static LDS.
+int wy_lds_limit(mh_handle h, size_t *out) {
+  int per_block = 0;
+  MH_HIP(h, hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                  h->device));
+  hipFuncAttributes fa{};
+  MH_HIP(h, hipFuncGetAttributes(&fa, (const void *)k_wy_apply));
+  *out = (per_block > (int)fa.sharedSizeBytes) ? (size_t)per_block - fa.sharedSizeBytes : 0;
+  return MH_OK;
+}
 
 // Blocked back-transformation of the selected columns: T blocks for every
 // (matrix, reflector block), then one workgroup per WY_CT-column tile.
@@ -1049,7 +1145,9 @@ extern "C" int mh_ormtr_batched(mh_handle h, int nmat, const int *n, const int64
     nmax = std::max(nmax, n[m]);
   }
   MH_HIP(h, hipSetDevice(h->device));
-  if ((size_t)WY_CT * nmax * sizeof(double) <= WY_MAX_LDS && !getenv("MH_ORMTR_COLS"))
+  size_t wy_max = 0;
+  MH_TRY(wy_lds_limit(h, &wy_max));
+  if ((size_t)WY_CT * nmax * sizeof(double) <= wy_max && !getenv("MH_ORMTR_COLS"))
     return ormtr_wy(h, nmat, n, a_off, v_off, A, tau, z_off, col0, ncol, Z);
   const size_t lds = (size_t)OR_WAVES * nmax * sizeof(double);
   int m = 0;

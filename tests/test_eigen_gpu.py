@@ -70,10 +70,11 @@ def test_sytrd_matches_lapack(gpu):
                 np.testing.assert_allclose(Am[i + 2:, i], a_ref[i + 2:, i], atol=1e-8, rtol=0)
 
 
-@pytest.mark.parametrize("sizes", [[1, 2, 7, 64, 300, 801], list(range(1, 71))])
+@pytest.mark.parametrize("sizes", [[1, 2, 7, 64, 300, 801], list(range(1, 71)), [1100, 63, 1024]])
 def test_eigenpairs(gpu, sizes):
     """Eigenvalues vs eigvalsh, residuals and orthonormality; the second case
-    spans more matrices than one launch's table holds."""
+    spans more matrices than one launch's table holds, the third takes the
+    global-memory tridiagonalisation (an order above the LDS form's 1024)."""
     import torch
     rng = np.random.default_rng(len(sizes))
     mats = _mats(rng, sizes)
