@@ -16,6 +16,7 @@ SOURCES = [
     "mh_adjoint.hip",
     "mh_eigen.hip",
     "mh_tridc.hip",
+    "mh_batched.hip",
     "mh_heev.hip",
     "mh_bs_planner.cpp",
     "mh_schur.cpp",

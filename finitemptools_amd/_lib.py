@@ -142,6 +142,7 @@ SIGNATURES = {
                                  c_void_p, c_void_p]),
     "mh_stedc_dc_batched": (c_int, [c_void_p, c_int, P(c_int), P(c_int64), P(c_int64), c_void_p,
                                     c_void_p, c_void_p]),
+    "mh_gemm_batched": (c_int, [c_void_p, c_int, P(c_int64), c_double, c_double]),
     "mh_ormtr_batched": (c_int, [c_void_p, c_int, P(c_int), P(c_int64), P(c_int64), c_void_p,
                                  c_void_p, P(c_int64), P(c_int), P(c_int), c_void_p]),
     "mh_eig_check": (c_int, [c_void_p, c_int]),

@@ -655,6 +655,330 @@ def _batched_eigh(grams, dev):
     return [r for r, _ in results]
 
 
+def _strided_pieces(pieces, device):
+    """Element indices of rectangular pieces copied between two strided
+    buffers: row p = (blk_off, ldb, nr, nc, src_base, sr, sc, r0, c0); element
+    (i, j) of the piece (i < nr, j < nc) is blk_off + i + j*ldb on the block
+    side and src_base + (r0 + i)*sr + (c0 + j)*sc on the source side (strides
+    may be negative).  Returns (blk_idx, src_idx) on `device`."""
+    if len(pieces) == 0:
+        z = torch.zeros(0, dtype=torch.int64, device=device)
+        return z, z
+    P = torch.as_tensor(np.asarray(pieces, dtype=np.int64), device=device)
+    sizes = P[:, 2] * P[:, 3]
+    total = int(sum(int(p[2]) * int(p[3]) for p in pieces))
+    pid = torch.repeat_interleave(torch.arange(len(pieces), device=device), sizes,
+                                  output_size=total)
+    start = torch.cumsum(sizes, 0) - sizes
+    local = torch.arange(total, device=device) - start[pid]
+    Pp = P[pid]
+    i, j = local % Pp[:, 2], local // Pp[:, 2]
+    return (Pp[:, 0] + i + j * Pp[:, 1],
+            Pp[:, 4] + (Pp[:, 7] + i) * Pp[:, 5] + (Pp[:, 8] + j) * Pp[:, 6])
+
+
+def _gemms(descs, alpha=1.0, beta=0.0):
+    """One mh_gemm_batched launch over raw (pointer, extent, stride) rows."""
+    if not descs:
+        return
+    from ._lib import load, raise_for
+    from .engine import engine
+    import ctypes
+    d = np.asarray(descs, dtype=np.int64).reshape(-1, 12)
+    e = engine(torch.device("cuda", torch.cuda.current_device()))
+    h = e.bind_stream()
+    raise_for(load().mh_gemm_batched(h, len(d), d.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     float(alpha), float(beta)), h, "mh_gemm_batched")
+
+
+def _split_rho_sector(tmap, vec, trunc, absorb):
+    """split_two_site_rho with every per-sector product batched: the sector
+    density matrices, the projections on eigenvector blocks, both levels of
+    the exact cut and the assembly of the two new site tensors run as a few
+    launches per bond (mh_gemm_batched, the batched eigensolver, gathers over
+    flat buffers) instead of a handful of library calls per sector.  Same
+    arithmetic as the per-sector path, in the same order of operations per
+    sector.  Returns None when a bond needs the per-sector path: a merged
+    spectrum whose kept values are not signal-then-tail within a sector, or a
+    cut the second level cannot resolve (the caller then takes that path, or
+    the SVD)."""
+    xl = tmap.xl
+    dev = vec.device
+    left = absorb == "right"
+    charges = list(tmap.charges)
+    cat = torch.empty_like(vec)
+    cat[tmap.dest(dev)] = vec
+    pc = cat.data_ptr()
+    # sector c: X_c = M_c (left sweep, n = R rows of width w = C) or M_c^T
+    # (right sweep, n = C, w = R), both read in place from the row-major M_c
+    sec = []
+    for c in charges:
+        R, C = tmap.shape[c]
+        b = tmap.base[c]
+        if left:
+            sec.append((c, R, C, b, C, 1, R, C))
+        else:
+            sec.append((c, C, R, b, 1, C, R, C))
+    ns = [t[1] for t in sec]
+    se = eigen.SectorEigh(ns, dev)
+    pA, pZ = se.A.data_ptr(), se.Z.data_ptr()
+    # level 1: rho_c = X X^T straight into the eigensolver's matrices
+    _gemms([(pc + 8 * b, pc + 8 * b, pA + 8 * int(se.a_off[j]), n, n, w, sr, sc_, sc_, sr, 1, n)
+            for j, (c, n, w, b, sr, sc_, R, C) in enumerate(sec)])
+    lams = se.eigenvalues()
+    lam_all = torch.cat(lams).cpu().numpy()
+    se.check()
+    lam_host, pos = {}, 0
+    for j, (c, n, w, b, sr, sc_, R, C) in enumerate(sec):
+        r = min(R, C)
+        lam_host[c] = np.sqrt(np.maximum(lam_all[pos:pos + n][::-1][:r], 0.0))
+        pos += n
+    keys = charges
+    s2 = np.concatenate([lam_host[c] ** 2 for c in keys])
+    noise = np.concatenate([np.full(len(lam_host[c]), tr.RHO_NOISE * ns[j] * tr.EPS64 *
+                                    float(lam_host[c][0] ** 2 if len(lam_host[c]) else 0.0))
+                            for j, c in enumerate(keys)])
+    srt = np.argsort(-s2, kind="stable")
+    wmax = max(t[2] for t in sec)
+    z_off = [int(o) for o in se.a_off]
+    if tr.rho_cut_resolved(s2[srt], noise[srt], trunc):
+        keep = _global_keep_host({c: (None, sv) for c, sv in lam_host.items()}, trunc)
+        keep = {c: k for c, k in keep.items() if lam_host[c][k - 1] > 0.0}
+        se.vectors([keep.get(c, 0) for c in charges])
+        # per kept sector: nsig = k columns of descending eigenvectors, no tail
+        plan = {c: (k, 0) for c, k in keep.items()}
+        Tbuf, tb_off, pT = None, {}, 0
+    else:
+        out = _refine_sector(tmap, sec, se, lam_host, lam_all, trunc, dev, pc, wmax)
+        if out is None:
+            return None
+        plan, Tbuf, tb_off = out
+        pT = Tbuf.data_ptr() if Tbuf is not None else 0
+    # projections of X on the kept vectors: rows of s v (left) or of (u s)^T
+    # (right), signal rows from the descending eigenvectors (read backwards
+    # from the eigensolver's columns), tail rows from the tail vectors
+    order = [c for c in charges if c in plan]
+    kept = {c: plan[c][0] + plan[c][1] for c in order}
+    rowb, acc = {}, 0
+    for c in order:
+        rowb[c] = acc
+        acc += kept[c]
+    Bk = torch.zeros((max(acc, 1), wmax), dtype=torch.float64, device=dev)
+    pB = Bk.data_ptr()
+    descs = []
+    info = {t[0]: (j, t) for j, t in enumerate(sec)}
+    for c in order:
+        j, (_, n, w, b, sr, sc_, R, C) = info[c]
+        ks, kt = plan[c]
+        zlast = z_off[j] + (n - 1) * n   # column n-1 of Z_j: the largest eigenvalue
+        if ks:
+            descs.append((pZ + 8 * zlast, pc + 8 * b, pB + 8 * rowb[c] * wmax,
+                          ks, w, n, -n, 1, sr, sc_, wmax, 1))
+        if kt:
+            descs.append((pT + 8 * tb_off[c], pc + 8 * b, pB + 8 * (rowb[c] + ks) * wmax,
+                          kt, w, n, n, 1, sr, sc_, wmax, 1))
+    _gemms(descs)
+    svec = torch.linalg.vector_norm(Bk[:acc], dim=1)
+    norm = torch.linalg.vector_norm(svec)
+    s_host = (svec / norm).cpu().numpy()
+    mid = Space(kept.items())
+    l1 = BlockLayout(xl.lspace, xl.phys, 1, mid)
+    l2 = BlockLayout(mid, xl.phys, 1, xl.rspace)
+    svals = {c: s_host[rowb[c]:rowb[c] + kept[c]] for c in order}
+    # assembly: the eigenvector side straight from Z (and the tail vectors),
+    # the projected side from Bk scaled by 1/norm
+    zp, tp, bp = [], [], []
+    for c in order:
+        j, (_, n, w, b, sr, sc_, R, C) = info[c]
+        ks, kt = plan[c]
+        k = ks + kt
+        zlast = z_off[j] + (n - 1) * n
+        ro = 0
+        for qa, s1, Da in tmap.rows[c]:
+            blk = l1.index[(qa, (s1,))].off
+            if left:   # u[ro + i, col] = eigenvector col at row ro + i
+                if ks:
+                    zp.append((blk, Da, Da, ks, zlast, 1, -n, ro, 0))
+                if kt:
+                    tp.append((blk + ks * Da, Da, Da, kt, tb_off[c], 1, n, ro, 0))
+            else:      # (u s)[ro + i, col] = Bk[row col, ro + i]
+                bp.append((blk, Da, Da, k, rowb[c] * wmax, 1, wmax, ro, 0))
+            ro += Da
+        co = 0
+        for s2_, qb, Db in tmap.cols[c]:
+            blk = l2.index[(c, (s2_,))].off
+            if left:   # (s v)[col, co + jj] = Bk[row col, co + jj]
+                bp.append((blk, k, k, Db, rowb[c] * wmax, wmax, 1, 0, co))
+            else:      # v[col, co + jj] = eigenvector col at row co + jj
+                if ks:
+                    zp.append((blk, k, ks, Db, zlast, -n, 1, 0, co))
+                if kt:
+                    tp.append((blk + ks, k, kt, Db, tb_off[c], n, 1, 0, co))
+            co += Db
+    d1 = torch.zeros(l1.size, dtype=torch.float64, device=dev)
+    d2 = torch.zeros(l2.size, dtype=torch.float64, device=dev)
+    dz, dp = (d1, d2) if left else (d2, d1)
+    for pieces, src, dst, scale in ((zp, se.Z, dz, None), (tp, Tbuf, dz, None),
+                                    (bp, Bk.reshape(-1), dp, norm)):
+        if not pieces:
+            continue
+        bi, si = _strided_pieces(pieces, dev)
+        vals = src[si]
+        dst[bi] = vals / scale if scale is not None else vals
+    return BlockTensor(l1, d1), BlockTensor(l2, d2), svals
+
+
+def _refine_sector(tmap, sec, se, lam_host, lam_all, trunc, dev, pc, wmax):
+    """Second level of the exact cut for _split_rho_sector, batched over the
+    sectors (the per-sector form and its reasoning: _refine_tail).  Returns
+    ({c: (signal kept, tail kept)}, tail vector buffer, {c: its offset}) or
+    None when the bond needs the per-sector path."""
+    se.vectors(se.sizes)
+    pZ = se.Z.data_ptr()
+    z_off = [int(o) for o in se.a_off]
+    parts = []   # (j, c, n, w, b, sr, sc, r, ks)
+    for j, (c, n, w, b, sr, sc_, R, C) in enumerate(sec):
+        lam = lam_host[c] ** 2
+        lam_max = float(lam[0]) if len(lam) else 0.0
+        bound = tr.RHO_NOISE * n * tr.EPS64 * lam_max
+        ks = int(np.count_nonzero(lam > SIGNAL_MARGIN * bound))
+        parts.append((j, c, n, w, b, sr, sc_, min(R, C), ks, lam_max))
+    # B = Q^T X for every sector (rows descending), signal rows first
+    rb, acc = [], 0
+    for p in parts:
+        rb.append(acc)
+        acc += p[2]
+    Bp = torch.zeros((max(acc, 1), wmax), dtype=torch.float64, device=dev)
+    pB = Bp.data_ptr()
+    _gemms([(pZ + 8 * (z_off[j] + (n - 1) * n), pc + 8 * b, pB + 8 * rb[q] * wmax,
+             n, w, n, -n, 1, sr, sc_, wmax, 1)
+            for q, (j, c, n, w, b, sr, sc_, r, ks, lm) in enumerate(parts)])
+    bnorm = torch.linalg.vector_norm(Bp, dim=1)
+    # normalised signal rows, two Newton-Schulz steps (a batched product each)
+    sig_rows = np.concatenate([np.arange(rb[q], rb[q] + p[8]) for q, p in enumerate(parts)]
+                              + [np.zeros(0, dtype=np.int64)]).astype(np.int64)
+    vb, acc_v = [], 0
+    for p in parts:
+        vb.append(acc_v)
+        acc_v += p[8]
+    tails = [q for q, p in enumerate(parts) if p[8] < p[7]]
+    if acc_v:
+        sr_idx = torch.as_tensor(sig_rows, device=dev)
+        Vs = Bp[sr_idx] / bnorm[sr_idx, None]
+        gsz = [p[8] * p[8] for p in parts]
+        go = np.concatenate([[0], np.cumsum(gsz)[:-1]]).astype(np.int64)
+        G = torch.empty(max(int(sum(gsz)), 1), dtype=torch.float64, device=dev)
+        pG = G.data_ptr()
+        for _ in range(2):
+            pV = Vs.data_ptr()
+            _gemms([(pV + 8 * vb[q] * wmax, pV + 8 * vb[q] * wmax, pG + 8 * int(go[q]),
+                     p[8], p[8], p[3], wmax, 1, 1, wmax, 1, p[8])
+                    for q, p in enumerate(parts) if p[8]])
+            Vn = 1.5 * Vs
+            pN = Vn.data_ptr()
+            _gemms([(pG + 8 * int(go[q]), pV + 8 * vb[q] * wmax, pN + 8 * vb[q] * wmax,
+                     p[8], p[3], p[8], 1, p[8], wmax, 1, wmax, 1)
+                    for q, p in enumerate(parts) if p[8]], alpha=-0.5, beta=1.0)
+            Vs = Vn
+        # tail rows projected off the signal rows, twice
+        psz = [(p[2] - p[8]) * p[8] for p in parts]
+        po = np.concatenate([[0], np.cumsum(psz)[:-1]]).astype(np.int64)
+        P = torch.empty(max(int(sum(psz)), 1), dtype=torch.float64, device=dev)
+        pP, pV = P.data_ptr(), Vs.data_ptr()
+        proj = [q for q in tails if parts[q][8]]
+        for _ in range(2):
+            _gemms([(pB + 8 * (rb[q] + parts[q][8]) * wmax, pV + 8 * vb[q] * wmax,
+                     pP + 8 * int(po[q]), parts[q][2] - parts[q][8], parts[q][8], parts[q][3],
+                     wmax, 1, 1, wmax, 1, parts[q][2] - parts[q][8]) for q in proj])
+            _gemms([(pP + 8 * int(po[q]), pV + 8 * vb[q] * wmax,
+                     pB + 8 * (rb[q] + parts[q][8]) * wmax, parts[q][2] - parts[q][8],
+                     parts[q][3], parts[q][8], 1, parts[q][2] - parts[q][8], wmax, 1, wmax, 1)
+                    for q in proj], alpha=-1.0, beta=1.0)
+    # the tail Gram matrices straight into a second batched eigensolver
+    se2 = eigen.SectorEigh([parts[q][2] - parts[q][8] for q in tails], dev) if tails else None
+    if se2 is not None:
+        pA2 = se2.A.data_ptr()
+        _gemms([(pB + 8 * (rb[q] + parts[q][8]) * wmax, pB + 8 * (rb[q] + parts[q][8]) * wmax,
+                 pA2 + 8 * int(se2.a_off[t]), parts[q][2] - parts[q][8],
+                 parts[q][2] - parts[q][8], parts[q][3], wmax, 1, 1, wmax, 1,
+                 parts[q][2] - parts[q][8]) for t, q in enumerate(tails)])
+        mus = se2.eigenvalues()
+        flat = [m.reshape(-1) for m in mus]
+    else:
+        flat = []
+    ss_dev = bnorm[torch.as_tensor(sig_rows, device=dev)] if len(sig_rows) else bnorm[:0]
+    host = torch.cat(flat + [ss_dev]).cpu().numpy()
+    if se2 is not None:
+        se2.check()
+    mu_host, pos = {}, 0
+    for t, q in enumerate(tails):
+        m = parts[q][2] - parts[q][8]
+        mu_host[q] = host[pos:pos + m]
+        pos += m
+    ss_host = {}
+    for q, p in enumerate(parts):
+        ss_host[q] = host[pos:pos + p[8]]
+        pos += p[8]
+    vals, s2_all, nz_all = {}, [], []
+    for q, p in enumerate(parts):
+        j, c, n, w, b, sr, sc_, r, ks, lam_max = p
+        tail = np.zeros(0)
+        if q in mu_host:
+            m = n - ks
+            tail = np.maximum(mu_host[q][::-1][:min(r - ks, m)], 0.0)
+        sv = np.concatenate([ss_host[q] ** 2, tail])
+        vals[c] = (sv, ks)
+        s2_all.append(sv)
+        nz_all.append(np.full(len(sv), (tr.RHO_NOISE * n * tr.EPS64) ** 2 * lam_max))
+    s2 = np.concatenate(s2_all)
+    nz = np.concatenate(nz_all)
+    srt = np.argsort(-s2, kind="stable")
+    if not tr.rho_cut_resolved(s2[srt], nz[srt], trunc):
+        return None
+    keep = _global_keep_host({c: (None, np.sqrt(np.sort(v[0])[::-1])) for c, v in vals.items()},
+                             trunc)
+    plan = {}
+    for q, p in enumerate(parts):
+        c, ks = p[1], p[8]
+        k = keep.get(c, 0)
+        if k == 0:
+            continue
+        sv = vals[c][0]
+        # the kept values of a sector must be its signal values (descending)
+        # followed by its largest tail values: anything else -> per-sector path
+        sig = sv[:ks]
+        if np.any(np.diff(sig) > 0) or (len(sv) > ks and ks and sv[ks:].max() > sig[-1]):
+            return None
+        if np.sqrt(np.sort(sv)[::-1][k - 1]) <= 0.0:
+            continue
+        plan[c] = (min(k, ks), max(0, k - ks))
+    # tail vectors Q_n W (top kept columns of W, descending) for the kept tails
+    need = {q: plan[parts[q][1]][1] for q in tails if parts[q][1] in plan and plan[parts[q][1]][1]}
+    Tbuf, tb_off = None, {}
+    if need:
+        kt_all = [need.get(q, 0) for q in tails]
+        se2.vectors(kt_all)
+        pZ2 = se2.Z.data_ptr()
+        acc_t = 0
+        for q in need:
+            tb_off[parts[q][1]] = acc_t
+            acc_t += parts[q][2] * need[q]
+        Tbuf = torch.empty(acc_t, dtype=torch.float64, device=dev)
+        pT = Tbuf.data_ptr()
+        descs = []
+        for t, q in enumerate(tails):
+            if q not in need:
+                continue
+            j, c, n, w, b, sr, sc_, r, ks, lm = parts[q]
+            m, kt = n - ks, need[q]
+            zq = z_off[j] + (n - 1 - ks) * n         # Q column ks (descending)
+            zw = int(se2.a_off[t]) + (m - 1) * m     # W column m-1 (largest)
+            descs.append((pZ + 8 * zq, pZ2 + 8 * zw, pT + 8 * tb_off[c], n, kt, m,
+                          1, -n, 1, -m, 1, n))
+        _gemms(descs)
+    return plan, Tbuf, tb_off
+
+
 def split_two_site_rho(tmap, vec, trunc, absorb):
     """Device split by the sector reduced density matrices (White's original
     DMRG step): for the left sweep rho_c = M_c M_c^T gives u and s^2, and
@@ -668,6 +992,11 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
     for every sector, divide and conquer per sector, back-transform of the
     kept columns only); the truncation acts on the merged spectrum
     s = sqrt(lambda) exactly as for the SVD.  Returns (A1, A2, {c: s_c})."""
+    if (EIG_SOLVER == "sector" and not PLAIN and vec.device.type == "cuda"
+            and all(max(tmap.shape[c]) <= eigen.MAX_ORDER for c in tmap.charges)):
+        out = _split_rho_sector(tmap, vec, trunc, absorb)
+        if out is not None:
+            return out
     xl = tmap.xl
     dev = vec.device
     cat = torch.empty_like(vec)

@@ -69,6 +69,8 @@ struct mh_context {
   // Divide-and-conquer tridiagonal eigensolver (mh_tridc.hip): work buffer
   // and leaf convergence flag.
   void *dc_state = nullptr;
+  // Batched strided GEMMs (mh_batched.hip): task table buffer.
+  void *bg_state = nullptr;
   int autotune = 1;
   // Set for the duration of an allocation-free entry point: no tuning runs,
   // no tuner-table insertions and no profiling events (host or device).
@@ -180,6 +182,7 @@ int bs_coupled_copy(mh_handle h, const std::vector<int64_t> &pieces, const doubl
 // Release the eigensolver's streams, library handles and scratch.
 void eig_release(mh_handle h);
 void dc_release(mh_handle h);
+void bg_release(mh_handle h);
 int *dc_error_flag(mh_handle h);
 
 // Device address of the allocation-free entry points' static scratch array

@@ -171,6 +171,7 @@ int mh_destroy(mh_handle h) {
   if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
   mh::eig_release(h);
   mh::dc_release(h);
+  mh::bg_release(h);
   delete h;
   return MH_OK;
 }

@@ -36,6 +36,32 @@ def gemm(A, B, alpha=1.0, conjA=False, out=None, beta=0.0):
     return C
 
 
+def gemm_batched(tasks, alpha=1.0, beta=0.0):
+    """C = alpha A B + beta C for every (A, B, C) in `tasks` in one launch
+    (mh_gemm_batched): float64 device matrices of any strides (transposed
+    views included), all on one device; C must not overlap A or B."""
+    tasks = [t for t in tasks if t[2].numel() > 0]
+    if not tasks:
+        return
+    dev = tasks[0][2].device
+    desc = np.empty((len(tasks), 12), dtype=np.int64)
+    for i, (A, B, C) in enumerate(tasks):
+        M, K = A.shape
+        K2, N = B.shape
+        if K2 != K or tuple(C.shape) != (M, N):
+            raise _lib.DimensionMismatch(2, f"gemm_batched: {tuple(A.shape)} x {tuple(B.shape)} -> {tuple(C.shape)}")
+        for T in (A, B, C):
+            if T.dtype != torch.float64 or T.device != dev:
+                raise _lib.MHError(3, "gemm_batched: float64 operands on one device")
+        desc[i] = (A.data_ptr(), B.data_ptr(), C.data_ptr(), M, N, K,
+                   A.stride(0), A.stride(1), B.stride(0), B.stride(1), C.stride(0), C.stride(1))
+    e = engine(dev)
+    h = e.bind_stream()
+    _lib.raise_for(_lib.load().mh_gemm_batched(
+        h, len(tasks), desc.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), float(alpha),
+        float(beta)), h, "mh_gemm_batched")
+
+
 def as_matrix(t, nrow_legs):
     """Column-major tensor -> the (prod first legs) x (prod remaining legs)
     matrix view (TensorMap codomain/domain split)."""

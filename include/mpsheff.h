@@ -468,6 +468,15 @@ int mh_stedc_dc_batched(mh_handle h, int nmat, const int *n, const int64_t *v_of
 int mh_ormtr_batched(mh_handle h, int nmat, const int *n, const int64_t *a_off,
                      const int64_t *v_off, const double *A, const double *tau,
                      const int64_t *z_off, const int *col0, const int *ncol, double *Z);
+/* Batched strided FP64 GEMMs of unrelated shapes in one launch on the
+ * handle's stream: C_t = alpha op(A_t) op(B_t) + beta C_t for t < count (the
+ * per-sector products of the graded split, which TensorKit runs as one BLAS
+ * call per sector block).  desc: count rows of 12 int64 (host array): device
+ * pointers A, B, C, extents M, N, K, and element strides sam, sak (A[m, k]
+ * at A + m sam + k sak), sbk, sbn, scm, scn.  C is not read when beta = 0;
+ * C must not overlap A or B; tasks with M = 0 or N = 0 are skipped. */
+int mh_gemm_batched(mh_handle h, int count, const int64_t *desc, double alpha, double beta);
+
 /* Waits for the handle's stream; MH_ERR_NOCONV if a dstedc call of the last
  * mh_stedc_batched (nmat matrices) or a leaf of the last mh_stedc_dc_batched
  * failed to converge, MH_ERR_HIP if a

@@ -169,6 +169,67 @@ def test_device_split_matches_svd_split(gpu, absorb, solver, monkeypatch):
     np.testing.assert_allclose(rec[1], rec[0], atol=1e-11, rtol=0)
 
 
+def _graded_bond(seed, top_range, decades, nmin, nmax, dscale=None):
+    from finitemptools_amd import blockmps as bm
+    from finitemptools_amd import blocksparse as bs
+    from test_blocksparse import rand_space, reachable
+    rng = np.random.default_rng(seed)
+    phys = bm.hubbard_charges()
+    Sl = rand_space(rng, [(a, b) for a in range(1, 3) for b in range(1, 3)], nmin, nmax)
+    Sr = rand_space(rng, reachable(Sl, phys, 2), nmin, nmax)
+    xl = bs.BlockLayout(Sl, phys, 2, Sr)
+    tmap = bm.TwoSiteMap(xl)
+    cat = np.zeros(xl.size)
+    for c in tmap.charges:
+        R, C = tmap.shape[c]
+        r = min(R, C)
+        top = rng.uniform(*top_range)
+        s = np.logspace(top, top - decades, r)
+        U = np.linalg.qr(rng.standard_normal((R, r)))[0]
+        V = np.linalg.qr(rng.standard_normal((C, r)))[0]
+        cat[tmap.base[c]:tmap.base[c] + R * C] = ((U * s) @ V.T).ravel()
+    x = np.empty(xl.size)
+    x[tmap.perm] = cat
+    return xl, tmap, x
+
+
+@pytest.mark.parametrize("absorb", ["left", "right"])
+@pytest.mark.parametrize("case", ["noise_cut_1e-8", "noise_cut_1e-10", "D_binds"])
+def test_batched_split_matches_sector_svds(gpu, absorb, case):
+    """The batched device split (every per-sector product in a few launches,
+    both levels of the exact cut) keeps what the reference's sector SVDs keep
+    (truncation.jl:27-47 on the merged spectrum), with the same values and the
+    same truncated two-site tensor: a cutoff that falls inside the noise of
+    sqrt(lambda) (second level, no fallback) and a cut where D binds."""
+    import torch
+    from finitemptools_amd import blockmps as bm
+    from finitemptools_amd import blocksparse as bs
+    from finitemptools_amd import truncation as tr_mod
+    from finitemptools_amd.truncation import MPSTruncation
+    from plan_exec import execute
+    if case == "D_binds":
+        xl, tmap, x = _graded_bond(14, (-0.3, 0.0), 12, 4, 9)
+        tr = MPSTruncation(D=40, eps=1e-8)
+    else:
+        xl, tmap, x = _graded_bond(13, (-1.0, 0.0), 13, 3, 8)
+        tr = MPSTruncation(D=10 ** 6, eps=float(case.split("_")[-1]))
+    a1, a2, s_ref = bm.split_two_site(tmap, x, tr, absorb, "cpu")
+    out = bm._split_rho_sector(tmap, torch.as_tensor(x).cuda(), tr, absorb)
+    assert out is not None, "the batched split handed the bond to the per-sector path"
+    b1, b2, s_dev = out
+    assert sorted(s_ref) == sorted(s_dev)
+    smax = max(float(np.max(v)) for v in s_ref.values())
+    for c in s_ref:
+        assert len(s_ref[c]) == len(s_dev[c])
+        assert np.allclose(s_ref[c], s_dev[c], rtol=1e-10, atol=50 * tr_mod.EPS64 * smax)
+    rec = []
+    for A1, A2 in ((a1, a2), (b1, b2)):
+        tab = bs.plan_two_site(A1.layout, A2.layout, xl)
+        rec.append(execute(tab, [A1.data.cpu().numpy(), np.zeros(xl.size), A2.data.cpu().numpy(),
+                                 None, None, None])[1])
+    assert np.allclose(rec[0], rec[1], atol=1e-11)
+
+
 def test_block_dmrg_with_sector_eigensolver_matches_ed(gpu, monkeypatch):
     """Two-site DMRG of the Hubbard chain (L = 8, half filling, random start)
     with every bond split by the batched sector eigensolver reaches the
