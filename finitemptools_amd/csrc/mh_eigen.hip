@@ -234,6 +234,8 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_group(SyTable tab, double 
 //     counter, wavefront 0 polls all of them), not at one contended atomic.
 constexpr int SL_KR = 16;                 // rows per lane
 constexpr int SL_MAXN = 64 * SL_KR;       // largest order of the LDS form
+constexpr int SL_THREADS = 512;           // 8 wavefronts share a workgroup's columns
+constexpr int SL_WAVES = SL_THREADS / 64;
 
 struct SlTask {
   int64_t a_off, v_off, x_off;  // matrix, D/E/tau, exchange rows
@@ -315,7 +317,7 @@ __device__ inline void flag_barrier(unsigned *flags, int G, int g, unsigned step
   __syncthreads();
 }
 
-__global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restrict__ tabv, int nt,
+__global__ __launch_bounds__(SL_THREADS) void k_sytrd_lds(const SlTask *__restrict__ tabv, int nt,
                                                           double *__restrict__ A,
                                                           double *__restrict__ D,
                                                           double *__restrict__ E,
@@ -351,11 +353,11 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
   for (int q = 0; q < ncl; ++q) {
     const int j = g + q * G;
     if (j >= n) break;
-    for (int r = threadIdx.x; r < np; r += SY_THREADS)
+    for (int r = threadIdx.x; r < np; r += SL_THREADS)
       cols[(size_t)q * np + r] = (r < n) ? a[r + (int64_t)j * n] : 0.0;
   }
   if (g == 0)
-    for (int r = threadIdx.x; r < n; r += SY_THREADS) xstore(B0 + r, a[r]);
+    for (int r = threadIdx.x; r < n; r += SL_THREADS) xstore(B0 + r, a[r]);
   flag_barrier(fl, G, g, ++nbar, err);
 
   double vp[SL_KR], vv[SL_KR];  // wavefront 0: v_{i-1}, v_i
@@ -469,7 +471,7 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_lds(const SlTask *__restri
       double *pc = (i & 1) ? P1 : P0;
       double *bn = ((i + 1) & 1) ? B1 : B0;
       const int q0 = std::max(0, (i + 1 - g + G - 1) / G);
-      for (int q = q0 + wv; q < ncl; q += SY_WAVES) {
+      for (int q = q0 + wv; q < ncl; q += SL_WAVES) {
         const int j = g + q * G;
         if (j >= n) break;
         const double vpj = sP[j], wpj = sW[j];
@@ -844,7 +846,7 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
   auto capacity = [&](size_t lds, int *cap) -> int {
     int per_cu = 0;
     MH_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_sytrd_lds,
-                                                            SY_THREADS, lds));
+                                                            SL_THREADS, lds));
     *cap = per_cu * ncu;
     return MH_OK;
   };
@@ -907,7 +909,7 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
     int g = 0;
     for (int pi = L.first; pi < L.first + L.second; ++pi) g += tab[pi].G;
     void *args[] = {&t0, &nt, &A, &D, &E, &tau, &X, &xhalf, &flags, &err};
-    MH_HIP(h, hipLaunchCooperativeKernel((const void *)k_sytrd_lds, dim3(g), dim3(SY_THREADS), args,
+    MH_HIP(h, hipLaunchCooperativeKernel((const void *)k_sytrd_lds, dim3(g), dim3(SL_THREADS), args,
                                          (unsigned)lds, h->stream));
   }
   return MH_OK;
