@@ -567,16 +567,20 @@ struct WyTask {
   int n, col0, ncol, w0;               // w0: first workgroup (k_wy_apply) or block (k_wy_t)
 };
 
-__device__ inline double refl(const double *a, int n, int i, int r) {
-  // entry r of v_i
-  return (r < i + 1) ? 0.0 : ((r == i + 1) ? 1.0 : a[r + (int64_t)i * n]);
+typedef double wy_d4 __attribute__((ext_vector_type(4)));
+
+// entry r of reflector i (zero for i outside [0, nrefl) or r outside the column)
+__device__ inline double vref(const double *a, int n, int nrefl, int i, int r) {
+  if (i >= nrefl || r >= n || r <= i) return 0.0;
+  return (r == i + 1) ? 1.0 : a[r + (int64_t)i * n];
 }
+
 
 __global__ __launch_bounds__(256) void k_wy_t(const WyTask *__restrict__ tab, int nt,
                                               const double *__restrict__ A,
                                               const double *__restrict__ tau,
                                               double *__restrict__ T) {
-  __shared__ double G[WY_NB][WY_NB + 1];
+  __shared__ double Gp[4][WY_NB][WY_NB + 1];
   __shared__ double Ts[WY_NB][WY_NB + 1];
   int lo = 0, hi = nt - 1;
   while (lo < hi) {
@@ -589,16 +593,31 @@ __global__ __launch_bounds__(256) void k_wy_t(const WyTask *__restrict__ tab, in
   const int i0 = b * WY_NB, nb = min(WY_NB, nrefl - i0);
   const double *a = A + tk.a_off;
   const double *tv = tau + tk.v_off;
-  // G[l][j] = v_l . v_j (l < j); v_j is zero above row i0 + j + 1
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  // G = V^T V on FP64 MFMA: the four waves take alternate 4-row K steps; an
+  // operand lane holds v_{i0 + li}[r] for A (V^T) and for B (V) alike
+  wy_d4 g00 = {0.0, 0.0, 0.0, 0.0}, g01 = g00, g10 = g00, g11 = g00;
+  for (int ks = wv; i0 + 1 + 4 * ks < n; ks += 4) {
+    const int r = i0 + 1 + 4 * ks + lk;
+    const double a0 = vref(a, n, nrefl, i0 + li, r);
+    const double a1 = vref(a, n, nrefl, i0 + 16 + li, r);
+    g00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, g00, 0, 0, 0);
+    g01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a1, g01, 0, 0, 0);
+    g10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a0, g10, 0, 0, 0);
+    g11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, g11, 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    Gp[wv][lk + 4 * j][li] = g00[j];
+    Gp[wv][lk + 4 * j][16 + li] = g01[j];
+    Gp[wv][16 + lk + 4 * j][li] = g10[j];
+    Gp[wv][16 + lk + 4 * j][16 + li] = g11[j];
+  }
+  __syncthreads();
   for (int e = threadIdx.x; e < WY_NB * WY_NB; e += 256) {
     const int l = e / WY_NB, j = e % WY_NB;
-    double s = 0.0;
-    if (l < j && j < nb) {
-      const int il = i0 + l, ij = i0 + j;
-      s = refl(a, n, il, ij + 1);  // v_j[ij+1] = 1
-      for (int r = ij + 2; r < n; ++r) s += a[r + (int64_t)il * n] * a[r + (int64_t)ij * n];
-    }
-    G[l][j] = s;
+    Gp[0][l][j] = Gp[0][l][j] + Gp[1][l][j] + Gp[2][l][j] + Gp[3][l][j];
     Ts[l][j] = 0.0;
   }
   __syncthreads();
@@ -608,7 +627,7 @@ __global__ __launch_bounds__(256) void k_wy_t(const WyTask *__restrict__ tab, in
     if (threadIdx.x < j) {
       const int l = threadIdx.x;
       double s = 0.0;
-      for (int m = l; m < j; ++m) s += Ts[l][m] * G[m][j];
+      for (int m = l; m < j; ++m) s += Ts[l][m] * Gp[0][m][j];
       Ts[l][j] = -tj * s;
     }
     if (threadIdx.x == 0) Ts[j][j] = tj;
@@ -616,14 +635,6 @@ __global__ __launch_bounds__(256) void k_wy_t(const WyTask *__restrict__ tab, in
   }
   double *to = T + tk.t_off + (int64_t)b * WY_NB * WY_NB;
   for (int e = threadIdx.x; e < WY_NB * WY_NB; e += 256) to[e] = Ts[e / WY_NB][e % WY_NB];
-}
-
-typedef double wy_d4 __attribute__((ext_vector_type(4)));
-
-// entry r of reflector i (zero for i outside [0, nrefl) or r outside the column)
-__device__ inline double vref(const double *a, int n, int nrefl, int i, int r) {
-  if (i >= nrefl || r >= n || r <= i) return 0.0;
-  return (r == i + 1) ? 1.0 : a[r + (int64_t)i * n];
 }
 
 // One workgroup per WY_CT-column tile of Z (all n rows in LDS, row-major),

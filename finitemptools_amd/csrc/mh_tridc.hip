@@ -78,74 +78,76 @@ __device__ inline double wave_sum(double v) {
 }
 
 // ---- leaves: implicit QL (tql2) on an order <= 32 tridiagonal -------------
-// One wavefront per leaf.  Every lane runs the same scalar recurrence (d, e
-// in LDS, identical values written by all lanes, a barrier after each write
-// group); lane r keeps row r of the eigenvector matrix.  The leaf also clears
-// its row band of Z (the columns outside its block), so that Z is block
-// diagonal for the merges above it.
+// One wavefront per leaf.  Lane r holds d[r] and e[r] (e[r] couples r and
+// r + 1) in registers; the scalar recurrence reads them with readlane (the
+// index is uniform) and writes them back with a lane-select, so the whole
+// wavefront runs the same scalar code without touching LDS.  Lane r keeps
+// row r of the eigenvector matrix in LDS and applies every plane rotation to
+// it.  The leaf also clears its row band of Z (the columns outside its
+// block), so that Z is block diagonal for the merges above it.
+__device__ inline double rdl(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __global__ __launch_bounds__(64) void k_dc_leaf(const DcLeaf *__restrict__ lv,
                                                 double *__restrict__ D,
                                                 const double *__restrict__ E,
                                                 double *__restrict__ Z, int *__restrict__ err) {
-  __shared__ double sd[DC_LEAF], se[DC_LEAF];
   __shared__ double sz[DC_LEAF][DC_LEAF + 1];
   __shared__ int srank[DC_LEAF];
   const DcLeaf L = lv[blockIdx.x];
   const int n = L.sz, r = threadIdx.x;
   const double *e = E + L.v_off;
   double *d = D + L.v_off;
+  double dr = 0.0, er = 0.0;
   if (r < n) {
-    double di = d[L.off + r];
+    dr = d[L.off + r];
     // the tears of the merges above: |beta| off both touching diagonals
-    if (r == 0 && L.off > 0) di -= fabs(e[L.off - 1]);
-    if (r == n - 1 && L.off + n < L.ld) di -= fabs(e[L.off + n - 1]);
-    sd[r] = di;
-    se[r] = (r < n - 1) ? e[L.off + r] : 0.0;
+    if (r == 0 && L.off > 0) dr -= fabs(e[L.off - 1]);
+    if (r == n - 1 && L.off + n < L.ld) dr -= fabs(e[L.off + n - 1]);
+    er = (r < n - 1) ? e[L.off + r] : 0.0;
     for (int c = 0; c < n; ++c) sz[r][c] = (r == c) ? 1.0 : 0.0;
   }
-  __syncthreads();
   bool bad = false;
   for (int l = 0; l < n && !bad; ++l) {
     int iter = 0, m;
     do {
-      for (m = l; m < n - 1; ++m) {
-        const double dd = fabs(sd[m]) + fabs(sd[m + 1]);
-        if (fabs(se[m]) <= DC_EPS * dd) break;
-      }
+      // m: first index >= l with a negligible coupling e[m] (n - 1 if none)
+      const double dn = __shfl_down(dr, 1);
+      const bool small = r >= l && r < n - 1 && fabs(er) <= DC_EPS * (fabs(dr) + fabs(dn));
+      const unsigned long long mask = __ballot(small);
+      m = mask ? (int)__ffsll((long long)mask) - 1 : n - 1;
       if (m != l) {
         if (iter++ == DC_QL_MAXIT) {
           bad = true;
           break;
         }
-        double g = (sd[l + 1] - sd[l]) / (2.0 * se[l]);
+        const double dl = rdl(dr, l), el = rdl(er, l);
+        double g = (rdl(dr, l + 1) - dl) / (2.0 * el);
         double rr = hypot(g, 1.0);
-        g = sd[m] - sd[l] + se[l] / (g + copysign(rr, g));
+        g = rdl(dr, m) - dl + el / (g + copysign(rr, g));
         double s = 1.0, c = 1.0, p = 0.0;
-        int i;
         bool underflow = false;
-        for (i = m - 1; i >= l; --i) {
-          const double f = s * se[i], b = c * se[i];
+        for (int i = m - 1; i >= l; --i) {
+          const double ei = rdl(er, i);
+          const double f = s * ei, b = c * ei;
           rr = hypot(f, g);
-          __syncthreads();
-          se[i + 1] = rr;
-          __syncthreads();
+          er = (r == i + 1) ? rr : er;
           if (rr == 0.0) {
-            const double t = sd[i + 1] - p;
-            __syncthreads();
-            sd[i + 1] = t;
-            se[m] = 0.0;
-            __syncthreads();
+            dr = (r == i + 1) ? dr - p : dr;
+            er = (r == m) ? 0.0 : er;
             underflow = true;
             break;
           }
           s = f / rr;
           c = g / rr;
-          g = sd[i + 1] - p;
-          rr = (sd[i] - g) * s + 2.0 * c * b;
+          g = rdl(dr, i + 1) - p;
+          rr = (rdl(dr, i) - g) * s + 2.0 * c * b;
           p = s * rr;
-          __syncthreads();
-          sd[i + 1] = g + p;
-          __syncthreads();
+          dr = (r == i + 1) ? g + p : dr;
           g = c * rr - b;
           if (r < n) {
             const double zf = sz[r][i + 1], zi = sz[r][i];
@@ -154,12 +156,9 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const DcLeaf *__restrict__ lv,
           }
         }
         if (underflow) continue;
-        const double dl = sd[l] - p;
-        __syncthreads();
-        sd[l] = dl;
-        se[l] = g;
-        se[m] = 0.0;
-        __syncthreads();
+        dr = (r == l) ? dr - p : dr;
+        er = (r == l) ? g : er;
+        er = (r == m) ? 0.0 : er;
       }
     } while (m != l);
   }
@@ -168,12 +167,14 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const DcLeaf *__restrict__ lv,
     return;  // uniform: every lane took the same path
   }
   // ascending order: rank of each eigenvalue, ties broken by index
+  int k = 0;
+  for (int j = 0; j < n; ++j) {
+    const double dj = rdl(dr, j);
+    k += (dj < dr) || (dj == dr && j < r);
+  }
   if (r < n) {
-    const double v = sd[r];
-    int k = 0;
-    for (int j = 0; j < n; ++j) k += (sd[j] < v) || (sd[j] == v && j < r);
     srank[r] = k;
-    d[L.off + k] = v;
+    d[L.off + k] = dr;
   }
   __syncthreads();
   if (r < n) {
