@@ -44,6 +44,7 @@ EXTRA_FLAGS = {
     "mh_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
     "mh_gemm_mixed.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
     "mh_blocksparse.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+    "mh_batched.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
 }
 
 

@@ -7,8 +7,7 @@
 // every operand (a transposed operand is a stride swap), given by device
 // pointers; the host uploads the task table once and the grid covers every
 // task's 64 x 64 output tiles.  Each workgroup stages 64 x 16 panels of A and
-// B in LDS and keeps a 4 x 4 block of C per thread (the flops per bond are
-// small; what the batching removes is the launch count).
+// B in LDS and runs them on FP64 MFMA.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,10 +27,17 @@ struct BgTask {
   int tm, pad;         // tiles along M
 };
 
+typedef double bg_d4 __attribute__((ext_vector_type(4)));
+
+// 64 x 64 output tile per workgroup on FP64 MFMA (16x16x4): wave w owns the
+// 32 x 32 quarter (w % 2, w / 2) as 2 x 2 MFMA tiles; 16-deep K panels of A
+// and B are staged in LDS from any strides.  MFMA operand lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; accumulator register q of lane l is
+// C[l/16 + 4q][l%16].
 __global__ __launch_bounds__(256) void k_gemm_batched(const BgTask *__restrict__ tab, int nt,
                                                       double alpha, double beta) {
-  __shared__ double As[16][64 + 1];
-  __shared__ double Bs[16][64 + 1];
+  __shared__ double As[16][64 + 2];
+  __shared__ double Bs[16][64 + 2];
   int lo = 0, hi = nt - 1;
   while (lo < hi) {
     const int md = (lo + hi + 1) >> 1;
@@ -40,47 +46,50 @@ __global__ __launch_bounds__(256) void k_gemm_batched(const BgTask *__restrict__
   const BgTask t = tab[lo];
   const int tile = (int)blockIdx.x - t.tile0;
   const int r0 = (tile % t.tm) * 64, c0 = (tile / t.tm) * 64;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4] = {};
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+  bg_d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = bg_d4{0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < t.K; k0 += 16) {
     for (int e = threadIdx.x; e < 16 * 64; e += 256) {
       const int kk = e >> 6, rr = e & 63;
-      const int gk = k0 + kk, gm = r0 + rr;
+      const int gk = k0 + kk, gm = r0 + rr, gn = c0 + rr;
       As[kk][rr] = (gk < t.K && gm < t.M) ? t.A[gm * t.sam + gk * t.sak] : 0.0;
-    }
-    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-      const int kk = e >> 6, cc = e & 63;
-      const int gk = k0 + kk, gn = c0 + cc;
-      Bs[kk][cc] = (gk < t.K && gn < t.N) ? t.B[gk * t.sbk + gn * t.sbn] : 0.0;
+      Bs[kk][rr] = (gk < t.K && gn < t.N) ? t.B[gk * t.sbk + gn * t.sbn] : 0.0;
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      double a[4], b[4];
+    for (int s = 0; s < 4; ++s) {
+      double a[2], b[2];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        a[q] = As[kk][tx + 16 * q];
-        b[q] = Bs[kk][ty + 16 * q];
+      for (int i = 0; i < 2; ++i) {
+        a[i] = As[4 * s + lk][wm + 16 * i + li];
+        b[i] = Bs[4 * s + lk][wn + 16 * i + li];
       }
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[p][q] = fma(a[p], b[q], acc[p][q]);
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int gn = c0 + ty + 16 * q;
-    if (gn >= t.N) continue;
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const int gm = r0 + tx + 16 * p;
-      if (gm >= t.M) continue;
-      double *c = t.C + gm * t.scm + gn * t.scn;
-      *c = (beta == 0.0) ? alpha * acc[p][q] : alpha * acc[p][q] + beta * *c;
-    }
-  }
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int gm = r0 + wm + 16 * i + lk + 4 * q, gn = c0 + wn + 16 * j + li;
+        if (gm < t.M && gn < t.N) {
+          double *c = t.C + gm * t.scm + gn * t.scn;
+          *c = (beta == 0.0) ? alpha * acc[i][j][q] : alpha * acc[i][j][q] + beta * *c;
+        }
+      }
 }
 
 struct BgState {
