@@ -4,10 +4,10 @@
 //
 //   mh_sytrd_batched : Householder tridiagonalisation of every matrix of a
 //                      batch in ONE cooperative launch.  Matrix m gets G_m
-//                      workgroups that own its columns cyclically and meet at
-//                      a group barrier once per column, so a step costs one
-//                      barrier instead of the handful of launches a
-//                      library reduction spends per column.  Output follows
+//                      workgroups that own its columns cyclically and
+//                      exchange one vector and one column per step, instead
+//                      of the handful of launches a library reduction spends
+//                      per column.  Output follows
 //                      LAPACK dsytrd (uplo = 'L'): D, E, tau, and reflector
 //                      m's vector v_i (v_i[i+1] = 1 implicit) in A[i+2:, i].
 //   mh_stedc_batched : eigen-decomposition of the tridiagonals by rocSOLVER's
@@ -224,23 +224,30 @@ __global__ __launch_bounds__(SY_THREADS) void k_sytrd_group(SyTable tab, double 
 
 // The same reduction with every workgroup's columns resident in LDS for the
 // whole launch (orders up to SL_MAXN).  The step is the one above; what
-// changes is where the data lives:
+// changes is where the data lives and how the workgroups meet:
 //   - a workgroup keeps its columns (cyclic, j = g + q G) in LDS, so the lazy
 //     rank-2 update and the dot products of a step touch no global memory;
 //   - wavefront 0 forms w_{i-1}, the column i of A^(i) and v_i in registers
 //     (lane l holds rows l + 64 k) and hands them to the other waves through
 //     LDS; only p and the next column travel through global memory;
-//   - the workgroups of a matrix meet at a flag barrier (each writes its own
-//     counter, wavefront 0 polls all of them), not at one contended atomic.
-constexpr int SL_KR = 16;                 // rows per lane
-constexpr int SL_MAXN = 64 * SL_KR;       // largest order of the LDS form
+//   - every step has exchange slots of its own (p_i and the column i+1 of
+//     A^(i+1)), filled with an "empty" bit pattern before the launch; a
+//     reader polls the rows it needs until none is empty.  Data arrival is
+//     the only synchronisation between workgroups: no barrier, no flag, and
+//     a writer never waits for its stores to be acknowledged.
+constexpr int SL_KRMAX = 18;              // rows per lane, largest template
+constexpr int SL_MAXN = 64 * SL_KRMAX;    // largest order of the LDS form
 constexpr int SL_THREADS = 512;           // 8 wavefronts share a workgroup's columns
 constexpr int SL_WAVES = SL_THREADS / 64;
+// An exchange slot not yet written: a signalling NaN.  Arithmetic only ever
+// produces the canonical quiet NaN, and the one raw copy (column 0) is
+// canonicalised, so no published value has this pattern.
+constexpr long long SL_EMPTY = 0x7FF4DEAD7FF4DEADLL;
 
 struct SlTask {
-  int64_t a_off, v_off, x_off;  // matrix, D/E/tau, exchange rows
+  int64_t a_off, v_off, x_off;  // matrix, D/E/tau, exchange slots (2 n xs doubles)
   int n, G, g0, ncl;            // order, workgroups, first workgroup, LDS columns each
-  int f_off, np;                // first barrier flag, order padded to a multiple of 64
+  int xs, pad;                  // exchange slot length: order padded to a multiple of 64
 };
 
 // A double moved between lanes by one DPP move of each 32-bit half.
@@ -276,235 +283,336 @@ __device__ inline double wave_sum64(double v) {
 }
 
 // value at row idx of a lane-distributed vector (lane idx % 64, slot idx / 64)
-__device__ inline double row_value(const double (&x)[SL_KR], int idx) {
+template <int KR>
+__device__ inline double row_value(const double (&x)[KR], int idx) {
   const int kk = idx >> 6;
   double t = 0.0;
 #pragma unroll
-  for (int k = 0; k < SL_KR; ++k)
+  for (int k = 0; k < KR; ++k)
     if (k == kk) t = x[k];
   return lane_value(t, idx & 63);
 }
 
-// Barrier of the G workgroups of one matrix: workgroup g raises its own flag
-// to `step`; wavefront 0 polls every flag (lanes in parallel).  Everything
-// the workgroups exchange inside the launch is written with device-scope
-// (sc1) stores and read with device-scope loads, so ordering needs no cache
-// write-back or invalidation: every wavefront waits until its own stores are
-// acknowledged (s_waitcnt 0) before the workgroup barrier, and only then is
-// the flag stored.  Bounded like group_barrier: past the spin limit the error
-// flag is raised, and a raised error flag ends every later wait at once.
-__device__ inline void flag_barrier(unsigned *flags, int G, int g, unsigned step, int *err) {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    if (threadIdx.x == 0)
-      __hip_atomic_store(flags + g, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned spins = 0;
-    for (;;) {
-      bool ok = true;
-      for (int q = threadIdx.x; q < G; q += 64)
-        ok = ok && (__hip_atomic_load(flags + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= step);
-      if (__ballot(!ok) == 0) break;
-      const int e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (__ballot(e != 0) != 0) break;
-      if (++spins > SY_SPIN_LIMIT) {
-        if (threadIdx.x == 0) atomicOr(err, 1);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+__device__ inline bool sl_empty(double v) { return __double_as_longlong(v) == SL_EMPTY; }
+
+// A wavefront's read of step i's inputs: its rows (slots wv + 8 t) of
+// p_{i-1} (if with_p) and of the column i of A^(i), polled until no needed
+// row is empty.  Slots below k0 hold only finished rows and slots from kr on
+// lie past the order, so neither is read; every load stays inside the
+// matrix's slots.  Rows outside [i, n) come back as zero.  The wait is
+// bounded: past the spin limit the error flag is raised, and a raised error
+// flag ends every wait.
+template <int T>
+__device__ inline void sl_poll(const double *pp, const double *bc, bool with_p, int i, int n,
+                               int kr, int wv, int lane, double (&wp)[T], double (&bv)[T],
+                               int *err) {
+  const int k0 = i >> 6;
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const int sl = wv + SL_WAVES * t;
+    wp[t] = bv[t] = 0.0;
+    if (sl >= k0 && sl < kr) {
+      const int r = lane + 64 * sl;
+      if (with_p) wp[t] = xload(pp + r);
+      bv[t] = xload(bc + r);
     }
   }
-  __syncthreads();
+  for (unsigned spins = 0;; ++spins) {
+    unsigned miss = 0;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int sl = wv + SL_WAVES * t;
+      if (sl >= k0 && sl < kr) {
+        const int r = lane + 64 * sl;
+        const bool need = r >= i && r < n;
+        if (need && with_p && sl_empty(wp[t])) miss |= 1u << (2 * t);
+        if (need && sl_empty(bv[t])) miss |= 2u << (2 * t);
+      }
+    }
+    if (__ballot(miss != 0) == 0) break;
+    const int e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (e != 0 || spins >= SY_SPIN_LIMIT) {
+      if (e == 0 && lane == 0) atomicOr(err, 1);
+      break;
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int sl = wv + SL_WAVES * t;
+      if (sl >= k0 && sl < kr) {
+        const int r = lane + 64 * sl;
+        if (miss & (1u << (2 * t))) wp[t] = xload(pp + r);
+        if (miss & (2u << (2 * t))) bv[t] = xload(bc + r);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const int r = lane + 64 * (wv + SL_WAVES * t);
+    const bool in = r >= i && r < n;
+    wp[t] = in ? wp[t] : 0.0;
+    bv[t] = in ? bv[t] : 0.0;
+  }
 }
 
+// the element of a row-distributed vector at slot t (uniform t < T)
+template <int T>
+__device__ inline double slot_value(const double (&x)[T], int t) {
+  double v = x[0];
+#pragma unroll
+  for (int q = 1; q < T; ++q)
+    if (q == t) v = x[q];
+  return v;
+}
+
+// KR: row slots per lane of the launch's largest order; a matrix of order n
+// uses its first kr = ceil(n / 64) slots.  A workgroup's ncl columns: the
+// last 8 CW (the longest-lived) in registers, wavefront w holding columns
+// nlds + w + 8 m (m < CW) with row l + 64 k of column m in lane l, slot k;
+// the first nlds = ncl - 8 CW (if any) in LDS at stride 64 kr.
+//
+// The vector work of a step (forming w_{i-1}, the column i of A^(i), v_i and
+// tau_i) is spread over the eight wavefronts by rows: wavefront w holds the
+// row slots w + 8 t (lane l: row l + 64 slot), reads its rows of the exchange
+// slots itself and keeps its rows of v in registers; the two dot products
+// meet in LDS at the workgroup barriers B1 and B2, in a fixed order, so every
+// workgroup of the matrix computes the same bits.  After B3 the columns are
+// processed one per wavefront: step i-1's rank-2 update and the dot product
+// with v_i in one pass.
+template <int KR, int CW>
 __global__ __launch_bounds__(SL_THREADS) void k_sytrd_lds(const SlTask *__restrict__ tabv, int nt,
                                                           double *__restrict__ A,
                                                           double *__restrict__ D,
                                                           double *__restrict__ E,
                                                           double *__restrict__ tau,
-                                                          double *__restrict__ X, int64_t xhalf,
-                                                          unsigned *__restrict__ flags,
+                                                          double *__restrict__ X,
                                                           int *__restrict__ err) {
+  constexpr int T = (KR + SL_WAVES - 1) / SL_WAVES;  // row slots per lane of a wavefront
   extern __shared__ double sm[];
-  __shared__ double scal[2];
+  __shared__ double red[2][SL_WAVES];
+  __shared__ double sc[4];  // v_{i-1}[i], p_{i-1}[i], x[i], x[i+1]
   int lo = 0, hi = nt - 1;
   while (lo < hi) {
     const int md = (lo + hi + 1) >> 1;
     if (tabv[md].g0 <= (int)blockIdx.x) lo = md; else hi = md - 1;
   }
   const SlTask tk = tabv[lo];
-  const int n = tk.n, np = tk.np, G = tk.G, g = (int)blockIdx.x - tk.g0, ncl = tk.ncl;
-  const int kr = np >> 6;  // row slots per lane in use
+  const int n = tk.n, G = tk.G, g = (int)blockIdx.x - tk.g0, ncl = tk.ncl, xs = tk.xs;
+  const int kr = xs >> 6;  // slots in use: rows padded to xs = 64 kr
+  const int np = xs;
+  const int nlds = std::max(0, ncl - SL_WAVES * CW);  // columns kept in LDS
   double *a = A + tk.a_off;
-  // LDS: ncl columns of np rows (rows n..np-1 zero), v_i, w_{i-1}, v_{i-1}
-  // (np each)
+  // LDS: nlds columns of np rows (rows n..np-1 zero), w_{i-1}, and a ring of
+  // two v vectors (v_i at ring slot i & 1)
   double *cols = sm;
-  double *sV = sm + (size_t)ncl * np;
-  double *sW = sV + np;
-  double *sP = sW + np;
-  // exchange rows (np per matrix): p of even / odd steps, next column of
-  // even / odd steps
-  double *const P0 = X + tk.x_off, *const P1 = P0 + xhalf;
-  double *const B0 = P1 + xhalf, *const B1 = B0 + xhalf;
-  unsigned *fl = flags + tk.f_off;
-  unsigned nbar = 0;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double *sW = sm + (size_t)nlds * np;
+  double *sR = sW + np;
+  // exchange slots: column i of A^(i) at xb + 2 i xs, p_i at xb + (2 i + 1) xs
+  double *const xb = X + tk.x_off;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 
-  for (int q = 0; q < ncl; ++q) {
+  for (int q = 0; q < nlds; ++q) {
     const int j = g + q * G;
     if (j >= n) break;
     for (int r = threadIdx.x; r < np; r += SL_THREADS)
       cols[(size_t)q * np + r] = (r < n) ? a[r + (int64_t)j * n] : 0.0;
   }
-  if (g == 0)
-    for (int r = threadIdx.x; r < n; r += SL_THREADS) xstore(B0 + r, a[r]);
-  flag_barrier(fl, G, g, ++nbar, err);
-
-  double vp[SL_KR], vv[SL_KR];  // wavefront 0: v_{i-1}, v_i
+  double colr[CW > 0 ? CW : 1][KR];  // (with CW = 0 unused)
 #pragma unroll
-  for (int k = 0; k < SL_KR; ++k) vp[k] = vv[k] = 0.0;
+  for (int m = 0; m < CW; ++m) {
+    const int q = nlds + wv + SL_WAVES * m;
+    const int j = g + q * G;
+    const bool live = q < ncl && j < n;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const int r = lane + 64 * k;
+      colr[m][k] = (live && r < n) ? a[r + (int64_t)j * n] : 0.0;
+    }
+  }
+  for (int r = threadIdx.x; r < 3 * np; r += SL_THREADS) sW[r] = 0.0;
+  if (g == 0)
+    for (int r = threadIdx.x; r < n; r += SL_THREADS)
+      xstore(xb + r, __builtin_canonicalize(a[r]));
+  __syncthreads();
+
+  double vpr[T];  // own rows of v_{i-1}
+#pragma unroll
+  for (int t = 0; t < T; ++t) vpr[t] = 0.0;
   double tau_prev = 0.0;
   for (int i = 0; i < n; ++i) {
     const bool last = (i == n - 1);
-    if (wv == 0) {
-      const double *bc = (i & 1) ? B1 : B0;
-      const double *pp = ((i - 1) & 1) ? P1 : P0;
-      // p_{i-1} and the column i of A^(i-1) in one round trip; every load
-      // lies inside the matrix's padded exchange rows, the mask selects
-      double wp[SL_KR], bcv[SL_KR];
-      const int k0 = i >> 6;  // slots below k0 hold only finished rows
+    const int k0 = i >> 6;  // slots below k0 hold only finished rows
+    double *const vprev = sR + ((i + 1) & 1) * np;  // v_{i-1}
+    double *const vcur = sR + (i & 1) * np;         // v_i
+    const int si = i >> 6, si1 = (i + 1) >> 6;      // slots of rows i and i + 1
+    const bool row_i = (wv == (si & 7)) && lane == (i & 63);
+    const bool row_i1 = (wv == (si1 & 7)) && lane == ((i + 1) & 63);
+    double wpr[T], bcr[T];
+    {
+      const double *bc = xb + 2 * (int64_t)i * xs;
+      sl_poll<T>(bc - xs, bc, i > 0, i, n, kr, wv, lane, wpr, bcr, err);
+    }
+    {
+      double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < SL_KR; ++k) {
-        wp[k] = bcv[k] = 0.0;
-        if (k >= k0 && k < kr) {
-          const int r = lane + 64 * k;
-          const double pv = (i > 0) ? xload(pp + r) : 0.0;
-          const double bv = xload(bc + r);
-          const bool in = r >= i && r < n;
-          wp[k] = in ? pv : 0.0;
-          bcv[k] = in ? bv : 0.0;
-        }
+      for (int t = 0; t < T; ++t) s += wpr[t] * vpr[t];
+      s = wave_sum64(s);
+      if (lane == 0) red[0][wv] = s;
+      if (row_i) {
+        sc[0] = slot_value(vpr, si >> 3);
+        sc[1] = slot_value(wpr, si >> 3);
       }
-      if (i > 0) {
-        double s = 0.0;
+    }
+    __syncthreads();  // B1
+    double xr[T];
+    {
+      double S = 0.0;
 #pragma unroll
-        for (int k = 0; k < SL_KR; ++k) s += wp[k] * vp[k];
-        const double hh = 0.5 * tau_prev * wave_sum64(s);
-#pragma unroll
-        for (int k = 0; k < SL_KR; ++k) wp[k] -= hh * vp[k];
-      }
-      const double vpi = row_value(vp, i), wpi = row_value(wp, i);
-      double x[SL_KR];
+      for (int w = 0; w < SL_WAVES; ++w) S += red[0][w];
+      const double hh = (i > 0) ? 0.5 * tau_prev * S : 0.0;
+      const double vpi = sc[0], wpi = sc[1] - hh * sc[0];
       double s2 = 0.0;
 #pragma unroll
-      for (int k = 0; k < SL_KR; ++k) {
-        const int r = lane + 64 * k;
-        x[k] = bcv[k] - vp[k] * wpi - wp[k] * vpi;  // zero outside [i, n)
-        if (r >= i + 2) s2 += x[k] * x[k];
+      for (int t = 0; t < T; ++t) {
+        const int sl = wv + SL_WAVES * t;
+        const int r = lane + 64 * sl;
+        wpr[t] -= hh * vpr[t];                             // w_{i-1}
+        xr[t] = bcr[t] - vpr[t] * wpi - wpr[t] * vpi;      // zero outside [i, n)
+        if (r >= i + 2) s2 += xr[t] * xr[t];
+        if (sl >= k0 && sl < kr) sW[r] = wpr[t];
       }
-      const double xn2 = wave_sum64(s2);
-      const double di = row_value(x, i);
-      double beta = 0.0, tu = 0.0;
-      const bool own = (i % G == g);
+      s2 = wave_sum64(s2);
+      if (lane == 0) red[1][wv] = s2;
+      if (row_i) sc[2] = slot_value(xr, si >> 3);
+      if (row_i1) sc[3] = slot_value(xr, si1 >> 3);
+    }
+    __syncthreads();  // B2
+    double tu = 0.0;
+    {
+      double xn2 = 0.0;
+#pragma unroll
+      for (int w = 0; w < SL_WAVES; ++w) xn2 += red[1][w];
+      const double di = sc[2];
+      double beta = 0.0, scale = 1.0;
       if (!last) {
-        const double alpha = row_value(x, i + 1);
-        double scale = 1.0;
+        const double alpha = sc[3];
         beta = alpha;
         if (xn2 > 0.0) {
           beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
           tu = (beta - alpha) / beta;
           scale = 1.0 / (alpha - beta);
         }
-#pragma unroll
-        for (int k = 0; k < SL_KR; ++k) {
-          const int r = lane + 64 * k;
-          vv[k] = (r >= i + 2) ? x[k] * scale : ((r == i + 1) ? 1.0 : 0.0);
-          if (k < kr) {
-            sV[r] = vv[k];
-            sW[r] = wp[k];
-            sP[r] = vp[k];
-          }
-        }
-        if (own) {
-#pragma unroll
-          for (int k = 0; k < SL_KR; ++k) {
-            const int r = lane + 64 * k;
-            if (r >= i + 2 && r < n) a[r + (int64_t)i * n] = vv[k];
-          }
-          if (lane == 0) {
-            a[i + (int64_t)i * n] = di;
-            a[i + 1 + (int64_t)i * n] = beta;
-          }
-        }
-      } else if (own && lane == 0) {
-        a[i + (int64_t)i * n] = di;
       }
-      if (lane == 0) {
+      const bool own = (i % G == g);
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int sl = wv + SL_WAVES * t;
+        const int r = lane + 64 * sl;
+        const double vv = (r >= i + 2) ? xr[t] * scale : ((r == i + 1) ? 1.0 : 0.0);
+        if (sl >= k0 && sl < kr) vcur[r] = vv;
+        if (own && r >= i + 2 && r < n) a[r + (int64_t)i * n] = vv;
+        vpr[t] = vv;
+      }
+      if (wv == 0 && lane == 0) {
+        if (own) {
+          a[i + (int64_t)i * n] = di;
+          if (!last) a[i + 1 + (int64_t)i * n] = beta;
+        }
         if (g == 0) {
           D[tk.v_off + i] = di;
           E[tk.v_off + i] = beta;
           tau[tk.v_off + i] = tu;
         }
-        scal[0] = tu;
       }
     }
-    __syncthreads();
+    __syncthreads();  // B3: v_i and w_{i-1} in LDS
     if (last) break;
-    const double tu = scal[0];
+    tau_prev = tu;
     {
-      // step i-1's rank-2 update of the own columns and their p_i entries,
-      // branch-free over the padded rows: v_{i-1} and w_{i-1} vanish above
-      // row i and v_i above row i + 1, so rows that are already final are
-      // left as they are or touched only where no later step reads them, and
-      // the zero padding stays zero
-      double vpB[SL_KR], wpB[SL_KR], vcB[SL_KR];
+      // step i-1's rank-2 update of the own columns and their p_i entries
+      // over the slots from k0 on: v_{i-1} and w_{i-1} vanish above row i and
+      // v_i above row i + 1, so the rows of the slots below are final and
+      // contribute nothing; the zero padding stays zero
+      double vpB[KR], wpB[KR], vcB[KR];
 #pragma unroll
-      for (int k = 0; k < SL_KR; ++k) {
+      for (int k = 0; k < KR; ++k) {
         vpB[k] = wpB[k] = vcB[k] = 0.0;
-        if (k < kr) {
+        if (k >= k0 && k < kr) {
           const int r = lane + 64 * k;
-          vpB[k] = sP[r];
+          vpB[k] = vprev[r];
           wpB[k] = sW[r];
-          vcB[k] = sV[r];
+          vcB[k] = vcur[r];
         }
       }
-      double *pc = (i & 1) ? P1 : P0;
-      double *bn = ((i + 1) & 1) ? B1 : B0;
+      double *pc = xb + (2 * (int64_t)i + 1) * xs;
+      double *bn = pc + xs;
+      // register columns: no LDS traffic; finished slots see zero factors
+#pragma unroll
+      for (int m = 0; m < CW; ++m) {
+        const int q = nlds + wv + SL_WAVES * m;
+        const int j = g + q * G;
+        if (q < ncl && j > i && j < n) {
+          const double vpj = vprev[j], wpj = sW[j];
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < KR; ++k) {
+            colr[m][k] = colr[m][k] - vpB[k] * wpj - wpB[k] * vpj;
+            acc += colr[m][k] * vcB[k];
+          }
+          if (j == i + 1) {  // the next step's column, for every workgroup
+#pragma unroll
+            for (int k = 0; k < KR; ++k) {
+              const int r = lane + 64 * k;
+              if (k >= k0 && k < kr && r >= i + 1 && r < n) xstore(bn + r, colr[m][k]);
+            }
+          }
+          acc = wave_sum64(acc);
+          if (lane == 0) xstore(pc + j, tu * acc);
+        }
+      }
       const int q0 = std::max(0, (i + 1 - g + G - 1) / G);
-      for (int q = q0 + wv; q < ncl; q += SL_WAVES) {
+      for (int q = q0 + wv; q < nlds; q += SL_WAVES) {
         const int j = g + q * G;
         if (j >= n) break;
-        const double vpj = sP[j], wpj = sW[j];
+        const double vpj = vprev[j], wpj = sW[j];
         double *col = cols + (size_t)q * np;
-        double xr[SL_KR];
+        // loads, arithmetic and stores in separate passes: the arithmetic
+        // has no branches (finished slots compute zeros), so the loads are
+        // all in flight before the first wait and no store waits on another
+        double xc[KR];
 #pragma unroll
-        for (int k = 0; k < SL_KR; ++k)
-          if (k < kr) xr[k] = col[lane + 64 * k];
+        for (int k = 0; k < KR; ++k) {
+          xc[k] = 0.0;
+          if (k >= k0 && k < kr) xc[k] = col[lane + 64 * k];
+        }
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < SL_KR; ++k) {
-          if (k < kr) {
-            xr[k] = xr[k] - vpB[k] * wpj - wpB[k] * vpj;
-            col[lane + 64 * k] = xr[k];
-            acc += xr[k] * vcB[k];
-          }
+        for (int k = 0; k < KR; ++k) {
+          xc[k] = xc[k] - vpB[k] * wpj - wpB[k] * vpj;
+          acc += xc[k] * vcB[k];
         }
+#pragma unroll
+        for (int k = 0; k < KR; ++k)
+          if (k >= k0 && k < kr) col[lane + 64 * k] = xc[k];
         if (j == i + 1) {  // the next step's column, for every workgroup
 #pragma unroll
-          for (int k = 0; k < SL_KR; ++k) {
+          for (int k = 0; k < KR; ++k) {
             const int r = lane + 64 * k;
-            if (k < kr && r >= i + 1 && r < n) xstore(bn + r, xr[k]);
+            if (k >= k0 && k < kr && r >= i + 1 && r < n) xstore(bn + r, xc[k]);
           }
         }
         acc = wave_sum64(acc);
         if (lane == 0) xstore(pc + j, tu * acc);
       }
     }
-#pragma unroll
-    for (int k = 0; k < SL_KR; ++k) vp[k] = vv[k];
-    tau_prev = tu;
-    flag_barrier(fl, G, g, ++nbar, err);
   }
+}
+
+// fill the exchange slots with the empty pattern
+__global__ void k_sl_fill(long long *__restrict__ x, size_t len) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < len;
+       e += (size_t)gridDim.x * blockDim.x)
+    x[e] = SL_EMPTY;
 }
 
 // z <- Q z for selected columns, Q = H_0 H_1 ... H_{n-2}; one wavefront per
@@ -814,12 +922,29 @@ void eig_release(mh_handle h) {
   h->eig_bytes = 0;
 }
 
-// Plan and launch k_sytrd_lds: every matrix gets G workgroups holding
-// ceil(n / G) columns each, within an LDS budget per workgroup; the budget
-// is the largest (fewest workgroups, cheapest barriers) for which every
-// workgroup of the batch is resident at once.  If no budget admits one
-// launch, the matrices go, largest first, into consecutive launches that each
-// fit at the largest budget.
+// k_sytrd_lds for KR = 1 .. SL_KRMAX row slots per lane, with CW register
+// columns per wavefront: as many as the register file holds beside the three
+// broadcast vectors and an LDS column in flight, (CW + 4) KR doubles within
+// about 99 per lane; the table keeps every instance free of register spills
+constexpr int kSlCw[SL_KRMAX + 1] = {0, 6, 6, 6, 6, 6, 6, 6, 6, 6, 5, 4, 4, 3, 2, 2, 1, 0, 0};
+constexpr int sl_cw(int kr) { return kSlCw[kr]; }
+template <int KR>
+constexpr const void *sl_kernel() { return (const void *)k_sytrd_lds<KR, sl_cw(KR)>; }
+const void *const kSlKernels[SL_KRMAX] = {
+    sl_kernel<1>(),  sl_kernel<2>(),  sl_kernel<3>(),  sl_kernel<4>(),  sl_kernel<5>(),
+    sl_kernel<6>(),  sl_kernel<7>(),  sl_kernel<8>(),  sl_kernel<9>(),  sl_kernel<10>(),
+    sl_kernel<11>(), sl_kernel<12>(), sl_kernel<13>(), sl_kernel<14>(), sl_kernel<15>(),
+    sl_kernel<16>(), sl_kernel<17>(), sl_kernel<18>()};
+
+// Plan and launch k_sytrd_lds.  Matrix m gets G_m workgroups holding
+// ncl = ceil(n / G) columns each: up to 8 CW in registers, the rest in LDS.
+// The planner starts from the fewest workgroups the LDS budget allows and
+// hands the device's remaining workgroup slots, one at a time, to the matrix
+// with the longest estimated run (steps x per-step time, the step priced by
+// its fixed exchange and vector work plus a cost per register column and a
+// larger one per LDS column), so the largest matrices spread out most.  If
+// the batch does not fit at once, the matrices go, largest first, into
+// consecutive launches.
 int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const int64_t *v_off,
               double *A, double *D, double *E, double *tau) {
   int ncu = 0, lds_cu = 0;
@@ -827,59 +952,78 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
   MH_HIP(h, hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                                   h->device));
   if (lds_cu <= 0) lds_cu = 64 * 1024;
+  int nmax = 0;
+  for (int m = 0; m < nmat; ++m) nmax = std::max(nmax, n[m]);
+  const int KR = (nmax + 63) / 64;
+  if (KR < 1 || KR > SL_KRMAX) return MH_ERR_UNSUPPORTED;
+  const int regc = SL_WAVES * sl_cw(KR);  // register columns per workgroup
+  const void *kern = kSlKernels[KR - 1];
   std::vector<int> order(nmat);
   for (int m = 0; m < nmat; ++m) order[m] = m;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
-  std::vector<SlTask> tab(nmat);
-  // plan at budget level k: fills tab (in `order`), returns the LDS bytes
-  // per workgroup (0 if a matrix does not fit) and the workgroup total
-  auto plan = [&](int k, int *gsum) -> size_t {
-    const int64_t budget = (int64_t)(lds_cu / k) - 2048;  // columns + 3 vectors
-    size_t lds = 0;
-    int64_t xo = 0;
-    int gt = 0;
-    for (int pi = 0; pi < nmat; ++pi) {
-      const int m = order[pi];
-      const int nm = n[m];
-      const int np = (nm + 63) & ~63;
-      const int64_t capc = (budget / 8 - 3 * (int64_t)np) / np;  // columns that fit
-      if (capc < 1) return 0;
-      const int G = (int)((nm + capc - 1) / capc);
-      const int ncl = (nm + G - 1) / G;
-      tab[pi] = SlTask{a_off[m], v_off[m], xo, nm, G, 0, ncl, gt, np};
-      xo += np;
-      gt += G;
-      lds = std::max(lds, ((size_t)ncl + 3) * np * sizeof(double));
-    }
-    *gsum = gt;
-    return lds;
+  // per-step cost model (microseconds): fixed part, per register column,
+  // per LDS column
+  constexpr double kFix = 3.4, kReg = 0.02, kLds = 0.085;
+  auto est = [&](int nm, int G) {
+    const int ncl = (nm + G - 1) / G;
+    return nm * (kFix + kReg * std::min(ncl, regc) + kLds * std::max(0, ncl - regc));
   };
-  auto capacity = [&](size_t lds, int *cap) -> int {
-    int per_cu = 0;
-    MH_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_sytrd_lds,
-                                                            SL_THREADS, lds));
-    *cap = per_cu * ncu;
-    return MH_OK;
+  const int64_t budget = (int64_t)lds_cu - 2048;  // LDS columns + 3 vectors
+  std::vector<int> G(nmat);
+  std::vector<SlTask> tab(nmat);
+  int gmin = 0;
+  for (int pi = 0; pi < nmat; ++pi) {
+    const int nm = n[order[pi]];
+    const int xs = (nm + 63) & ~63;
+    const int64_t capc = (budget / 8 - 3 * (int64_t)xs) / xs + regc;
+    if (capc < 1) return MH_ERR_UNSUPPORTED;
+    G[pi] = (int)((nm + capc - 1) / capc);
+    gmin += G[pi];
+  }
+  auto lds_of = [&](int pi) {
+    const int nm = n[order[pi]];
+    const int xs = (nm + 63) & ~63;
+    const int ncl = (nm + G[pi] - 1) / G[pi];
+    return ((size_t)std::max(0, ncl - regc) + 3) * xs * sizeof(double);
   };
   size_t lds = 0;
-  int gtot = 0, cap = 0;
-  bool single = false;
-  for (int k = 1; k <= 4 && !single; ++k) {
-    lds = plan(k, &gtot);
-    if (lds == 0) continue;
-    MH_TRY(capacity(lds, &cap));
-    single = cap >= 1 && gtot <= cap;
+  for (int pi = 0; pi < nmat; ++pi) lds = std::max(lds, lds_of(pi));
+  int per_cu = 0;
+  MH_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, SL_THREADS, lds));
+  const int cap = per_cu * ncu;
+  if (cap < 1) return MH_ERR_UNSUPPORTED;
+  for (int pi = 0; pi < nmat; ++pi)
+    if (G[pi] > cap) return MH_ERR_UNSUPPORTED;
+  if (gmin <= cap) {
+    // spread: extra workgroups to the longest estimated run (only while it
+    // shortens that run; the LDS need only falls as G grows)
+    int gt = gmin;
+    while (gt < cap) {
+      int best = -1;
+      double tb = 0.0;
+      for (int pi = 0; pi < nmat; ++pi) {
+        const int nm = n[order[pi]];
+        if (G[pi] >= nm) continue;
+        const double t = est(nm, G[pi]);
+        if (t > tb && est(nm, G[pi] + 1) < t) {
+          tb = t;
+          best = pi;
+        }
+      }
+      if (best < 0) break;
+      ++G[best];
+      ++gt;
+    }
   }
-  if (!single) {
-    lds = plan(1, &gtot);
-    if (lds == 0) return MH_ERR_UNSUPPORTED;
-    MH_TRY(capacity(lds, &cap));
-    if (cap < 1) return MH_ERR_UNSUPPORTED;
-    for (const SlTask &t : tab)
-      if (t.G > cap) return MH_ERR_UNSUPPORTED;
+  int64_t xsum = 0;  // exchange doubles of the batch
+  for (int pi = 0; pi < nmat; ++pi) {
+    const int m = order[pi];
+    const int xs = (n[m] + 63) & ~63;
+    tab[pi] = SlTask{a_off[m], v_off[m], xsum, n[m], G[pi], 0, (n[m] + G[pi] - 1) / G[pi], xs, 0};
+    xsum += 2 * (int64_t)n[m] * xs;
   }
   // launches: consecutive runs of the plan whose workgroups fit at once;
-  // g0 counts from the launch's first workgroup, flags stay distinct
+  // g0 counts from the launch's first workgroup
   std::vector<std::pair<int, int>> launches;  // (first task, task count)
   {
     int first = 0, g = 0;
@@ -894,34 +1038,34 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
     }
     launches.push_back({first, nmat - first});
   }
-  int64_t xsum = 0;
-  for (int m = 0; m < nmat; ++m) xsum += (n[m] + 63) & ~63;
-  // scratch: 4 exchange halves of xsum doubles, the flags, the task table,
-  // the error flag
-  const size_t xbytes = (size_t)xsum * 4 * sizeof(double);
-  const size_t fbytes = ((size_t)gtot * sizeof(unsigned) + 255) & ~size_t(255);
+  // scratch: the exchange slots (every step of every matrix), the task
+  // table, the error flag
+  const size_t xbytes = ((size_t)xsum * sizeof(double) + 255) & ~size_t(255);
   const size_t tbytes = ((size_t)nmat * sizeof(SlTask) + 255) & ~size_t(255);
-  const size_t xalign = (xbytes + 255) & ~size_t(255);
-  MH_TRY(ensure_eig_scratch(h, xalign + fbytes + tbytes + 256));
+  MH_TRY(ensure_eig_scratch(h, xbytes + tbytes + 256));
   char *base = (char *)h->eig;
   double *X = (double *)base;
-  unsigned *flags = (unsigned *)(base + xalign);
-  SlTask *dtab = (SlTask *)(base + xalign + fbytes);
-  int *err = (int *)(base + xalign + fbytes + tbytes);
+  SlTask *dtab = (SlTask *)(base + xbytes);
+  int *err = (int *)(base + xbytes + tbytes);
   h->eig_err = err;
   MH_HIP(h, hipMemsetAsync(err, 0, sizeof(int), h->stream));
-  MH_HIP(h, hipMemsetAsync(flags, 0, (size_t)gtot * sizeof(unsigned), h->stream));
   MH_HIP(h, hipMemcpyAsync(dtab, tab.data(), (size_t)nmat * sizeof(SlTask), hipMemcpyHostToDevice,
                            h->stream));
-  int64_t xhalf = xsum;
+  {
+    const size_t len = (size_t)xsum;
+    const int blocks = (int)std::min<size_t>((len + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_sl_fill, dim3(std::max(blocks, 1)), dim3(256), 0, h->stream,
+                       (long long *)X, len);
+    MH_TRY(check_launch(h, "k_sl_fill"));
+  }
   for (const auto &L : launches) {
     SlTask *t0 = dtab + L.first;
     int nt = L.second;
     int g = 0;
     for (int pi = L.first; pi < L.first + L.second; ++pi) g += tab[pi].G;
-    void *args[] = {&t0, &nt, &A, &D, &E, &tau, &X, &xhalf, &flags, &err};
-    MH_HIP(h, hipLaunchCooperativeKernel((const void *)k_sytrd_lds, dim3(g), dim3(SL_THREADS), args,
-                                         (unsigned)lds, h->stream));
+    void *args[] = {&t0, &nt, &A, &D, &E, &tau, &X, &err};
+    MH_HIP(h, hipLaunchCooperativeKernel(kern, dim3(g), dim3(SL_THREADS), args, (unsigned)lds,
+                                         h->stream));
   }
   return MH_OK;
 }
@@ -1196,7 +1340,7 @@ extern "C" int mh_eig_check(mh_handle h, int nmat) {
   if (h->eig_err) {
     int err = 0;
     MH_HIP(h, hipMemcpy(&err, h->eig_err, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(h, MH_ERR_HIP, "mh_sytrd_batched: a group barrier timed out");
+    if (err) return fail(h, MH_ERR_HIP, "mh_sytrd_batched: a wait between workgroups timed out");
   }
   if (int *dce = dc_error_flag(h)) {
     int err = 0;
