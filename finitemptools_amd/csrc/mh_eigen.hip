@@ -282,6 +282,26 @@ __device__ inline double wave_sum64(double v) {
   return lane_value(v, 63);
 }
 
+// wave_sum64 of N values at once (the same additions per value, the N
+// reduction chains interleaved)
+template <int N>
+__device__ inline void wave_sum64_n(double (&v)[N]) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] += dpp_f64<0xB1, 0xF>(v[q]);
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] += dpp_f64<0x4E, 0xF>(v[q]);
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] += dpp_f64<0x124, 0xF>(v[q]);
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] += dpp_f64<0x128, 0xF>(v[q]);
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] += dpp_f64<0x142, 0xA>(v[q]);
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] += dpp_f64<0x143, 0xC>(v[q]);
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = lane_value(v[q], 63);
+}
+
 // value at row idx of a lane-distributed vector (lane idx % 64, slot idx / 64)
 template <int KR>
 __device__ inline double row_value(const double (&x)[KR], int idx) {
@@ -546,28 +566,39 @@ __global__ __launch_bounds__(SL_THREADS) void k_sytrd_lds(const SlTask *__restri
       }
       double *pc = xb + (2 * (int64_t)i + 1) * xs;
       double *bn = pc + xs;
-      // register columns: no LDS traffic; finished slots see zero factors
+      // register columns: no LDS traffic; finished slots see zero factors;
+      // the dot products of all of them reduced together
+      if (CW > 0) {
+        double acc[CW > 0 ? CW : 1];
 #pragma unroll
-      for (int m = 0; m < CW; ++m) {
-        const int q = nlds + wv + SL_WAVES * m;
-        const int j = g + q * G;
-        if (q < ncl && j > i && j < n) {
-          const double vpj = vprev[j], wpj = sW[j];
-          double acc = 0.0;
-#pragma unroll
-          for (int k = 0; k < KR; ++k) {
-            colr[m][k] = colr[m][k] - vpB[k] * wpj - wpB[k] * vpj;
-            acc += colr[m][k] * vcB[k];
-          }
-          if (j == i + 1) {  // the next step's column, for every workgroup
+        for (int m = 0; m < CW; ++m) {
+          const int q = nlds + wv + SL_WAVES * m;
+          const int j = g + q * G;
+          acc[m] = 0.0;
+          if (q < ncl && j > i && j < n) {
+            const double vpj = vprev[j], wpj = sW[j];
 #pragma unroll
             for (int k = 0; k < KR; ++k) {
-              const int r = lane + 64 * k;
-              if (k >= k0 && k < kr && r >= i + 1 && r < n) xstore(bn + r, colr[m][k]);
+              colr[m][k] = colr[m][k] - vpB[k] * wpj - wpB[k] * vpj;
+              acc[m] += colr[m][k] * vcB[k];
+            }
+            if (j == i + 1) {  // the next step's column, for every workgroup
+#pragma unroll
+              for (int k = 0; k < KR; ++k) {
+                const int r = lane + 64 * k;
+                if (k >= k0 && k < kr && r >= i + 1 && r < n) xstore(bn + r, colr[m][k]);
+              }
             }
           }
-          acc = wave_sum64(acc);
-          if (lane == 0) xstore(pc + j, tu * acc);
+        }
+        wave_sum64_n(acc);
+        if (lane == 0) {
+#pragma unroll
+          for (int m = 0; m < CW; ++m) {
+            const int q = nlds + wv + SL_WAVES * m;
+            const int j = g + q * G;
+            if (q < ncl && j > i && j < n) xstore(pc + j, tu * acc[m]);
+          }
         }
       }
       const int q0 = std::max(0, (i + 1 - g + G - 1) / G);

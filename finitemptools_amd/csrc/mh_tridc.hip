@@ -39,7 +39,7 @@ namespace mh {
 
 namespace {
 
-constexpr int DC_LEAF = 32;       // largest leaf order
+constexpr int DC_LEAF = 16;       // largest leaf order
 constexpr int DC_MAXN = 4096;     // largest matrix order
 constexpr int DC_QL_MAXIT = 60;   // QL iterations per eigenvalue of a leaf
 constexpr int DC_ROOT_MAXIT = 100;
@@ -190,6 +190,29 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const DcLeaf *__restrict__ lv,
 // a Givens deflation carries its surviving index to the next comparison).
 constexpr int PREP_THREADS = 256;
 
+// Exclusive prefix sum of one int per thread over the workgroup (wavefront
+// scans, then the wavefront totals); *total gets the sum.
+__device__ inline int prep_scan(int v, int *wsum, int *total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < PREP_THREADS / 64; ++q) {
+    off += (q < wv) ? wsum[q] : 0;
+    tot += wsum[q];
+  }
+  __syncthreads();  // wsum is reused by the next scan
+  *total = tot;
+  return off + x - v;
+}
+
 __global__ __launch_bounds__(PREP_THREADS) void k_dc_prep(const DcNode *__restrict__ tab,
                                                           const double *__restrict__ D,
                                                           const double *__restrict__ E,
@@ -197,12 +220,14 @@ __global__ __launch_bounds__(PREP_THREADS) void k_dc_prep(const DcNode *__restri
                                                           DcWork w) {
   extern __shared__ double smem[];
   __shared__ double red[PREP_THREADS / 64];
+  __shared__ int wsum[PREP_THREADS / 64];
   const DcNode nd = tab[blockIdx.x];
   const int n1 = nd.n1, n2 = nd.n2, n = n1 + n2, tid = threadIdx.x;
   double *sD = smem;
   double *sZ = smem + n;
   int *sC = (int *)(smem + 2 * n);
   int *st = sC + n;
+  int *sL = st + n;  // positions of the entries with a non-negligible z
   const double *lam = D + nd.v_off + nd.off;
   const double beta = E[nd.v_off + nd.off + n1 - 1];
   const double sgn = beta >= 0.0 ? 1.0 : -1.0;
@@ -245,13 +270,28 @@ __global__ __launch_bounds__(PREP_THREADS) void k_dc_prep(const DcNode *__restri
   for (int i = 0; i < PREP_THREADS / 64; ++i) dmax = fmax(dmax, red[i]);
   const double tol = 8.0 * DC_EPS * fmax(dmax, rho);
   const int64_t e0 = nd.e_off;
+  // every thread owns a contiguous chunk of positions
+  const int per = (n + PREP_THREADS - 1) / PREP_THREADS;
+  const int j0 = min(n, tid * per), j1 = min(n, j0 + per);
+  // (a) entries with a negligible z deflate at once (no rotation ever
+  // touches them first); the others are listed in position order
+  int cnt = 0;
+  for (int j = j0; j < j1; ++j) {
+    st[j] = 0;
+    cnt += (rho * fabs(sZ[j]) > tol) ? 1 : 0;
+  }
+  int m = 0;
+  int q = prep_scan(cnt, wsum, &m);
+  for (int j = j0; j < j1; ++j)
+    if (rho * fabs(sZ[j]) > tol) sL[q++] = j;
+  __syncthreads();
+  // (b) the Givens deflation of close neighbours: a chain along the listed
+  // entries, so one thread walks it (the graded matrices list few entries at
+  // the upper merges)
   if (tid == 0) {
     int nrot = 0, pj = -1;
-    for (int j = 0; j < n; ++j) {
-      if (rho * fabs(sZ[j]) <= tol) {
-        st[j] = 0;
-        continue;
-      }
+    for (int k = 0; k < m; ++k) {
+      const int j = sL[k];
       if (pj < 0) {
         pj = j;
         continue;
@@ -278,24 +318,28 @@ __global__ __launch_bounds__(PREP_THREADS) void k_dc_prep(const DcNode *__restri
       pj = j;
     }
     if (pj >= 0) st[pj] = 1;
-    int K = 0;
-    for (int j = 0; j < n; ++j) K += st[j];
-    // Q2 column order: kept (ascending d), then deflated (ascending position)
-    int kk = 0, kd = K;
-    for (int j = 0; j < n; ++j) {
-      const int c = st[j] ? kk++ : kd++;
-      w.gcol[e0 + c] = sC[j];
-      if (st[j]) {
-        w.dk[e0 + c] = sD[j];
-        w.zk[e0 + c] = sZ[j];
-      } else {
-        w.dval[e0 + c] = sD[j];
-      }
-    }
-    w.nK[nd.id] = K;
     w.nRot[nd.id] = nrot;
     w.nRho[nd.id] = rho;
   }
+  __syncthreads();
+  // (c) Q2 column order: kept (ascending d), then deflated (ascending
+  // position)
+  int kc = 0;
+  for (int j = j0; j < j1; ++j) kc += st[j];
+  int K = 0;
+  int ck = prep_scan(kc, wsum, &K);
+  int cd = K + (j0 - ck);  // deflated entries before this chunk follow the kept ones
+  for (int j = j0; j < j1; ++j) {
+    const int c = st[j] ? ck++ : cd++;
+    w.gcol[e0 + c] = sC[j];
+    if (st[j]) {
+      w.dk[e0 + c] = sD[j];
+      w.zk[e0 + c] = sZ[j];
+    } else {
+      w.dval[e0 + c] = sD[j];
+    }
+  }
+  if (tid == 0) w.nK[nd.id] = K;
 }
 
 // ---- merge step 2: Givens rotations of the deflation on Z's columns -------
@@ -778,7 +822,7 @@ extern "C" int mh_stedc_dc_batched(mh_handle h, int nmat, const int *n, const in
     if (nn == 0) continue;
     const DcNode *tab = (const DcNode *)(base + o_tab + hstart[k]);
     const int nm = hmax[k];
-    const size_t prep_lds = (size_t)nm * (2 * sizeof(double) + 2 * sizeof(int));
+    const size_t prep_lds = (size_t)nm * (2 * sizeof(double) + 3 * sizeof(int));
     int n2p = 1;
     while (n2p < nm) n2p <<= 1;
     const size_t ord_lds = (size_t)n2p * (sizeof(double) + sizeof(int));
