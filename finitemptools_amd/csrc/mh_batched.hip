@@ -36,8 +36,8 @@ typedef double bg_d4 __attribute__((ext_vector_type(4)));
 // C[l/16 + 4q][l%16].
 __global__ __launch_bounds__(256) void k_gemm_batched(const BgTask *__restrict__ tab, int nt,
                                                       double alpha, double beta) {
-  __shared__ double As[16][64 + 2];
-  __shared__ double Bs[16][64 + 2];
+  __shared__ double As[2][16][64 + 2];
+  __shared__ double Bs[2][16][64 + 2];
   int lo = 0, hi = nt - 1;
   while (lo < hi) {
     const int md = (lo + hi + 1) >> 1;
@@ -54,21 +54,53 @@ __global__ __launch_bounds__(256) void k_gemm_batched(const BgTask *__restrict__
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = bg_d4{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < t.K; k0 += 16) {
-    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-      const int kk = e >> 6, rr = e & 63;
-      const int gk = k0 + kk, gm = r0 + rr, gn = c0 + rr;
-      As[kk][rr] = (gk < t.K && gm < t.M) ? t.A[gm * t.sam + gk * t.sak] : 0.0;
-      Bs[kk][rr] = (gk < t.K && gn < t.N) ? t.B[gk * t.sbk + gn * t.sbn] : 0.0;
+  // staging order per operand: consecutive threads walk the unit-stride
+  // dimension (k when |stride along k| == 1, else the row), so a wavefront's
+  // loads fall on few cache lines; the next panel is loaded into registers
+  // while the current one is multiplied (two LDS buffers, one barrier per
+  // panel)
+  const bool a_k = (t.sak == 1 || t.sak == -1), b_k = (t.sbk == 1 || t.sbk == -1);
+  double ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      {
+        const int kk = a_k ? (e & 15) : (e >> 6), rr = a_k ? (e >> 4) : (e & 63);
+        const int gk = k0 + kk, gm = r0 + rr;
+        ra[q] = (gk < t.K && gm < t.M) ? t.A[gm * t.sam + gk * t.sak] : 0.0;
+      }
+      {
+        const int kk = b_k ? (e & 15) : (e >> 6), rr = b_k ? (e >> 4) : (e & 63);
+        const int gk = k0 + kk, gn = c0 + rr;
+        rb[q] = (gk < t.K && gn < t.N) ? t.B[gk * t.sbk + gn * t.sbn] : 0.0;
+      }
     }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      As[buf][a_k ? (e & 15) : (e >> 6)][a_k ? (e >> 4) : (e & 63)] = ra[q];
+      Bs[buf][b_k ? (e & 15) : (e >> 6)][b_k ? (e >> 4) : (e & 63)] = rb[q];
+    }
+  };
+  const int nk = (t.K + 15) / 16;
+  if (nk > 0) {
+    load(0);
+    store(0);
     __syncthreads();
+  }
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load(16 * (kt + 1));
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       double a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        a[i] = As[4 * s + lk][wm + 16 * i + li];
-        b[i] = Bs[4 * s + lk][wn + 16 * i + li];
+        a[i] = As[buf][4 * s + lk][wm + 16 * i + li];
+        b[i] = Bs[buf][4 * s + lk][wn + 16 * i + li];
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -76,6 +108,7 @@ __global__ __launch_bounds__(256) void k_gemm_batched(const BgTask *__restrict__
         for (int j = 0; j < 2; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    if (kt + 1 < nk) store(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
