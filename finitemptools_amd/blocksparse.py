@@ -34,10 +34,14 @@ from . import _lib
 
 
 def qadd(a, b):
+    if len(a) == 2:  # U(1) x U(1), the common case: no generator
+        return (a[0] + b[0], a[1] + b[1])
     return tuple(x + y for x, y in zip(a, b))
 
 
 def qsub(a, b):
+    if len(a) == 2:
+        return (a[0] - b[0], a[1] - b[1])
     return tuple(x - y for x, y in zip(a, b))
 
 
@@ -116,13 +120,14 @@ class BlockLayout:
                 qs = self.phys[s] if qs is None else qadd(qs, self.phys[s])
             combos.append((p, qs))
         off = 0
+        rdim = rspace.dim
         for qa, Da in lspace.sectors:
             g0, ncols = off, 0
             for p, qs in combos:
                 qb = qa if qs is None else qadd(qa, qs)
-                if qb not in rspace:
+                Db = rdim.get(qb)
+                if Db is None:
                     continue
-                Db = rspace.dim[qb]
                 b = Block(qa, p, qb, off, Da, Db, ncols)
                 blocks.append(b)
                 index[(qa, p)] = b

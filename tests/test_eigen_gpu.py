@@ -33,10 +33,14 @@ def _mats(rng, sizes):
     return out
 
 
-def test_sytrd_matches_lapack(gpu):
+# the largest order of a batch picks the kernel instance: 600 (10 row slots
+# per lane, 5 register columns per wavefront), 744 (12 slots, 4 register
+# columns), 1100 (18 slots, columns only in LDS)
+@pytest.mark.parametrize("sizes", [[1, 2, 3, 5, 17, 64, 100, 257, 600, 33], [744, 3, 700, 65],
+                                   [1100, 130, 2]])
+def test_sytrd_matches_lapack(gpu, sizes):
     import torch
     rng = np.random.default_rng(1)
-    sizes = [1, 2, 3, 5, 17, 64, 100, 257, 600, 33]
     mats = _mats(rng, sizes)
     se = eigen.SectorEigh(sizes, "cuda")
     for j, M in enumerate(mats):
