@@ -325,7 +325,7 @@ __device__ inline bool sl_empty(double v) { return __double_as_longlong(v) == SL
 template <int T>
 __device__ inline void sl_poll(const double *pp, const double *bc, bool with_p, int i, int n,
                                int kr, int wv, int lane, double (&wp)[T], double (&bv)[T],
-                               int *err) {
+                               int *err, unsigned spin_limit) {
   const int k0 = i >> 6;
 #pragma unroll
   for (int t = 0; t < T; ++t) {
@@ -351,7 +351,7 @@ __device__ inline void sl_poll(const double *pp, const double *bc, bool with_p, 
     }
     if (__ballot(miss != 0) == 0) break;
     const int e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (e != 0 || spins >= SY_SPIN_LIMIT) {
+    if (e != 0 || spins >= spin_limit) {
       if (e == 0 && lane == 0) atomicOr(err, 1);
       break;
     }
@@ -405,7 +405,8 @@ __global__ __launch_bounds__(SL_THREADS) void k_sytrd_lds(const SlTask *__restri
                                                           double *__restrict__ E,
                                                           double *__restrict__ tau,
                                                           double *__restrict__ X,
-                                                          int *__restrict__ err) {
+                                                          int *__restrict__ err,
+                                                          unsigned spin_limit, int drop_step) {
   constexpr int T = (KR + SL_WAVES - 1) / SL_WAVES;  // row slots per lane of a wavefront
   extern __shared__ double sm[];
   __shared__ double red[2][SL_WAVES];
@@ -470,7 +471,7 @@ __global__ __launch_bounds__(SL_THREADS) void k_sytrd_lds(const SlTask *__restri
     double wpr[T], bcr[T];
     {
       const double *bc = xb + 2 * (int64_t)i * xs;
-      sl_poll<T>(bc - xs, bc, i > 0, i, n, kr, wv, lane, wpr, bcr, err);
+      sl_poll<T>(bc - xs, bc, i > 0, i, n, kr, wv, lane, wpr, bcr, err, spin_limit);
     }
     {
       double s = 0.0;
@@ -566,6 +567,9 @@ __global__ __launch_bounds__(SL_THREADS) void k_sytrd_lds(const SlTask *__restri
       }
       double *pc = xb + (2 * (int64_t)i + 1) * xs;
       double *bn = pc + xs;
+      // failure-path test hook: workgroup 0 withholds its p entries of one
+      // step, so the readers' bounded wait must end in the error flag
+      const bool drop = (i == drop_step && g == 0);
       // register columns: no LDS traffic; finished slots see zero factors;
       // the dot products of all of them reduced together
       if (CW > 0) {
@@ -597,7 +601,7 @@ __global__ __launch_bounds__(SL_THREADS) void k_sytrd_lds(const SlTask *__restri
           for (int m = 0; m < CW; ++m) {
             const int q = nlds + wv + SL_WAVES * m;
             const int j = g + q * G;
-            if (q < ncl && j > i && j < n) xstore(pc + j, tu * acc[m]);
+            if (q < ncl && j > i && j < n && !drop) xstore(pc + j, tu * acc[m]);
           }
         }
       }
@@ -633,7 +637,7 @@ __global__ __launch_bounds__(SL_THREADS) void k_sytrd_lds(const SlTask *__restri
           }
         }
         acc = wave_sum64(acc);
-        if (lane == 0) xstore(pc + j, tu * acc);
+        if (lane == 0 && !drop) xstore(pc + j, tu * acc);
       }
     }
   }
@@ -989,6 +993,12 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
   if (KR < 1 || KR > SL_KRMAX) return MH_ERR_UNSUPPORTED;
   const int regc = SL_WAVES * sl_cw(KR);  // register columns per workgroup
   const void *kern = kSlKernels[KR - 1];
+  // test hooks of the failure path: the bound of every wait between
+  // workgroups (polls) and a step whose p entries workgroup 0 withholds
+  unsigned spin_limit = SY_SPIN_LIMIT;
+  int drop_step = -1;
+  if (const char *e = getenv("MH_SYTRD_SPIN")) spin_limit = (unsigned)std::max(1, atoi(e));
+  if (const char *e = getenv("MH_SYTRD_DROP")) drop_step = atoi(e);
   std::vector<int> order(nmat);
   for (int m = 0; m < nmat; ++m) order[m] = m;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
@@ -1094,7 +1104,7 @@ int sytrd_lds(mh_handle h, int nmat, const int *n, const int64_t *a_off, const i
     int nt = L.second;
     int g = 0;
     for (int pi = L.first; pi < L.first + L.second; ++pi) g += tab[pi].G;
-    void *args[] = {&t0, &nt, &A, &D, &E, &tau, &X, &err};
+    void *args[] = {&t0, &nt, &A, &D, &E, &tau, &X, &err, &spin_limit, &drop_step};
     MH_HIP(h, hipLaunchCooperativeKernel(kern, dim3(g), dim3(SL_THREADS), args, (unsigned)lds,
                                          h->stream));
   }

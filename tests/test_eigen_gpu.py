@@ -74,6 +74,48 @@ def test_sytrd_matches_lapack(gpu, sizes):
                 np.testing.assert_allclose(Am[i + 2:, i], a_ref[i + 2:, i], atol=1e-8, rtol=0)
 
 
+def test_exchange_wait_times_out_cleanly(gpu, monkeypatch):
+    """The failure path of the tridiagonalisation's exchange between
+    workgroups, forced from the host: workgroup 0 withholds its p entries of
+    step 5 (MH_SYTRD_DROP) and every wait is bounded by 2000 polls
+    (MH_SYTRD_SPIN), so the readers' waits end in the error flag, the launch
+    completes and mh_eig_check reports the time-out; with the hooks cleared
+    the next reduction equals LAPACK again."""
+    import torch
+    from finitemptools_amd import _lib
+    from finitemptools_amd.engine import ptr
+    rng = np.random.default_rng(5)
+    sizes = [300, 40]
+    mats = _mats(rng, sizes)
+    h = gpu.bind_stream()
+    lib = _lib.load()
+
+    def run():
+        se = eigen.SectorEigh(sizes, "cuda")
+        for j, M in enumerate(mats):
+            se.matrix(j).copy_(torch.from_numpy(M))
+        st = lib.mh_sytrd_batched(h, len(sizes), se._n[1], se._a[1], se._v[1], ptr(se.A),
+                                  ptr(se.D), ptr(se.E), ptr(se.tau))
+        _lib.raise_for(st, h)
+        return se, lib.mh_eig_check(h, 0)
+
+    monkeypatch.setenv("MH_SYTRD_SPIN", "2000")
+    monkeypatch.setenv("MH_SYTRD_DROP", "5")
+    print("mh_sytrd_batched with a withheld exchange step", flush=True)
+    _, st = run()
+    assert st == 4  # MH_ERR_HIP
+    assert "timed out" in lib.mh_last_error(h).decode()
+    monkeypatch.delenv("MH_SYTRD_SPIN")
+    monkeypatch.delenv("MH_SYTRD_DROP")
+    se, st = run()
+    _lib.raise_for(st, h)
+    D = se.D.cpu().numpy()
+    for j, (n, M) in enumerate(zip(sizes, mats)):
+        d_ref = lapack.dsytrd(M, lower=1)[1]
+        vo = int(se.v_off[j])
+        np.testing.assert_allclose(D[vo:vo + n], d_ref, atol=1e-13 * n * np.abs(M).max(), rtol=0)
+
+
 @pytest.mark.parametrize("sizes", [[1, 2, 7, 64, 300, 801], list(range(1, 71)), [1100, 63, 1024]])
 def test_eigenpairs(gpu, sizes):
     """Eigenvalues vs eigvalsh, residuals and orthonormality; the second case
