@@ -121,20 +121,33 @@ class BlockLayout:
             combos.append((p, qs))
         off = 0
         rdim = rspace.dim
+        two = all(len(q) == 2 for _, q in combos if q is not None)
         for qa, Da in lspace.sectors:
             g0, ncols = off, 0
-            for p, qs in combos:
-                qb = qa if qs is None else qadd(qa, qs)
-                Db = rdim.get(qb)
+            if two:
+                a0, a1 = qa
+                hits = [(p, qb, rdim.get(qb)) for p, qb in
+                        ((p, qa if qs is None else (a0 + qs[0], a1 + qs[1])) for p, qs in combos)]
+            else:
+                hits = [(p, qb, rdim.get(qb)) for p, qb in
+                        ((p, qa if qs is None else qadd(qa, qs)) for p, qs in combos)]
+            row = []
+            for p, qb, Db in hits:
                 if Db is None:
                     continue
                 b = Block(qa, p, qb, off, Da, Db, ncols)
-                blocks.append(b)
+                row.append(b)
                 index[(qa, p)] = b
-                by_right.setdefault(qb, []).append(b)
-                by_left.setdefault(qa, []).append(b)
+                rl = by_right.get(qb)
+                if rl is None:
+                    by_right[qb] = [b]
+                else:
+                    rl.append(b)
                 off += Da * Db
                 ncols += Db
+            if row:
+                blocks.extend(row)
+                by_left[qa] = row
             if ncols:
                 group[qa] = (g0, ncols)
         return blocks, index, group, by_right, by_left, off
