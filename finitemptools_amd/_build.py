@@ -18,6 +18,7 @@ SOURCES = [
     "mh_tridc.hip",
     "mh_batched.hip",
     "mh_heev.hip",
+    "mh_svd.hip",
     "mh_bs_planner.cpp",
     "mh_schur.cpp",
 ]

@@ -149,6 +149,9 @@ SIGNATURES = {
     "mh_heev_small": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int,
                               c_void_p]),
     "mh_heev_small_max": (c_int, [c_void_p, c_int, P(c_int)]),
+    "mh_svd_small": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                             c_void_p, c_void_p]),
+    "mh_svd_small_fits": (c_int, [c_void_p, c_int, c_int, c_int, P(c_int)]),
     "mh_profile_enable": (c_int, [c_void_p, c_int]),
     "mh_profile_read": (c_int, [c_void_p, c_int, P(c_double), P(c_int64), P(c_double)]),
     "mh_profile_reset": (c_int, [c_void_p]),

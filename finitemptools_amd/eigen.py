@@ -150,6 +150,59 @@ def small_eigh_max(dtype, device):
     return _SMALL_MAX[key]
 
 
+_SVD_FITS = {}
+
+
+def svd_small_fits(shape, dtype, device):
+    """Whether `svd_small` takes a matrix of this shape on this device."""
+    device = torch.device(device)
+    if device.type != "cuda" or dtype not in (torch.float64, torch.complex128):
+        return False
+    m, n = max(shape), min(shape)
+    e = engine(device)
+    key = (e.device, dtype, m, n)
+    if key not in _SVD_FITS:
+        h = e.bind_stream()
+        out = ctypes.c_int(0)
+        code = _lib.MH_C64 if dtype == torch.complex128 else _lib.MH_F64
+        _lib.raise_for(_lib.load().mh_svd_small_fits(h, code, m, n, ctypes.byref(out)), h,
+                       "mh_svd_small_fits")
+        _SVD_FITS[key] = bool(out.value)
+    return _SVD_FITS[key]
+
+
+def svd_small(M, sweeps=None):
+    """Thin SVD of a device matrix M (m x n, float64 or complex128) in one
+    launch (mh_svd_small, one-sided Jacobi): (U m x r, S r descending, Vh
+    r x n), r = min(m, n), as torch.linalg.svd(M, full_matrices=False).  A
+    wide matrix is factorised through its conjugate transpose.  S is NaN if
+    the Jacobi sweeps did not converge.  sweeps: an optional one-element
+    device int32 tensor that receives the sweeps run (-1: not converged)."""
+    m, n = M.shape
+    e = engine(M.device)
+    h = e.bind_stream()
+    code = _lib.MH_C64 if M.dtype == torch.complex128 else _lib.MH_F64
+    tall = m >= n
+    # column-major storage of the tall operand: M itself (row-major M^T), or
+    # M^H (row-major conj(M))
+    mm, nn = (m, n) if tall else (n, m)
+    M = M.resolve_conj()
+    At = (M.t() if tall else torch.conj_physical(M)).contiguous()
+    Ut = torch.empty((nn, mm), dtype=M.dtype, device=M.device)   # U, column-major
+    S = torch.empty(nn, dtype=torch.float64, device=M.device)
+    Vt = torch.empty((nn, nn), dtype=M.dtype, device=M.device)   # V, column-major
+    _lib.raise_for(_lib.load().mh_svd_small(h, code, mm, nn, ptr(At), mm, ptr(Ut), ptr(S),
+                                            ptr(Vt), ptr(sweeps) if sweeps is not None else None),
+                   h, "mh_svd_small")
+    # (conjugates are taken physically: the engine reads raw storage, which
+    # a lazily conjugated view would not change)
+    U1, V1 = Ut.t(), Vt.t()
+    if tall:
+        return U1, S, torch.conj_physical(Vt)
+    # M^H = U1 S V1^H  ->  M = V1 S U1^H
+    return V1, S, torch.conj_physical(Ut)
+
+
 def small_eigh(Gt, col0=0):
     """Eigenpairs of a Hermitian (complex128) or symmetric (float64) n x n
     device matrix G given as Gt = G^T (row-major), i.e. G in column-major

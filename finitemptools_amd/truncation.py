@@ -21,10 +21,14 @@ DEFAULT_NONSYMMETRIC_D = 100  # truncation.jl:51
 # against 1.8 ms).
 SMALL_EIGH = True
 SMALL_EIGH_ORDER = 24
-# SVDs of two-site matrices up to this size (larger extent) run in host
-# LAPACK, which the reference calls too: at these orders a device SVD is
-# launch-latency bound.
+# SVDs of two-site matrices too large for the one-workgroup device SVD but up
+# to this size (larger extent) run in host LAPACK, which the reference calls
+# too: at these orders the device library's SVD is launch-latency bound.
 HOST_SVD_MAX = 256
+# SVDs whose matrices fit one workgroup's LDS run on the device first
+# (mh_svd_small: one launch, one-sided Jacobi); False sends them to the
+# branches below.
+DEVICE_SVD = True
 
 # A density-matrix eigenvalue of an order-n matrix carries an absolute error
 # of about n * eps64 * lambda_max (eps64 = 2^-52); singular values taken as
@@ -145,10 +149,15 @@ def split_rho(x, trunc, absorb, normalize=True):
 
 
 def stable_tsvd(x, trunc):
-    """x[a, s1, s2, b] -> u[a, s1, k], s[k] (host + device), v[k, s2, b], err."""
+    """x[a, s1, s2, b] -> u[a, s1, k], s[k], v[k, s2, b], err: the device
+    one-workgroup SVD where the matrix fits it, else LAPACK gesdd (gesvd if
+    it fails) up to HOST_SVD_MAX, else the device library."""
     Dl, d1, d2, Dr = x.shape
     M = as_matrix(x, 2)
-    if max(M.shape) <= HOST_SVD_MAX:
+    if x.is_cuda and DEVICE_SVD and eigen.svd_small_fits(M.shape, M.dtype, x.device):
+        # one launch on the device (one-sided Jacobi, mh_svd_small)
+        U, S, Vh = eigen.svd_small(M)
+    elif max(M.shape) <= HOST_SVD_MAX:
         # LAPACK gesdd, gesvd if it fails (tsvd(alg=SDD()) and its fallback)
         import scipy.linalg as sla
         Mh = M.detach().cpu().numpy()
@@ -163,6 +172,8 @@ def stable_tsvd(x, trunc):
         except RuntimeError:
             U, S, Vh = torch.linalg.svd(M, full_matrices=False, driver="gesvd")
     s_host = S.detach().cpu().numpy()
+    if not np.isfinite(s_host).all():
+        raise MHError(6, "stable_tsvd: the SVD did not converge")
     k, err = trunc.truncate(s_host)
     k = max(k, 1)
     u = from_matrix(U[:, :k], (Dl, d1, k))

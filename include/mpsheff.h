@@ -480,7 +480,7 @@ int mh_gemm_batched(mh_handle h, int count, const int64_t *desc, double alpha, d
 /* Waits for the handle's stream; MH_ERR_NOCONV if a dstedc call of the last
  * mh_stedc_batched (nmat matrices) or a leaf of the last mh_stedc_dc_batched
  * failed to converge, MH_ERR_HIP if a
- * group barrier of the last tridiagonalisation timed out. */
+ * wait between workgroups of the last tridiagonalisation timed out. */
 int mh_eig_check(mh_handle h, int nmat);
 
 /* Eigen-decomposition of `batch` Hermitian (MH_C64) or symmetric (MH_F64)
@@ -497,6 +497,22 @@ int mh_heev_small(mh_handle h, int dtype, int n, int batch, const void *A, doubl
                   int col0, int *info);
 /* largest order mh_heev_small takes on the handle's device for dtype */
 int mh_heev_small_max(mh_handle h, int dtype, int *nmax);
+
+/* Thin SVD A = U diag(S) V^H of one m x n matrix with m >= n (MH_F64 or
+ * MH_C64), one workgroup with A and V in LDS, by one-sided Jacobi: the
+ * truncated SVD of the dense two-site split (auxiliary/stable_svd.jl:4-18,
+ * LAPACK gesdd in the reference's tsvd) at the small bonds where a library
+ * gesvd is launch-latency bound.  A: column-major, leading dimension lda
+ * (read only).  U: m x n, S: n (descending), V: n x n (right singular vectors
+ * as columns), all column-major on the device; stream-ordered, no host
+ * synchronisation.  sweeps (device, nullable): the Jacobi sweeps run (the
+ * last one rotated no pair), or -1 when the sweep limit was reached; every S
+ * entry is then NaN.  MH_ERR_SHAPE when m < n or the matrices do not fit the
+ * device's LDS (mh_svd_small_fits tells beforehand). */
+int mh_svd_small(mh_handle h, int dtype, int m, int n, const void *A, int lda, void *U, double *S,
+                 void *V, int *sweeps);
+/* *fits = 1 if mh_svd_small takes an m x n matrix of dtype on this device */
+int mh_svd_small_fits(mh_handle h, int dtype, int m, int n, int *fits);
 
 /* ---- measurement --------------------------------------------------------- */
 enum mh_kernel_class { MH_K_GEMM = 0, MH_K_WSTAGE = 1, MH_K_VEC = 2, MH_K_NCLASS = 3 };
