@@ -499,7 +499,9 @@ int mh_heev_small(mh_handle h, int dtype, int n, int batch, const void *A, doubl
 int mh_heev_small_max(mh_handle h, int dtype, int *nmax);
 
 /* Thin SVD A = U diag(S) V^H of one m x n matrix with m >= n (MH_F64 or
- * MH_C64), one workgroup with A and V in LDS, by one-sided Jacobi: the
+ * MH_C64), one workgroup with A and V in LDS, by column-pivoted QR and
+ * one-sided Jacobi on R^H (MH_SVD_PLAIN=1 in the environment: Jacobi on A
+ * itself; an input rank-deficient far below eps takes that path too): the
  * truncated SVD of the dense two-site split (auxiliary/stable_svd.jl:4-18,
  * LAPACK gesdd in the reference's tsvd) at the small bonds where a library
  * gesvd is launch-latency bound.  A: column-major, leading dimension lda

@@ -144,15 +144,34 @@ def test_device_svd_split_matches_host_lapack(gpu, monkeypatch, cplx, eps):
 @pytest.mark.parametrize("cplx", [False, True])
 def test_sweep_count(gpu, cplx):
     """Jacobi converges quadratically: a random 48 x 48 matrix takes about 10
-    sweeps (LAPACK-style cyclic Jacobi takes the same on the CPU), a graded
-    one about 25; a wrong rotation angle still converges, only slowly, and
-    shows here first."""
+    sweeps (cyclic Jacobi on the CPU takes the same), a graded one about 8
+    after the pivoted QR (25 without); a wrong rotation angle still
+    converges, only slowly, and shows here first."""
     import torch
     rng = np.random.default_rng(19)
     sw = torch.zeros(1, dtype=torch.int32, device="cuda")
-    for spectrum, limit in ((None, 14), (np.logspace(0, -14, 48), 32)):
+    for spectrum, limit in ((None, 14), (np.logspace(0, -14, 48), 12)):
         M = torch.from_numpy(_mat(rng, 48, 48, cplx, spectrum=spectrum)).cuda()
         eigen.svd_small(M, sweeps=sw)
         n_sw = int(sw.item())
         print(f"{'graded' if spectrum is not None else 'random'}: {n_sw} sweeps", flush=True)
         assert 1 <= n_sw <= limit
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_plain_and_preconditioned_agree(gpu, monkeypatch, cplx):
+    """The pivoted-QR preconditioned iteration (the default) and the plain
+    one on A (MH_SVD_PLAIN=1) give the same SVD; the preconditioned one
+    takes a third of the sweeps on a graded spectrum."""
+    import torch
+    rng = np.random.default_rng(23)
+    M = _mat(rng, 40, 32, cplx, spectrum=np.logspace(0, -14, 32))
+    sw = torch.zeros(1, dtype=torch.int32, device="cuda")
+    out = {}
+    for plain in ("0", "1"):
+        monkeypatch.setenv("MH_SVD_PLAIN", plain)
+        S, _ = _check(M)
+        eigen.svd_small(torch.from_numpy(M).cuda(), sweeps=sw)
+        out[plain] = (S, int(sw.item()))
+    np.testing.assert_allclose(out["0"][0], out["1"][0], atol=4 * 32 * np.finfo(float).eps, rtol=0)
+    assert out["0"][1] <= 12 and out["0"][1] * 2 <= out["1"][1], (out["0"][1], out["1"][1])
