@@ -296,13 +296,26 @@ class BlockExpectationCache:
         for i in range(center):
             self.updateleft(i)
 
+    def _prefetched_env(self, A, old):
+        """(new layout, plan) made for exactly these layouts by the sweep's
+        prefetch, or None."""
+        got = _prefetched(self).get("env")
+        if got and got[0] is A.layout and got[1] is old.layout and got[4] == self.prec:
+            return got[2], got[3]
+        return None
+
     def updateleft(self, site):
         """hstorage[site+1] = updateleft(hstorage[site], A, W, A)"""
         t0 = time.perf_counter()
         A = self.mps[site]
         old = self.hstorage[site]
-        new_lay = EnvLayout(A.rspace, self.mpo.qw[site + 1])
-        plan = bs.make_plan("updateleft", A.layout, old.layout, self.mpo[site], new_lay, self.prec)
+        got = self._prefetched_env(A, old)
+        if got:
+            new_lay, plan = got
+        else:
+            new_lay = EnvLayout(A.rspace, self.mpo.qw[site + 1])
+            plan = bs.make_plan("updateleft", A.layout, old.layout, self.mpo[site], new_lay,
+                                self.prec)
         out = torch.zeros(new_lay.size, dtype=torch.float64, device=A.data.device)
         plan.apply(A.data, old.data, A.data, out)
         self.hstorage[site + 1] = BlockEnv(new_lay, out)
@@ -313,8 +326,13 @@ class BlockExpectationCache:
         t0 = time.perf_counter()
         A = self.mps[site]
         old = self.hstorage[site + 1]
-        new_lay = EnvLayout(A.lspace, self.mpo.qw[site])
-        plan = bs.make_plan("updateright", A.layout, old.layout, self.mpo[site], new_lay, self.prec)
+        got = self._prefetched_env(A, old)
+        if got:
+            new_lay, plan = got
+        else:
+            new_lay = EnvLayout(A.lspace, self.mpo.qw[site])
+            plan = bs.make_plan("updateright", A.layout, old.layout, self.mpo[site], new_lay,
+                                self.prec)
         out = torch.zeros(new_lay.size, dtype=torch.float64, device=A.data.device)
         plan.apply(A.data, old.data, A.data, out)
         self.hstorage[site] = BlockEnv(new_lay, out)
@@ -437,10 +455,11 @@ def two_site_layout(A1, A2):
     return BlockLayout(A1.lspace, A1.layout.phys, 2, A2.rspace)
 
 
-def two_site(A1, A2, xl=None):
-    """x = A1 * A2 over the middle bond (dmrg.jl:98) by one grouped launch."""
+def two_site(A1, A2, xl=None, plan=None):
+    """x = A1 * A2 over the middle bond (dmrg.jl:98) by one grouped launch
+    (plan: one made for these layouts beforehand)."""
     xl = xl or two_site_layout(A1, A2)
-    plan = bs.make_plan("two_site", A1.layout, A2.layout, xl)
+    plan = plan or bs.make_plan("two_site", A1.layout, A2.layout, xl)
     out = torch.zeros(xl.size, dtype=torch.float64, device=A1.data.device)
     plan.apply(A1.data, A2.data, None, out)
     return BlockTensor(xl, out)
@@ -691,7 +710,7 @@ def _gemms(descs, alpha=1.0, beta=0.0):
                                      float(alpha), float(beta)), h, "mh_gemm_batched")
 
 
-def _split_rho_sector(tmap, vec, trunc, absorb):
+def _split_rho_sector(tmap, vec, trunc, absorb, hook=None):
     """split_two_site_rho with every per-sector product batched: the sector
     density matrices, the projections on eigenvector blocks, both levels of
     the exact cut and the assembly of the two new site tensors run as a few
@@ -780,10 +799,12 @@ def _split_rho_sector(tmap, vec, trunc, absorb):
     _gemms(descs)
     svec = torch.linalg.vector_norm(Bk[:acc], dim=1)
     norm = torch.linalg.vector_norm(svec)
-    s_host = (svec / norm).cpu().numpy()
     mid = Space(kept.items())
     l1 = BlockLayout(xl.lspace, xl.phys, 1, mid)
     l2 = BlockLayout(mid, xl.phys, 1, xl.rspace)
+    if hook is not None:
+        hook(l1, l2)   # host work while the device runs the products above
+    s_host = (svec / norm).cpu().numpy()
     svals = {c: s_host[rowb[c]:rowb[c] + kept[c]] for c in order}
     # assembly: the eigenvector side straight from Z (and the tail vectors),
     # the projected side from Bk scaled by 1/norm
@@ -979,7 +1000,7 @@ def _refine_sector(tmap, sec, se, lam_host, lam_all, trunc, dev, pc, wmax):
     return plan, Tbuf, tb_off
 
 
-def split_two_site_rho(tmap, vec, trunc, absorb):
+def split_two_site_rho(tmap, vec, trunc, absorb, hook=None):
     """Device split by the sector reduced density matrices (White's original
     DMRG step): for the left sweep rho_c = M_c M_c^T gives u and s^2, and
     s v = u^T M_c is an exact projection; for the right sweep rho_c = M_c^T M_c
@@ -994,7 +1015,7 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
     s = sqrt(lambda) exactly as for the SVD.  Returns (A1, A2, {c: s_c})."""
     if (EIG_SOLVER == "sector" and not PLAIN and vec.device.type == "cuda"
             and all(max(tmap.shape[c]) <= eigen.MAX_ORDER for c in tmap.charges)):
-        out = _split_rho_sector(tmap, vec, trunc, absorb)
+        out = _split_rho_sector(tmap, vec, trunc, absorb, hook)
         if out is not None:
             return out
     xl = tmap.xl
@@ -1089,6 +1110,8 @@ def split_two_site_rho(tmap, vec, trunc, absorb):
         s_dev.append(torch.linalg.vector_norm(B, dim=1 if left else 0))
     s_all = torch.cat(s_dev)
     norm = torch.linalg.vector_norm(s_all)
+    if hook is not None:
+        hook(l1, l2)   # host work while the device runs the products above
     s_host = (s_all / norm).cpu().numpy()
     svals, us, vs, upieces, vpieces = {}, [], [], [], []
     ubase = vbase = spos = 0
@@ -1230,15 +1253,99 @@ def _global_keep_host(svd, trunc):
     return {q: int(c) for q, c in zip(keys, counts) if c > 0}
 
 
+def _prefetch(m, alg, site, absorb, l1, l2, xl0, dev):
+    """Plans the sweep needs next, made from the new site layouts l1, l2 of
+    the bond (site, site + 1) as soon as its cut is known, on a host thread
+    beside the rest of the split: the overlap x' = A1' A2' (x layout xl0),
+    the environment update that follows the bond, and the next bond's
+    two-site tensor and H_eff.  Returns them by name; each consumer checks
+    that the layouts it meets are these very objects, so a plan is only ever
+    the one it would have made itself."""
+    t0 = time.perf_counter()
+    mps, hs, mpo = m.mps, m.hstorage, m.mpo
+
+    def make(*a):
+        return bs.make_plan(*a, device=dev)
+
+    pre = {"overlap": (l1, l2, make("two_site", l1, l2, xl0))}
+    if absorb == "right":       # left sweep: hs[site+1] from l1, then bond site+1
+        old = hs[site]
+        new_lay = EnvLayout(l1.rspace, mpo.qw[site + 1])
+        pre["env"] = (l1, old.layout, new_lay,
+                      make("updateleft", l1, old.layout, mpo[site], new_lay, m.prec), m.prec)
+        nxt = site + 1
+        if nxt + 1 < len(mps):
+            B = mps[nxt + 1]
+            xl = BlockLayout(l2.lspace, l2.phys, 2, B.rspace)
+            pre["two_site"] = (l2, B.layout, xl, make("two_site", l2, B.layout, xl))
+            R = hs[nxt + 2]
+            pre["heff2"] = (xl, new_lay, R.layout, alg.prec,
+                            make("heff2", xl, new_lay, R.layout, mpo[nxt], mpo[nxt + 1],
+                                 alg.prec))
+    else:                       # right sweep: hs[site+1] from l2, then bond site-1
+        old = hs[site + 2]
+        new_lay = EnvLayout(l2.lspace, mpo.qw[site + 1])
+        pre["env"] = (l2, old.layout, new_lay,
+                      make("updateright", l2, old.layout, mpo[site + 1], new_lay, m.prec),
+                      m.prec)
+        nxt = site - 1
+        if nxt >= 0:
+            A = mps[nxt]
+            xl = BlockLayout(A.lspace, A.layout.phys, 2, l1.rspace)
+            pre["two_site"] = (A.layout, l1, xl, make("two_site", A.layout, l1, xl))
+            Lh = hs[nxt]
+            pre["heff2"] = (xl, Lh.layout, new_lay, alg.prec,
+                            make("heff2", xl, Lh.layout, new_lay, mpo[nxt], mpo[nxt + 1],
+                                 alg.prec))
+    pre["seconds"] = time.perf_counter() - t0
+    return pre
+
+
+_POOL = []
+
+
+def _planner_thread():
+    """The one host thread the sweeps' plan prefetch runs on."""
+    if not _POOL:
+        from concurrent.futures import ThreadPoolExecutor
+        _POOL.append(ThreadPoolExecutor(max_workers=1, thread_name_prefix="mh-plans"))
+    return _POOL[0]
+
+
+def _prefetched(m):
+    """The plans prefetched for m (waiting for the planner thread), or {}."""
+    fut = getattr(m, "_pre_future", None)
+    if fut is not None:
+        m._pre_future = None
+        m._pre = fut.result()
+    return getattr(m, "_pre", {})
+
+
+# Make the next plans on a host thread once the split's cut is known
+# (_prefetch); False makes every plan where it is used.
+PREFETCH = True
+
+
 def _bond_step(m, alg, site, absorb):
     mps, hs = m.mps, m.hstorage
     A1, A2 = mps[site], mps[site + 1]
     dev = A1.data.device
+    pre = _prefetched(m)
+    m._pre = {}
     t0 = time.perf_counter()
-    x = two_site(A1, A2)
+    got = pre.get("two_site")
+    if got and got[0] is A1.layout and got[1] is A2.layout:
+        x = two_site(A1, A2, got[2], got[3])
+    else:
+        x = two_site(A1, A2)
     L, R = hs[site], hs[site + 2]
-    plan = bs.make_plan("heff2", x.layout, L.layout, R.layout, m.mpo[site], m.mpo[site + 1],
-                        alg.prec)
+    got = pre.get("heff2")
+    if (got and got[0] is x.layout and got[1] is L.layout and got[2] is R.layout
+            and got[3] == alg.prec):
+        plan = got[4]
+    else:
+        plan = bs.make_plan("heff2", x.layout, L.layout, R.layout, m.mpo[site],
+                            m.mpo[site + 1], alg.prec)
     t1 = time.perf_counter()
     ev, vec, info, nit = plan.lanczos(L.data, R.data, x.data, alg.eig_maxiter, alg.eig_tol,
                                       reorth=alg.reorth)
@@ -1248,12 +1355,21 @@ def _bond_step(m, alg, site, absorb):
     if getattr(alg, "split", "rho") == "svd":
         n1, n2, svals = split_two_site(tmap, vec.cpu().numpy(), alg.trunc, absorb, dev)
     else:
-        n1, n2, svals = split_two_site_rho(tmap, vec, alg.trunc, absorb)
+        def hook(l1, l2):
+            m._pre_future = _planner_thread().submit(_prefetch, m, alg, site, absorb, l1, l2,
+                                                     x.layout, dev)
+
+        n1, n2, svals = split_two_site_rho(tmap, vec, alg.trunc, absorb,
+                                           hook if PREFETCH else None)
     t2b = time.perf_counter()
-    # err_1 = <u s v, twositemps> (dmrg.jl:109-110)
-    e = torch.dot(two_site(n1, n2, x.layout).data, x.data).item()
-    t3 = time.perf_counter()
+    # err_1 = <u s v, twositemps> (dmrg.jl:109-110), left on the device: the
+    # sweep reads all of them at its end
+    got = _prefetched(m).get("overlap")
+    oplan = got[2] if got and got[0] is n1.layout and got[1] is n2.layout else None
     st = alg.stats
+    st["prefetch_s"] = st.get("prefetch_s", 0.0) + m._pre.get("seconds", 0.0)
+    e = torch.dot(two_site(n1, n2, x.layout, oplan).data, x.data)
+    t3 = time.perf_counter()
     st["map_s"] = st.get("map_s", 0.0) + (t2a - t2)
     st["split_s"] = st.get("split_s", 0.0) + (t2b - t2a)
     st["overlap_s"] = st.get("overlap_s", 0.0) + (t3 - t2b)
@@ -1262,38 +1378,47 @@ def _bond_step(m, alg, site, absorb):
     st["svd_s"] += t3 - t2
     st["matvec_flops"] += plan.flops() * nit
     st["bonds"] += 1
-    return ev, n1, n2, svals, abs(1.0 - abs(e))
+    return ev, n1, n2, svals, e
+
+
+def _truncation_error(overlaps):
+    """max over bonds of |1 - |<u s v, x>|| from the device overlaps (one
+    host read)."""
+    if not overlaps:
+        return 0.0
+    vals = torch.stack(overlaps).cpu().numpy()
+    return float(np.max(np.abs(1.0 - np.abs(vals))))
 
 
 def leftsweep(m, alg):
     """dmrg.jl:89-115 on block-sparse tensors"""
     mps = m.mps
-    energies, delta = [], 0.0
+    energies, overlaps = [], []
     for site in range(len(mps) - 2):
-        ev, n1, n2, svals, err = _bond_step(m, alg, site, "right")
+        ev, n1, n2, svals, e = _bond_step(m, alg, site, "right")
         energies.append(ev)
         mps[site], mps[site + 1] = n1, n2
-        delta = max(delta, err)
+        overlaps.append(e)
         t0 = time.perf_counter()
         m.updateleft(site)
         alg.stats["env_s"] += time.perf_counter() - t0
-    return energies, delta
+    return energies, _truncation_error(overlaps)
 
 
 def rightsweep(m, alg):
     """dmrg.jl:117-144 on block-sparse tensors"""
     mps = m.mps
-    energies, delta = [], 0.0
+    energies, overlaps = [], []
     for site in range(len(mps) - 2, -1, -1):
-        ev, n1, n2, svals, err = _bond_step(m, alg, site, "left")
+        ev, n1, n2, svals, e = _bond_step(m, alg, site, "left")
         energies.append(ev)
         mps[site], mps[site + 1] = n1, n2
         mps.s[site + 1] = svals
-        delta = max(delta, err)
+        overlaps.append(e)
         t0 = time.perf_counter()
         m.updateright(site + 1)
         alg.stats["env_s"] += time.perf_counter() - t0
-    return energies, delta
+    return energies, _truncation_error(overlaps)
 
 
 def sweep(m, alg):

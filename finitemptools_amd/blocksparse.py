@@ -393,7 +393,9 @@ class Plan:
         self.tables = tables
         self.eng = engine(device)
         self.lib = self.eng.lib
-        self.h = self.eng.bind_stream()
+        # building a plan launches nothing (apply and lanczos bind the
+        # caller's stream), so any thread may build one
+        self.h = self.eng.h
         self.prec = prec
         if _native is not None:
             # built by a native planner: _native(h, &ptr) -> status
@@ -559,12 +561,13 @@ _PY_PLANNERS = {"heff2": lambda xl, La, Ra, W1, W2, prec: plan_heff2(xl, La, Ra,
 PLANNER = "native"
 
 
-def make_plan(kind, *args, planner=None):
-    """Plan of one contraction (kind and arguments as native_builder)."""
+def make_plan(kind, *args, planner=None, device=None):
+    """Plan of one contraction (kind and arguments as native_builder) on the
+    engine of ``device`` (default: the current device)."""
     if (planner or PLANNER) == "native":
-        return native_plan(kind, *args)
+        return native_plan(kind, *args, device=device)
     prec = args[-1] if kind != "two_site" else "f64"
-    return Plan(_PY_PLANNERS[kind](*args), prec)
+    return Plan(_PY_PLANNERS[kind](*args), prec, device)
 
 
 def read_tables(ptr, lib=None):
