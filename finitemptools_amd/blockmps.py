@@ -433,12 +433,24 @@ def piece_index_host(pieces):
     return P[pid, 0] + local, P[pid, 3] + (P[pid, 5] + i) * P[pid, 4] + P[pid, 6] + j
 
 
+def _upload(a, device):
+    """Host integer indices -> an int64 tensor on ``device`` without draining
+    the stream: a pageable host-to-device copy waits for every launch queued
+    before it, so device copies go through pinned host memory, whose block
+    torch's host allocator keeps until the copy has run."""
+    t = torch.from_numpy(np.ascontiguousarray(a, dtype=np.int64))
+    device = torch.device(device)
+    if device.type != "cuda":
+        return t.to(device)
+    return t.pin_memory().to(device, non_blocking=True)
+
+
 def piece_index(pieces, device):
     """piece_index_host computed on the device (one pass, no host loop)."""
     if len(pieces) == 0:
         z = torch.zeros(0, dtype=torch.int64, device=device)
         return z, z
-    P = torch.as_tensor(pieces, device=device)
+    P = _upload(pieces, device)
     sizes = P[:, 1] * P[:, 2]
     total = int((pieces[:, 1] * pieces[:, 2]).sum())
     pid = torch.repeat_interleave(torch.arange(len(pieces), device=device), sizes,
@@ -683,9 +695,10 @@ def _strided_pieces(pieces, device):
     if len(pieces) == 0:
         z = torch.zeros(0, dtype=torch.int64, device=device)
         return z, z
-    P = torch.as_tensor(np.asarray(pieces, dtype=np.int64), device=device)
+    Ph = np.asarray(pieces, dtype=np.int64).reshape(len(pieces), -1)
+    P = _upload(Ph, device)
     sizes = P[:, 2] * P[:, 3]
-    total = int(sum(int(p[2]) * int(p[3]) for p in pieces))
+    total = int((Ph[:, 2] * Ph[:, 3]).sum())
     pid = torch.repeat_interleave(torch.arange(len(pieces), device=device), sizes,
                                   output_size=total)
     start = torch.cumsum(sizes, 0) - sizes
@@ -884,7 +897,7 @@ def _refine_sector(tmap, sec, se, lam_host, lam_all, trunc, dev, pc, wmax):
         acc_v += p[8]
     tails = [q for q, p in enumerate(parts) if p[8] < p[7]]
     if acc_v:
-        sr_idx = torch.as_tensor(sig_rows, device=dev)
+        sr_idx = _upload(sig_rows, dev)
         Vs = Bp[sr_idx] / bnorm[sr_idx, None]
         gsz = [p[8] * p[8] for p in parts]
         go = np.concatenate([[0], np.cumsum(gsz)[:-1]]).astype(np.int64)
@@ -927,7 +940,7 @@ def _refine_sector(tmap, sec, se, lam_host, lam_all, trunc, dev, pc, wmax):
         flat = [m.reshape(-1) for m in mus]
     else:
         flat = []
-    ss_dev = bnorm[torch.as_tensor(sig_rows, device=dev)] if len(sig_rows) else bnorm[:0]
+    ss_dev = bnorm[_upload(sig_rows, dev)] if len(sig_rows) else bnorm[:0]
     host = torch.cat(flat + [ss_dev]).cpu().numpy()
     if se2 is not None:
         se2.check()
@@ -1230,7 +1243,7 @@ def _refine_tail(mats, left, order, lam_host, eig, dev, trunc):
         vv = torch.cat([Qs, tail_vec[c]], dim=1) if c in tail_vec else Qs
         o = np.argsort(-sv, kind="stable")
         out_vals[c] = np.sqrt(sv[o])
-        out_vecs[c] = vv[:, torch.as_tensor(o.copy(), device=vv.device)]
+        out_vecs[c] = vv[:, _upload(o, vv.device)]
         s2_all.append(sv)
         nz_all.append(np.full(len(sv), noise2[c]))
     s2 = np.concatenate(s2_all)
