@@ -549,8 +549,11 @@ __global__ __launch_bounds__(SV_THREADS) void k_svd_small(int m, int n, const T 
     __syncthreads();
     total = sv_jacobi<T, E>(sA, n, sV, n, &rot);
     sv_rank(sA, n, n, sig, rk);
-    for (int j = wv; j < n; j += SV_WAVES)
-      if (sig[j] * sig[j] < SV_TINY) flag = 1;  // benign race: all write 1
+    if (wv == 0) {
+      bool low = false;
+      for (int j = lane; j < n; j += 64) low = low || sig[j] * sig[j] < SV_TINY;
+      if (__ballot(low) != 0 && lane == 0) flag = 1;
+    }
     __syncthreads();
     plain = flag != 0;
     if (!plain) {
