@@ -21,6 +21,11 @@ DEFAULT_NONSYMMETRIC_D = 100  # truncation.jl:51
 # against 1.8 ms).
 SMALL_EIGH = True
 SMALL_EIGH_ORDER = 24
+# Real density matrices above that order go to the batched device
+# eigensolver (eigen.SectorEigh: LDS tridiagonalisation, divide and
+# conquer, back-transformation of the kept columns only) instead of the
+# library eigh, a chain of launches at these orders.
+SECTOR_EIGH = True
 # SVDs of two-site matrices too large for the one-workgroup device SVD but up
 # to this size (larger extent) run in host LAPACK, which the reference calls
 # too: at these orders the device library's SVD is launch-latency bound.
@@ -114,16 +119,27 @@ def split_rho(x, trunc, absorb, normalize=True):
     left = absorb == "right"
     r = min(M.shape)
     n = M.shape[0] if left else M.shape[1]
+    se = None
     if (SMALL_EIGH and x.is_cuda and n <= SMALL_EIGH_ORDER
             and n <= eigen.small_eigh_max(M.dtype, x.device)):
         # one-launch solver; it takes G column-major: G^T = conj(M) M^T (left)
         # or M^T conj(M) (right); only the top r columns are finished
         Gt = M.conj() @ M.t() if left else M.t() @ M.conj()
         lam, V, _ = eigen.small_eigh(Gt, col0=n - r)
+    elif SECTOR_EIGH and x.is_cuda and M.dtype == torch.float64 and n <= eigen.MAX_ORDER:
+        # the batched solver on one matrix; vectors only for the kept columns
+        se = eigen.SectorEigh([n], x.device)
+        if left:
+            torch.matmul(M, M.t(), out=se.matrix(0))
+        else:
+            torch.matmul(M.t(), M, out=se.matrix(0))
+        lam = se.eigenvalues()[0]
     else:
         G = M @ M.conj().t() if left else M.conj().t() @ M
         lam, V = torch.linalg.eigh(G)
     lam_h = lam.detach().cpu().numpy()[::-1][:r]
+    if se is not None:
+        se.check()
     if not np.isfinite(lam_h).all():
         raise MHError(6, "split_rho: the eigensolver did not converge")
     lam_max = max(float(lam_h[0]), 0.0)
@@ -132,7 +148,11 @@ def split_rho(x, trunc, absorb, normalize=True):
         return split_svd(x, trunc, absorb, normalize)
     k, _ = trunc.truncate(np.sqrt(lam_h))
     k = max(1, min(k, int(np.count_nonzero(lam_h > 0)) or 1))
-    Q = V[:, n - k:].flip(-1)
+    if se is not None:
+        V = se.vectors([k])[0]            # n x k, ascending
+        Q = V.flip(-1)
+    else:
+        Q = V[:, n - k:].flip(-1)
     if left:
         B = Q.conj().t() @ M                      # s v, rows of norm s
         s = torch.linalg.vector_norm(B, dim=1)

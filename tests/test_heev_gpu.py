@@ -129,6 +129,40 @@ def test_split_rho_matches_library_path(cplx, absorb):
     np.testing.assert_allclose(out[True][0].numpy(), out[False][0].numpy(), rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("absorb", ["left", "right"])
+@pytest.mark.parametrize("D", [8, 24])
+def test_split_rho_batched_solver_matches_library_path(absorb, D):
+    """Real density matrices above the one-launch solver's order go to the
+    batched eigensolver (SectorEigh, kept columns back-transformed only):
+    the split equals the library eigh split (kept values, truncated product)
+    where D cuts a graded spectrum.  (Kept vectors whose eigenvalues sit near
+    the noise floor are fixed only to eps64 lambda_max / gap by either
+    solver, so the cut is placed where the gaps are wide: there both agree
+    to rounding.)"""
+    from finitemptools_amd import truncation
+    print(f"split_rho batched vs library absorb={absorb} D={D}", flush=True)
+    rng = np.random.default_rng(21)
+    Dl, d, Dr = 20, 2, 24                      # orders 40 (left) and 48 (right)
+    r = min(Dl * d, d * Dr)
+    Q1 = np.linalg.qr(rng.standard_normal((Dl * d, r)))[0]
+    Q2 = np.linalg.qr(rng.standard_normal((d * Dr, r)))[0]
+    M = (Q1 * np.logspace(0, -6, r)) @ Q2.T
+    x = torch.as_tensor(M.reshape(Dl, d, d, Dr), device=DEV)
+    tr = truncation.MPSTruncation(D=D, eps=1e-10)
+    out = {}
+    for flag in (True, False):
+        truncation.SECTOR_EIGH = flag
+        try:
+            A1, A2, s = truncation.split_rho(x, tr, absorb)
+        finally:
+            truncation.SECTOR_EIGH = True
+        P = torch.einsum("asc,ctb->astb", A1.cpu(), A2.cpu())
+        out[flag] = (P, s.cpu())
+    assert len(out[True][1]) == len(out[False][1])
+    np.testing.assert_allclose(out[True][1].numpy(), out[False][1].numpy(), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(out[True][0].numpy(), out[False][0].numpy(), rtol=0, atol=1e-12)
+
+
 def test_graded_rank_deficient_gram():
     """Gram matrices of rank r < n with singular values spread over up to 12
     decades (the reduced density matrices of a converged state): every solve
