@@ -34,6 +34,10 @@ HOST_SVD_MAX = 256
 # (mh_svd_small: one launch, one-sided Jacobi); False sends them to the
 # branches below.
 DEVICE_SVD = True
+# Complex density-matrix splits of order above SMALL_EIGH_ORDER whose
+# matrix fits mh_svd_small take the SVD split directly instead of a
+# library eigh of rho.
+COMPLEX_SVD_SPLIT = True
 
 # A density-matrix eigenvalue of an order-n matrix carries an absolute error
 # of about n * eps64 * lambda_max (eps64 = 2^-52); singular values taken as
@@ -120,6 +124,11 @@ def split_rho(x, trunc, absorb, normalize=True):
     r = min(M.shape)
     n = M.shape[0] if left else M.shape[1]
     se = None
+    if (COMPLEX_SVD_SPLIT and DEVICE_SVD and x.is_cuda and M.dtype == torch.complex128
+            and n > SMALL_EIGH_ORDER and eigen.svd_small_fits(M.shape, M.dtype, x.device)):
+        # complex orders past the one-launch eigensolver: the one-launch SVD
+        # split (the reference's own stable_tsvd) rather than a library eigh
+        return split_svd(x, trunc, absorb, normalize)
     if (SMALL_EIGH and x.is_cuda and n <= SMALL_EIGH_ORDER
             and n <= eigen.small_eigh_max(M.dtype, x.device)):
         # one-launch solver; it takes G column-major: G^T = conj(M) M^T (left)
