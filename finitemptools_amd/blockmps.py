@@ -554,12 +554,16 @@ def _eig_workers(dev, n):
 
 
 def shutdown_workers():
-    """Join the eigensolve worker threads and drop their side streams."""
+    """Join the eigensolve worker threads and drop their side streams, and
+    join the plan-prefetch thread."""
     for pool, streams in list(_EIG_POOL.values()):
         pool.shutdown(wait=True)
         for s in streams:
             s.synchronize()
     _EIG_POOL.clear()
+    for pool in _POOL:
+        pool.shutdown(wait=True)
+    _POOL.clear()
 
 
 EIG_THREADS = 4

@@ -462,7 +462,13 @@ class Plan:
     def __del__(self):
         try:
             if getattr(self, "ptr", None) is not None and self.ptr.value:
-                self.lib.mh_bs_plan_destroy(self.h, self.ptr)
+                # the engine's live handle: once the engine is closed (at
+                # exit, before interpreter teardown collects what is left)
+                # its handle is freed and must not be passed on; the process
+                # returns the tables' memory then
+                h = getattr(self.eng, "h", None)
+                if h is not None:
+                    self.lib.mh_bs_plan_destroy(h, self.ptr)
                 self.ptr = None
         except Exception:
             pass

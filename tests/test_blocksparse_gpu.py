@@ -461,3 +461,34 @@ def test_d2048_bond_plan_matches_per_sector_oracle(gpu, site):
                       dict(xl.lspace.sectors), dict(xl.rspace.sectors))
     ref = OB.flat_from_blocks(yb, xe, xl.size)
     assert rel(got, ref) < 1e-12
+
+
+def test_process_exits_cleanly_with_plans_alive(gpu):
+    """Plans still referenced at interpreter exit (a module-global cache
+    whose last bond prefetched its plans) are collected after the engines'
+    atexit teardown; they must not pass the freed handle on (that aborted
+    the process at exit): a child process that ends this way exits 0."""
+    import subprocess
+    import sys
+    import textwrap
+    code = textwrap.dedent("""
+        from finitemptools_amd import blockmps as bm, models
+        from finitemptools_amd.dmrg import DMRG2
+        from finitemptools_amd.truncation import MPSTruncation
+        # the library eigensolver: the parent process (this test run) holds
+        # the device's cooperative-launch state, which the child cannot take
+        bm.EIG_SOLVER = "library"
+        L = 6
+        phys = bm.hubbard_charges()
+        psi = bm.prodmps(phys, [phys.index((0, 0)), phys.index((1, 1))] * (L // 2))
+        env = bm.BlockExpectationCache(bm.BlockMPO(models.hubbard_mpo(L, 1.0, 4.0), phys), psi)
+        e, _ = bm.sweep(env, DMRG2(trunc=MPSTruncation(D=16, eps=1e-10)))
+        assert env._pre, "the sweep left prefetched plans behind"
+        print("swept", e[-1], flush=True)
+    """)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("child process: block sweep with a module-global cache, then exit", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    assert "swept" in r.stdout
