@@ -635,7 +635,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="leave out the plain per-sector FP64 sweep (about a minute)")
     ap.add_argument("--no-tdvp", action="store_true")
-    ap.add_argument("--only", choices=("all", "headline", "blocksparse", "complex", "dense"),
+    ap.add_argument("--only", choices=("all", "headline", "blocksparse", "complex", "dense", "tdvp"),
                     default="all",
                     help="run one section only (profiling): the D=2048 middle-bond matvec, "
                          "the block-sparse ramp and sweeps, the complex or the dense D=512 matvec")
@@ -671,6 +671,9 @@ def main():
         return
     if args.only == "complex":
         print(json.dumps({"ac2_prime_D512_c64_tfim": complex_tfim_matvec()}))
+        return
+    if args.only == "tdvp":
+        print(json.dumps({"tdvp_tfim_L100_quench": tdvp_tfim_quench(precs=("f64",))}))
         return
 
     import ctypes
