@@ -119,10 +119,12 @@ def test_split_rho_matches_library_path(cplx, absorb):
     out = {}
     for flag in (True, False):
         truncation.SMALL_EIGH = flag
+        truncation.COMPLEX_SVD_SPLIT = False     # the density-matrix path itself
         try:
             A1, A2, s = truncation.split_rho(x, tr, absorb)
         finally:
             truncation.SMALL_EIGH = True
+            truncation.COMPLEX_SVD_SPLIT = True
         P = torch.einsum("asc,ctb->astb", A1.cpu(), A2.cpu())
         out[flag] = (P, s.cpu())
     np.testing.assert_allclose(out[True][1].numpy(), out[False][1].numpy(), rtol=0, atol=1e-12)
